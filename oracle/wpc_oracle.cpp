@@ -1,0 +1,1146 @@
+// CPU restatement of the warp-preconditioners hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// This file is the oracle: a serial, literal restatement of the reference's Warp
+// kernels, used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// as the checker.  The product path never links, loads or calls it.
+//
+// Every routine cites the reference file:line it restates (paths are relative to
+// the reference repository root).  Arithmetic follows the reference's operation
+// order literally and is compiled with -ffp-contract=off, so no multiply-add is
+// fused; sqrt and division are IEEE correctly rounded.  float16 arithmetic is
+// emulated as "compute in float, round each result to binary16 (RNE)", which is
+// exactly IEEE half arithmetic for + - * / sqrt because float carries
+// 24 >= 2*11+2 significand bits.
+//
+// Pinning (which checks hold each routine; the tests live in tests/):
+//   fsai_diagonal/fsai_build  -- dense-inverse identity on full patterns, per-row
+//       principal-submatrix solves, an independent numpy greedy-search
+//       restatement of the adaptive selection (tests/test_oracle_fsai.py), the
+//       Jacobi special case W=1, narrow-frontier counts, pivot truncation.
+//   sort_rows/pack/transpose -- structural invariants (sorted, diagonal last/first,
+//       G^T equals the dense transpose).
+//   refit               -- per-row principal-submatrix solves with e_last; full
+//       support refit equals A^-1.
+//   csr_spmv/fsai_apply -- scipy products and dense G^T G.
+//   sqlap               -- scipy (L diag(1/m) L + w M)[f,f] and the Dirichlet RHS.
+//   block jacobi/ilu0   -- dense block inverses, a numpy ILU(0) restatement,
+//       exact LU after enough sweeps.
+//   pcg/pcr             -- converge to scipy spsolve; iteration counts against
+//       the reference's recorded grid32/grid64 results (25-iteration buckets).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------- half emulation
+static float round_to_half(double x) {
+    // Round a double to the nearest binary16 value (ties to even), returned as float.
+    if (std::isnan(x)) return std::numeric_limits<float>::quiet_NaN();
+    double ax = std::fabs(x);
+    if (ax == 0.0) return (float)x;
+    if (ax >= 65520.0) return std::copysign(std::numeric_limits<float>::infinity(), x);
+    int e;
+    std::frexp(ax, &e);            // ax = m * 2^e, m in [0.5, 1)
+    int exp = e - 1;               // ax in [2^exp, 2^(exp+1))
+    if (exp < -14) exp = -14;      // subnormal range shares the quantum 2^-24
+    double quantum = std::ldexp(1.0, exp - 10);
+    double q = ax / quantum;       // exact (power-of-two scaling)
+    double r = std::nearbyint(q);  // default rounding mode: ties to even
+    double y = r * quantum;
+    return (float)std::copysign(y, x);
+}
+
+struct H {
+    float v;
+    H() : v(0.0f) {}
+    explicit H(double x) : v(round_to_half(x)) {}
+    static H raw(float f) { H h; h.v = f; return h; }
+    explicit operator double() const { return v; }
+};
+static inline H operator+(H a, H b) { return H((double)(a.v + b.v)); }
+static inline H operator-(H a, H b) { return H((double)(a.v - b.v)); }
+static inline H operator*(H a, H b) { return H((double)(a.v * b.v)); }
+static inline H operator/(H a, H b) { return H((double)(a.v / b.v)); }
+static inline H operator-(H a) { return H::raw(-a.v); }
+static inline H& operator+=(H& a, H b) { a = a + b; return a; }
+static inline H& operator-=(H& a, H b) { a = a - b; return a; }
+static inline H& operator*=(H& a, H b) { a = a * b; return a; }
+static inline bool operator>(H a, H b) { return a.v > b.v; }
+static inline bool operator<(H a, H b) { return a.v < b.v; }
+static inline bool operator<=(H a, H b) { return a.v <= b.v; }
+static inline bool operator==(H a, H b) { return a.v == b.v; }
+static inline bool operator!=(H a, H b) { return a.v != b.v; }
+static inline H h_sqrt(H a) { return H((double)std::sqrt(a.v)); }
+static inline H h_abs(H a) { return H::raw(std::fabs(a.v)); }
+static inline bool h_finite(H a) { return std::isfinite(a.v); }
+
+static inline double nsqrt(double a) { return std::sqrt(a); }
+static inline float nsqrt(float a) { return std::sqrt(a); }
+static inline H nsqrt(H a) { return h_sqrt(a); }
+static inline double nabs(double a) { return std::fabs(a); }
+static inline float nabs(float a) { return std::fabs(a); }
+static inline H nabs(H a) { return h_abs(a); }
+static inline bool nfinite(double a) { return std::isfinite(a); }
+static inline bool nfinite(float a) { return std::isfinite(a); }
+static inline bool nfinite(H a) { return h_finite(a); }
+static inline double todouble(double a) { return a; }
+static inline double todouble(float a) { return a; }
+static inline double todouble(H a) { return a.v; }
+
+// Storage <-> arithmetic.  float16 is stored as IEEE binary16 bits.
+static inline float half_bits_to_float(uint16_t h) {
+    uint32_t sign = (uint32_t)(h >> 15) << 31;
+    uint32_t exp = (h >> 10) & 0x1f;
+    uint32_t man = h & 0x3ff;
+    float f;
+    if (exp == 0) {
+        f = std::ldexp((float)man, -24);
+        return sign ? -f : f;
+    }
+    if (exp == 31) {
+        uint32_t bits = sign | 0x7f800000u | (man << 13);
+        std::memcpy(&f, &bits, 4);
+        return f;
+    }
+    uint32_t bits = sign | ((exp + 112) << 23) | (man << 13);
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+static inline uint16_t float_to_half_bits(float f) {
+    // f is assumed already representable in binary16 (callers round first).
+    float r = round_to_half(f);
+    uint32_t bits;
+    std::memcpy(&bits, &r, 4);
+    uint16_t sign = (uint16_t)((bits >> 31) << 15);
+    if (std::isnan(r)) return sign | 0x7e00;
+    if (std::isinf(r)) return sign | 0x7c00;
+    float a = std::fabs(r);
+    if (a == 0.0f) return sign;
+    if (a < 6.103515625e-05f) {  // subnormal
+        return sign | (uint16_t)std::lround(a * 16777216.0f);
+    }
+    int32_t exp = (int32_t)((bits >> 23) & 0xff) - 127 + 15;
+    uint16_t man = (uint16_t)((bits >> 13) & 0x3ff);
+    return sign | (uint16_t)(exp << 10) | man;
+}
+
+template <class T> struct Store;
+template <> struct Store<double> {
+    static double load(const void* p, long i) { return ((const double*)p)[i]; }
+    static void store(void* p, long i, double v) { ((double*)p)[i] = v; }
+};
+template <> struct Store<float> {
+    static float load(const void* p, long i) { return ((const float*)p)[i]; }
+    static void store(void* p, long i, float v) { ((float*)p)[i] = v; }
+};
+template <> struct Store<H> {
+    static H load(const void* p, long i) { return H::raw(half_bits_to_float(((const uint16_t*)p)[i])); }
+    static void store(void* p, long i, H v) { ((uint16_t*)p)[i] = float_to_half_bits(v.v); }
+};
+
+enum { DT_F16 = 1, DT_BF16 = 2, DT_F32 = 3, DT_F64 = 4 };
+
+// bfloat16 storage (round to nearest even) -- used only by factor conversion.
+static inline float bf16_bits_to_float(uint16_t b) {
+    uint32_t bits = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+static inline uint16_t float_to_bf16_bits(float f) {
+    uint32_t bits;
+    std::memcpy(&bits, &f, 4);
+    if (std::isnan(f)) return (uint16_t)((bits >> 16) | 0x40);
+    uint32_t lsb = (bits >> 16) & 1u;
+    bits += 0x7fffu + lsb;
+    return (uint16_t)(bits >> 16);
+}
+
+static double load_any(int dt, const void* p, long i) {
+    switch (dt) {
+        case DT_F64: return ((const double*)p)[i];
+        case DT_F32: return ((const float*)p)[i];
+        case DT_F16: return half_bits_to_float(((const uint16_t*)p)[i]);
+        case DT_BF16: return bf16_bits_to_float(((const uint16_t*)p)[i]);
+    }
+    return NAN;
+}
+static void store_any(int dt, void* p, long i, double v) {
+    switch (dt) {
+        case DT_F64: ((double*)p)[i] = v; break;
+        case DT_F32: ((float*)p)[i] = (float)v; break;
+        case DT_F16: ((uint16_t*)p)[i] = float_to_half_bits(round_to_half(v)); break;
+        case DT_BF16: ((uint16_t*)p)[i] = float_to_bf16_bits((float)v); break;
+    }
+}
+
+// ------------------------------------------------------------------ FSAI build
+// fsai.py:20-41 -- binary search for A[i, j] in a sorted CSR row (zero if absent).
+template <class T>
+static T entry(const int* offsets, const int* columns, const void* values, int i, int j) {
+    int lo = offsets[i];
+    int hi = offsets[i + 1];
+    int end = hi;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (columns[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    T result = T(0);
+    if (lo < end) {
+        if (columns[lo] == j) result = Store<T>::load(values, lo);
+    }
+    return result;
+}
+
+// fsai.py:43-57 -- scale_i = 1/sqrt(A_ii); invalid diagonals counted in status[0].
+template <class T>
+static void fsai_diagonal(int n, const int* off, const int* col, const void* val, void* scale,
+                          int* status) {
+    for (int i = 0; i < n; ++i) {
+        T d = entry<T>(off, col, val, i, i);
+        if (d > T(0) && nfinite(d)) {
+            Store<T>::store(scale, i, T(1) / nsqrt(d));
+        } else {
+            Store<T>::store(scale, i, T(0));
+            status[0] += 1;
+        }
+    }
+}
+
+// fsai.py:59-176 -- per-row adaptive FSAI: greedy top-s frontier search, bordered
+// Cholesky, incremental forward solve, back-solve, kappa test, emit.
+template <class T>
+static void fsai_build_row(int i, const int* offsets, const int* columns, const void* values,
+                           const void* scale_p, T tolerance, T pivot_floor, int width, int step,
+                           int* cols_out, void* vals_out, int* status_local) {
+    std::vector<int> pattern(width, -1);
+    std::vector<T> chol((size_t)width * width, T(0));
+    std::vector<T> z(width, T(0)), v(width, T(0)), w(width, T(0));
+    std::vector<int> candidates(step);
+    std::vector<T> scores(step);
+    auto scale = [&](int k) { return Store<T>::load(scale_p, k); };
+    auto C = [&](int r, int c) -> T& { return chol[(size_t)r * width + c]; };
+    pattern[0] = i;
+    C(0, 0) = T(1);
+    z[0] = T(1);
+    v[0] = T(1);
+    int size = 1;
+    while (size < width) {
+        for (int q = 0; q < step; ++q) { candidates[q] = -1; scores[q] = T(0); }
+        for (int p = 0; p < size; ++p) {
+            int row = pattern[p];
+            for (int e = offsets[row]; e < offsets[row + 1]; ++e) {
+                int c = columns[e];
+                bool selected = c >= i;
+                for (int q = 0; q < size; ++q)
+                    if (pattern[q] == c) selected = true;
+                for (int q = 0; q < step; ++q)
+                    if (candidates[q] == c) selected = true;
+                if (!selected) {
+                    T residual = T(0);
+                    for (int q = 0; q < size; ++q) {
+                        int j = pattern[q];
+                        residual += entry<T>(offsets, columns, values, c, j) * scale(c) * scale(j) * z[q];
+                    }
+                    T score = nabs(residual);
+                    for (int rank = 0; rank < step; ++rank) {
+                        if (score > scores[rank] || (score == scores[rank] && c < candidates[rank])) {
+                            for (int rev = 0; rev < step - 1; ++rev) {
+                                int dest = step - 1 - rev;
+                                if (dest > rank) {
+                                    candidates[dest] = candidates[dest - 1];
+                                    scores[dest] = scores[dest - 1];
+                                }
+                            }
+                            candidates[rank] = c;
+                            scores[rank] = score;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        if (candidates[0] < 0) break;
+        T old_z0 = z[0];
+        bool stopped = false;
+        for (int rank = 0; rank < step; ++rank) {
+            int best = candidates[rank];
+            if (best < 0 || size == width) break;
+            T pivot = T(1);
+            for (int p = 0; p < size; ++p) {
+                T a = entry<T>(offsets, columns, values, best, pattern[p]) * scale(best) * scale(pattern[p]);
+                for (int q = 0; q < p; ++q) a -= C(p, q) * w[q];
+                w[p] = a / C(p, p);
+                pivot -= w[p] * w[p];
+            }
+            if (pivot <= pivot_floor || !nfinite(pivot)) {
+                status_local[1] += 1;
+                stopped = true;
+                break;
+            }
+            pattern[size] = best;
+            for (int p = 0; p < size; ++p) C(size, p) = w[p];
+            C(size, size) = nsqrt(pivot);
+            T rhs = T(0);
+            for (int p = 0; p < size; ++p) rhs -= w[p] * v[p];
+            v[size] = rhs / C(size, size);
+            size += 1;
+        }
+        for (int rev = 0; rev < size; ++rev) {
+            int p = size - 1 - rev;
+            T a = v[p];
+            for (int q = p + 1; q < size; ++q) a -= C(q, p) * z[q];
+            z[p] = a / C(p, p);
+        }
+        if (stopped || (tolerance > T(0) && T(1) - old_z0 / z[0] <= tolerance)) break;
+    }
+    T norm = nsqrt(z[0]);
+    bool valid = true;
+    for (int p = 0; p < width; ++p) {
+        long e = (long)i * width + p;
+        cols_out[e] = -1;
+        Store<T>::store(vals_out, e, T(0));
+        if (p < size) {
+            cols_out[e] = pattern[p];
+            T value = z[p] / norm * scale(pattern[p]);
+            Store<T>::store(vals_out, e, value);
+            if (!nfinite(value)) valid = false;
+            if (p == 0 && value <= T(0)) valid = false;
+        }
+    }
+    if (!valid) status_local[2] += 1;
+}
+
+template <class T>
+static void fsai_build(int n, const int* off, const int* col, const void* val, const void* scale,
+                       double tolerance, double pivot_floor, int width, int step, int* cols_out,
+                       void* vals_out, int* status, int nthreads) {
+    int s1 = 0, s2 = 0;
+    T tol = T(tolerance), floor = T(pivot_floor);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(+ : s1, s2) if (nthreads > 1)
+    for (int i = 0; i < n; ++i) {
+        int local[3] = {0, 0, 0};
+        fsai_build_row<T>(i, off, col, val, scale, tol, floor, width, step, cols_out, vals_out, local);
+        s1 += local[1];
+        s2 += local[2];
+    }
+    status[1] += s1;
+    status[2] += s2;
+}
+
+// fsai_pack.py:15-39 -- per-row insertion sort by column, dropping -1 slots.
+template <class T>
+static void sort_rows(int n, int width, int* columns, void* values, int* counts) {
+    std::vector<int> cols(width);
+    std::vector<T> vals(width);
+    for (int i = 0; i < n; ++i) {
+        int size = 0;
+        for (int p = 0; p < width; ++p) {
+            int c = columns[(long)i * width + p];
+            if (c >= 0) {
+                T v = Store<T>::load(values, (long)i * width + p);
+                int q = size;
+                while (q > 0) {
+                    if (cols[q - 1] <= c) break;
+                    cols[q] = cols[q - 1];
+                    vals[q] = vals[q - 1];
+                    q -= 1;
+                }
+                cols[q] = c;
+                vals[q] = v;
+                size += 1;
+            }
+        }
+        counts[i + 1] = size;
+        for (int p = 0; p < size; ++p) {
+            columns[(long)i * width + p] = cols[p];
+            Store<T>::store(values, (long)i * width + p, vals[p]);
+        }
+    }
+}
+
+// fsai_pack.py:60-73 -- compact padded rows into CSR after the offsets scan.
+template <class T>
+static void pack(int n, int width, const int* offsets, const int* columns, const void* values,
+                 int* outcols, void* outvals) {
+    for (int i = 0; i < n; ++i)
+        for (int e = offsets[i]; e < offsets[i + 1]; ++e) {
+            long p = (long)i * width + e - offsets[i];
+            outcols[e] = columns[p];
+            Store<T>::store(outvals, e, Store<T>::load(values, p));
+        }
+}
+
+// fsai_pack.py:93-125 -- BSR b x b to scalar CSR, row r*b+c, keeping stored zeros.
+template <class T>
+static void scalarize(int nb, int b, const int* offsets, const int* columns, const void* values,
+                      int* outoff, int* outcol, void* outval) {
+    int n = nb * b;
+    for (int i = 0; i < n; ++i) {
+        int r = i / b;
+        int component = i % b;
+        int start = offsets[r];
+        int count = offsets[r + 1] - start;
+        long base = (long)start * b * b + (long)component * count * b;
+        outoff[i] = (int)base;
+        for (int p = 0; p < count; ++p)
+            for (int c = 0; c < b; ++c) {
+                long dest = base + (long)p * b + c;
+                outcol[dest] = columns[start + p] * b + c;
+                Store<T>::store(outval, dest,
+                                Store<T>::load(values, (long)(start + p) * b * b + component * b + c));
+            }
+        if (i == n - 1) outoff[i + 1] = (int)(base + (long)count * b);
+    }
+    if (n == 0) outoff[0] = 0;
+}
+
+// Warp bsr_transposed (fsai.py:337) -- CSR transpose with ascending columns.  Values
+// are moved as raw elements of `elem` bytes.
+static void transpose(int nrow, int ncol, const int* off, const int* col, const void* val,
+                      int elem, int* toff, int* tcol, void* tval) {
+    std::vector<int> count(ncol + 1, 0);
+    int nnz = off[nrow];
+    for (int e = 0; e < nnz; ++e) count[col[e] + 1] += 1;
+    for (int j = 0; j < ncol; ++j) count[j + 1] += count[j];
+    for (int j = 0; j <= ncol; ++j) toff[j] = count[j];
+    std::vector<int> cursor(count.begin(), count.end() - 1);
+    for (int i = 0; i < nrow; ++i)  // rows visited ascending -> each transposed row sorted
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            int p = cursor[col[e]]++;
+            tcol[p] = i;
+            std::memcpy((char*)tval + (long)p * elem, (const char*)val + (long)e * elem, elem);
+        }
+}
+
+// fsai.py:181-196 -- validate a compressed factor (finite values, positive diagonal).
+template <class T>
+static void validate_factor(int n, const int* off, const int* col, const void* val, int* status) {
+    for (int row = 0; row < n; ++row) {
+        bool invalid = false;
+        for (int e = off[row]; e < off[row + 1]; ++e) {
+            T v = Store<T>::load(val, e);
+            if (!nfinite(v)) invalid = true;
+            if (col[e] == row && v <= T(0)) invalid = true;
+        }
+        if (invalid) status[0] += 1;
+    }
+}
+
+// ------------------------------------------------------------------ refit
+// fsai_reuse.py:17-32 -- position of (i, j) in a sorted CSR row, or -1.
+static int find(const int* offsets, const int* columns, int i, int j) {
+    int lo = offsets[i], hi = offsets[i + 1], end = hi;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (columns[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    int result = -1;
+    if (lo < end && columns[lo] == j) result = lo;
+    return result;
+}
+
+// fsai_reuse.py:76-134 -- per-row dense Cholesky on the stored support (diagonal
+// last), z = A[S,S]^-1 e_last, G[i,S] = z / sqrt(z_last) * scale[S].
+template <class T, class S>
+static void refit(int n, const int* offsets, const int* columns, const void* values, const int* goff,
+                  const int* gcol, const void* scale_p, double floor_d, void* out, int* bad) {
+    T floor = T(floor_d);
+    auto scale = [&](int k) { return Store<T>::load(scale_p, k); };
+    for (int i = 0; i < n; ++i) {
+        int start = goff[i];
+        int size = goff[i + 1] - start;
+        std::vector<T> chol((size_t)size * size, T(0));
+        auto C = [&](int r, int c) -> T& { return chol[(size_t)r * size + c]; };
+        bool valid = true;
+        for (int p = 0; p < size; ++p) {
+            int r = gcol[start + p];
+            for (int q = 0; q < p + 1; ++q) {
+                int c = gcol[start + q];
+                int slot = find(offsets, columns, r, c);
+                T a = T(0);
+                if (slot >= 0) a = Store<T>::load(values, slot) * scale(r) * scale(c);
+                for (int k = 0; k < q; ++k) a -= C(p, k) * C(q, k);
+                if (p == q) {
+                    if (a <= floor || !nfinite(a)) {
+                        valid = false;
+                        a = T(1);
+                    }
+                    C(p, q) = nsqrt(a);
+                } else {
+                    C(p, q) = a / C(q, q);
+                }
+            }
+        }
+        std::vector<T> z(size, T(0));
+        for (int p = 0; p < size; ++p) {
+            T a = T(0);
+            if (gcol[start + p] == i) a = T(1);
+            for (int q = 0; q < p; ++q) a -= C(p, q) * z[q];
+            z[p] = a / C(p, p);
+        }
+        for (int rev = 0; rev < size; ++rev) {
+            int p = size - 1 - rev;
+            T a = z[p];
+            for (int q = p + 1; q < size; ++q) a -= C(q, p) * z[q];
+            z[p] = a / C(p, p);
+        }
+        if (size > 0) {
+            T norm = nsqrt(z[size - 1]);
+            for (int p = 0; p < size; ++p) {
+                T value_t = z[p] / norm * scale(gcol[start + p]);
+                S value = S(todouble(value_t));
+                Store<S>::store(out, start + p, value);
+                if (!nfinite(value)) valid = false;
+                if (p == size - 1 && value <= S(0)) valid = false;
+            }
+        }
+        if (!valid) bad[0] += 1;
+    }
+}
+
+// ------------------------------------------------------------------ products
+// sparse_operator.py:64-87 -- z = alpha*A*x + beta*y, accumulating in the vector type;
+// alpha == 0 never reads A or x, beta == 0 never reads y.
+template <class V>
+static void csr_spmv(int mat_dt, int n, const int* off, const int* col, const void* val,
+                     const void* x, const void* y, void* z, double alpha_d, double beta_d) {
+    V alpha = V(alpha_d), beta = V(beta_d);
+    std::vector<V> out(n);
+    for (int row = 0; row < n; ++row) {
+        V result = V(0);
+        if (alpha != V(0)) {
+            for (int e = off[row]; e < off[row + 1]; ++e)
+                result += V(load_any(mat_dt, val, e)) * Store<V>::load(x, col[e]);
+            result *= alpha;
+        }
+        if (beta != V(0)) result += beta * Store<V>::load(y, row);
+        out[row] = result;
+    }
+    for (int row = 0; row < n; ++row) Store<V>::store(z, row, out[row]);
+}
+
+// squared_laplacian.py:28-35 and :317-377 -- factored (L M^-1 L + w M)[f,f] product.
+template <class T>
+static void sqlap_apply(int n_full, const int* off, const int* cols, const void* vals,
+                        const void* inverse, const int* index, int n_free, const int* free,
+                        const void* mass, const void* x, double mass_weight_d, void* tmp,
+                        const void* y, void* z, double alpha_d, double beta_d, int boundary) {
+    for (int row = 0; row < n_full; ++row) {
+        T acc = T(0);
+        for (int e = off[row]; e < off[row + 1]; ++e) {
+            int c0 = cols[e];
+            int c = index[c0];
+            if (boundary) {
+                if (c < 0) acc += Store<T>::load(vals, e) * Store<T>::load(x, c0);
+            } else {
+                if (c >= 0) acc += Store<T>::load(vals, e) * Store<T>::load(x, c);
+            }
+        }
+        Store<T>::store(tmp, row, Store<T>::load(inverse, row) * acc);
+    }
+    T alpha = T(alpha_d), beta = T(beta_d), mass_weight = T(mass_weight_d);
+    std::vector<T> out(n_free);
+    for (int i = 0; i < n_free; ++i) {
+        T acc = T(0);
+        if (alpha != T(0)) {
+            int row = free[i];
+            for (int e = off[row]; e < off[row + 1]; ++e)
+                acc += Store<T>::load(vals, e) * Store<T>::load(tmp, cols[e]);
+        }
+        if (mass_weight != T(0) && alpha != T(0))
+            acc += mass_weight * Store<T>::load(mass, free[i]) * Store<T>::load(x, i);
+        acc *= alpha;
+        if (beta != T(0)) acc += beta * Store<T>::load(y, i);
+        out[i] = acc;
+    }
+    for (int i = 0; i < n_free; ++i) Store<T>::store(z, i, out[i]);
+}
+
+// ------------------------------------------------------------------ blocks
+// block_ilu.py:16-43 -- max-abs-scaled determinant test, inverse(A/s)/s.
+template <class T>
+static T det_n(const T* a, int b) {
+    if (b == 1) return a[0];
+    if (b == 2) return a[0] * a[3] - a[1] * a[2];
+    if (b == 3)
+        return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+               a[2] * (a[3] * a[7] - a[4] * a[6]);
+    // 4x4: cofactor expansion along the first row.
+    T total = T(0);
+    for (int c = 0; c < 4; ++c) {
+        T m[9];
+        int k = 0;
+        for (int r = 1; r < 4; ++r)
+            for (int cc = 0; cc < 4; ++cc)
+                if (cc != c) m[k++] = a[r * 4 + cc];
+        T d = det_n<T>(m, 3);
+        total += (c % 2 ? -a[c] : a[c]) * d;
+    }
+    return total;
+}
+template <class T>
+static void inverse_n(const T* a, int b, T* out) {
+    // Adjugate / determinant.
+    T d = det_n<T>(a, b);
+    if (b == 1) { out[0] = T(1) / a[0]; return; }
+    for (int r = 0; r < b; ++r)
+        for (int c = 0; c < b; ++c) {
+            T m[9];
+            int k = 0;
+            for (int rr = 0; rr < b; ++rr)
+                for (int cc = 0; cc < b; ++cc)
+                    if (rr != r && cc != c) m[k++] = a[rr * b + cc];
+            T cof = det_n<T>(m, b - 1);
+            if ((r + c) % 2) cof = -cof;
+            out[c * b + r] = cof / d;  // transpose of the cofactor matrix
+        }
+}
+template <class T>
+static void safe_inverse(const T* a, int b, T tolerance, T* inverse, int* bad) {
+    T scale = T(0);
+    for (int i = 0; i < b * b; ++i) scale = std::max(scale, nabs(a[i]));
+    for (int i = 0; i < b * b; ++i) inverse[i] = T(0);
+    if (scale > T(0) && nfinite(scale)) {
+        T normalized[16];
+        for (int i = 0; i < b * b; ++i) normalized[i] = a[i] / scale;
+        T det = b == 1 ? normalized[0] : det_n<T>(normalized, b);
+        if (nfinite(det) && nabs(det) > tolerance) {
+            if (b == 1) inverse[0] = T(1) / a[0];
+            else {
+                inverse_n<T>(normalized, b, inverse);
+                for (int i = 0; i < b * b; ++i) inverse[i] = inverse[i] / scale;
+            }
+        } else {
+            bad[0] += 1;
+        }
+    } else {
+        bad[0] += 1;
+    }
+    for (int i = 0; i < b * b; ++i)
+        if (!nfinite(inverse[i])) bad[0] += 1;
+}
+template <class T>
+static void matmul(const T* a, const T* b, int n, T* out) {
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) {
+            T s = T(0);
+            for (int k = 0; k < n; ++k) s += a[r * n + k] * b[k * n + c];
+            out[r * n + c] = s;
+        }
+}
+template <class T>
+static void matvec(const T* a, const T* x, int n, T* out) {
+    for (int r = 0; r < n; ++r) {
+        T s = T(0);
+        for (int k = 0; k < n; ++k) s += a[r * n + k] * x[k];
+        out[r] = s;
+    }
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+#define DISPATCH_T(dt, CALL)                                   \
+    do {                                                       \
+        if ((dt) == DT_F64) { using T = double; CALL; }        \
+        else if ((dt) == DT_F32) { using T = float; CALL; }    \
+        else if ((dt) == DT_F16) { using T = H; CALL; }        \
+        else return -1;                                        \
+    } while (0)
+
+int orc_fsai_diagonal(int dt, int n, const int* off, const int* col, const void* val, void* scale,
+                      int* status) {
+    DISPATCH_T(dt, (fsai_diagonal<T>(n, off, col, val, scale, status)));
+    return 0;
+}
+
+int orc_fsai_build(int dt, int n, const int* off, const int* col, const void* val, const void* scale,
+                   double tolerance, double pivot_floor, int width, int step, int* cols_out,
+                   void* vals_out, int* status, int nthreads) {
+    DISPATCH_T(dt, (fsai_build<T>(n, off, col, val, scale, tolerance, pivot_floor, width, step,
+                                  cols_out, vals_out, status, nthreads)));
+    return 0;
+}
+
+int orc_fsai_sort_rows(int dt, int n, int width, int* columns, void* values, int* counts) {
+    DISPATCH_T(dt, (sort_rows<T>(n, width, columns, values, counts)));
+    return 0;
+}
+
+void orc_scan_inclusive(int n, const int* in, int* out) {
+    long s = 0;
+    for (int i = 0; i < n; ++i) { s += in[i]; out[i] = (int)s; }
+}
+
+int orc_fsai_pack(int dt, int n, int width, const int* offsets, const int* columns,
+                  const void* values, int* outcols, void* outvals) {
+    DISPATCH_T(dt, (pack<T>(n, width, offsets, columns, values, outcols, outvals)));
+    return 0;
+}
+
+int orc_bsr_scalarize(int dt, int nb, int b, const int* off, const int* col, const void* val,
+                      int* outoff, int* outcol, void* outval) {
+    DISPATCH_T(dt, (scalarize<T>(nb, b, off, col, val, outoff, outcol, outval)));
+    return 0;
+}
+
+int orc_csr_transpose(int elem, int nrow, int ncol, const int* off, const int* col,
+                      const void* val, int* toff, int* tcol, void* tval) {
+    transpose(nrow, ncol, off, col, val, elem, toff, tcol, tval);
+    return 0;
+}
+
+void orc_convert(int src_dt, int dst_dt, long n, const void* src, void* dst) {
+    for (long i = 0; i < n; ++i) store_any(dst_dt, dst, i, load_any(src_dt, src, i));
+}
+
+int orc_validate_factor(int dt, int n, const int* off, const int* col, const void* val,
+                        int* status) {
+    DISPATCH_T(dt, (validate_factor<T>(n, off, col, val, status)));
+    return 0;
+}
+
+// fsai_reuse.py:35-48
+void orc_refit_source_map(int n, const int* off, const int* col, const int* src_off,
+                          const int* src_col, int b, int* mapping) {
+    for (int i = 0; i < n; ++i)
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            int j = col[e];
+            int slot = find(src_off, src_col, i / b, j / b);
+            mapping[e] = slot * b * b + (i % b) * b + j % b;
+        }
+}
+
+// fsai_reuse.py:51-61
+void orc_refit_transpose_map(int n, const int* off, const int* col, const int* toff,
+                             const int* tcol, int* mapping) {
+    for (int i = 0; i < n; ++i)
+        for (int e = off[i]; e < off[i + 1]; ++e) mapping[e] = find(toff, tcol, col[e], i);
+}
+
+// fsai_reuse.py:69-74 / 136-141 (element size in bytes selects the copy width)
+void orc_gather(int elem, long count, const void* src, const int* map, void* out) {
+    for (long e = 0; e < count; ++e)
+        std::memcpy((char*)out + e * elem, (const char*)src + (long)map[e] * elem, elem);
+}
+void orc_scatter(int elem, long count, const void* vals, const int* map, void* out) {
+    for (long e = 0; e < count; ++e)
+        std::memcpy((char*)out + (long)map[e] * elem, (const char*)vals + e * elem, elem);
+}
+
+int orc_fsai_refit(int dt, int storage, int n, const int* off, const int* col, const void* val,
+                   const int* goff, const int* gcol, const void* scale, double floor, void* out,
+                   int* bad) {
+    if (dt == DT_F64 && storage == DT_F64) refit<double, double>(n, off, col, val, goff, gcol, scale, floor, out, bad);
+    else if (dt == DT_F64 && storage == DT_F32) refit<double, float>(n, off, col, val, goff, gcol, scale, floor, out, bad);
+    else if (dt == DT_F32 && storage == DT_F32) refit<float, float>(n, off, col, val, goff, gcol, scale, floor, out, bad);
+    else if (dt == DT_F16 && storage == DT_F16) refit<H, H>(n, off, col, val, goff, gcol, scale, floor, out, bad);
+    else return -1;
+    return 0;
+}
+
+int orc_csr_spmv(int mat_dt, int vec_dt, int n, const int* off, const int* col, const void* val,
+                 const void* x, const void* y, void* z, double alpha, double beta) {
+    DISPATCH_T(vec_dt, (csr_spmv<T>(mat_dt, n, off, col, val, x, y, z, alpha, beta)));
+    return 0;
+}
+
+// fsai.py:374-381 -- z = alpha G^T G x + beta y through tmp; x consumed before z written.
+int orc_fsai_apply(int mat_dt, int vec_dt, int n, const int* goff, const int* gcol,
+                   const void* gval, const int* toff, const int* tcol, const void* tval,
+                   void* tmp, const void* x, const void* y, void* z, double alpha, double beta) {
+    if (alpha != 0.0) {
+        int rc = orc_csr_spmv(mat_dt, vec_dt, n, goff, gcol, gval, x, tmp, tmp, 1.0, 0.0);
+        if (rc) return rc;
+    }
+    return orc_csr_spmv(mat_dt, vec_dt, n, toff, tcol, tval, tmp, y, z, alpha, beta);
+}
+
+// squared_laplacian.py:11-20 -- inverse free map; status[0] range, status[1] duplicates.
+void orc_sqlap_index_free(int n_free, const int* free, int n, int* index, int* status) {
+    for (int i = 0; i < n; ++i) index[i] = -1;
+    for (int i = 0; i < n_free; ++i) {
+        int j = free[i];
+        if (j < 0 || j >= n) status[0] += 1;
+        else if (index[j] != -1) status[1] += 1;
+        else index[j] = i;
+    }
+}
+
+// squared_laplacian.py:306-314
+int orc_sqlap_mass_inverse(int dt, int n, const void* mass, void* inverse, int* status) {
+    DISPATCH_T(dt, ({
+        for (int i = 0; i < n; ++i) {
+            T m = Store<T>::load(mass, i);
+            if (m > T(0) && nfinite(m)) Store<T>::store(inverse, i, T(1) / m);
+            else status[2] += 1;
+        }
+    }));
+    return 0;
+}
+
+int orc_sqlap_apply(int dt, int n_full, const int* off, const int* cols, const void* vals,
+                    const void* inverse, const int* index, int n_free, const int* free,
+                    const void* mass, const void* x, double mass_weight, void* tmp, const void* y,
+                    void* z, double alpha, double beta, int boundary) {
+    DISPATCH_T(dt, (sqlap_apply<T>(n_full, off, cols, vals, inverse, index, n_free, free, mass, x,
+                                   mass_weight, tmp, y, z, alpha, beta, boundary)));
+    return 0;
+}
+
+// block_ilu.py:45-61
+void orc_block_topology(int nb, const int* off, const int* col, int* rows, int* diagonal, int* bad) {
+    for (int i = 0; i < nb; ++i) {
+        int found = -1;
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            rows[e] = i;
+            if (col[e] == i) found = e;
+        }
+        diagonal[i] = found;
+        if (found < 0) bad[0] += 1;
+    }
+}
+
+// block_ilu.py:63-73
+int orc_block_invert(int dt, int nb, int b, const void* values, const int* diagonal, void* inverse,
+                     double tolerance, int* bad) {
+    if (dt != DT_F64 && dt != DT_F32) return -1;
+    for (int i = 0; i < nb; ++i) {
+        if (diagonal[i] < 0) continue;
+        if (dt == DT_F64) {
+            const double* a = (const double*)values + (long)diagonal[i] * b * b;
+            safe_inverse<double>(a, b, tolerance, (double*)inverse + (long)i * b * b, bad);
+        } else {
+            const float* a = (const float*)values + (long)diagonal[i] * b * b;
+            safe_inverse<float>(a, b, (float)tolerance, (float*)inverse + (long)i * b * b, bad);
+        }
+    }
+    return 0;
+}
+
+// block_ilu.py:75-104 (compact storage only: counts == nullptr)
+int orc_block_diagonal_only(int dt, int nb, int b, const int* off, const int* col,
+                            const int* counts, const void* values, void* inverse,
+                            double tolerance, int* bad) {
+    for (int i = 0; i < nb; ++i) {
+        int lo = off[i];
+        int end = counts ? lo + counts[i] : off[i + 1];
+        int hi = end;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (col[mid] < i) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < end && col[lo] == i) {
+            if (dt == DT_F64)
+                safe_inverse<double>((const double*)values + (long)lo * b * b, b, tolerance,
+                                     (double*)inverse + (long)i * b * b, bad);
+            else
+                safe_inverse<float>((const float*)values + (long)lo * b * b, b, (float)tolerance,
+                                    (float*)inverse + (long)i * b * b, bad);
+        } else {
+            bad[0] += 1;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+template <class T>
+static void ilu_initialize(long nnzb, int b, const int* rows, const int* cols, const T* values,
+                           const T* inverse, T* factors) {
+    int bb = b * b;
+    for (long e = 0; e < nnzb; ++e) {
+        if (cols[e] < rows[e]) matmul<T>(values + e * bb, inverse + (long)cols[e] * bb, b, factors + e * bb);
+        else for (int k = 0; k < bb; ++k) factors[e * bb + k] = values[e * bb + k];
+    }
+}
+
+// block_ilu.py:120-150 -- one synchronous ILU(0) sweep.
+template <class T>
+static void ilu_sweep(long nnzb, int b, const int* off, const int* cols, const int* rows,
+                      const T* values, const T* old, const T* inverse, T* out, T relaxation) {
+    int bb = b * b;
+    for (long e = 0; e < nnzb; ++e) {
+        int i = rows[e], j = cols[e];
+        T value[16], prod[16], tmp[16];
+        for (int k = 0; k < bb; ++k) value[k] = values[e * bb + k];
+        for (int ik = off[i]; ik < off[i + 1]; ++ik) {
+            int k = cols[ik];
+            if (k >= std::min(i, j)) break;
+            int lo = off[k], hi = off[k + 1];
+            while (lo < hi) {
+                int mid = (lo + hi) / 2;
+                if (cols[mid] < j) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < off[k + 1] && cols[lo] == j) {
+                matmul<T>(old + (long)ik * bb, old + (long)lo * bb, b, prod);
+                for (int q = 0; q < bb; ++q) value[q] -= prod[q];
+            }
+        }
+        if (j < i) {
+            matmul<T>(value, inverse + (long)j * bb, b, tmp);
+            for (int q = 0; q < bb; ++q) value[q] = tmp[q];
+        }
+        for (int q = 0; q < bb; ++q)
+            out[e * bb + q] = (T(1) - relaxation) * old[e * bb + q] + relaxation * value[q];
+    }
+}
+
+// block_ilu.py:152-171 -- one Jacobi step of the unit-L (lower) or U solve.
+template <class T>
+static void ilu_triangular(int nb, int b, const int* off, const int* cols, const T* factors,
+                           const T* inverse, const T* rhs, const T* old, T* out, int lower) {
+    int bb = b * b;
+    for (int i = 0; i < nb; ++i) {
+        T value[4], prod[4], tmp[4];
+        for (int k = 0; k < b; ++k) value[k] = rhs[(long)i * b + k];
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            int j = cols[e];
+            if ((lower && j < i) || (!lower && j > i)) {
+                matvec<T>(factors + (long)e * bb, old + (long)j * b, b, prod);
+                for (int k = 0; k < b; ++k) value[k] -= prod[k];
+            }
+        }
+        if (!lower) {
+            matvec<T>(inverse + (long)i * bb, value, b, tmp);
+            for (int k = 0; k < b; ++k) value[k] = tmp[k];
+        }
+        for (int k = 0; k < b; ++k) out[(long)i * b + k] = value[k];
+    }
+}
+
+extern "C" {
+
+int orc_ilu_initialize(int dt, long nnzb, int b, const int* rows, const int* cols,
+                       const void* values, const void* inverse, void* factors) {
+    if (dt == DT_F64) ilu_initialize<double>(nnzb, b, rows, cols, (const double*)values, (const double*)inverse, (double*)factors);
+    else if (dt == DT_F32) ilu_initialize<float>(nnzb, b, rows, cols, (const float*)values, (const float*)inverse, (float*)factors);
+    else return -1;
+    return 0;
+}
+int orc_ilu_sweep(int dt, long nnzb, int b, const int* off, const int* cols, const int* rows,
+                  const void* values, const void* old, const void* inverse, void* out,
+                  double relaxation) {
+    if (dt == DT_F64) ilu_sweep<double>(nnzb, b, off, cols, rows, (const double*)values, (const double*)old, (const double*)inverse, (double*)out, relaxation);
+    else if (dt == DT_F32) ilu_sweep<float>(nnzb, b, off, cols, rows, (const float*)values, (const float*)old, (const float*)inverse, (float*)out, (float)relaxation);
+    else return -1;
+    return 0;
+}
+int orc_ilu_triangular(int dt, int nb, int b, const int* off, const int* cols, const void* factors,
+                       const void* inverse, const void* rhs, const void* old, void* out,
+                       int lower) {
+    if (dt == DT_F64) ilu_triangular<double>(nb, b, off, cols, (const double*)factors, (const double*)inverse, (const double*)rhs, (const double*)old, (double*)out, lower);
+    else if (dt == DT_F32) ilu_triangular<float>(nb, b, off, cols, (const float*)factors, (const float*)inverse, (const float*)rhs, (const float*)old, (float*)out, lower);
+    else return -1;
+    return 0;
+}
+
+// block_ilu.py:187-202 -- z_i = alpha D_i^-1 x_i + beta y_i.
+int orc_block_diagonal_apply(int dt, int nb, int b, const void* inverse, const void* x,
+                             const void* y, void* z, double alpha, double beta) {
+    if (dt != DT_F64 && dt != DT_F32) return -1;
+    int bb = b * b;
+    for (int i = 0; i < nb; ++i) {
+        double value[4] = {0, 0, 0, 0};
+        if (dt == DT_F64) {
+            const double* inv = (const double*)inverse + (long)i * bb;
+            double v[4] = {0, 0, 0, 0}, t[4];
+            if (alpha != 0.0) {
+                matvec<double>(inv, (const double*)x + (long)i * b, b, t);
+                for (int k = 0; k < b; ++k) v[k] = alpha * t[k];
+            }
+            if (beta != 0.0) for (int k = 0; k < b; ++k) v[k] += beta * ((const double*)y)[(long)i * b + k];
+            for (int k = 0; k < b; ++k) value[k] = v[k];
+            for (int k = 0; k < b; ++k) ((double*)z)[(long)i * b + k] = value[k];
+        } else {
+            const float* inv = (const float*)inverse + (long)i * bb;
+            float v[4] = {0, 0, 0, 0}, t[4];
+            float a = (float)alpha, be = (float)beta;
+            if (a != 0.0f) {
+                matvec<float>(inv, (const float*)x + (long)i * b, b, t);
+                for (int k = 0; k < b; ++k) v[k] = a * t[k];
+            }
+            if (be != 0.0f) for (int k = 0; k < b; ++k) v[k] += be * ((const float*)y)[(long)i * b + k];
+            for (int k = 0; k < b; ++k) ((float*)z)[(long)i * b + k] = v[k];
+        }
+    }
+    return 0;
+}
+
+// Warp bsr_mv (sparse_operator.py:104) for b x b blocks: z = alpha A x + beta y.
+int orc_bsr_spmv(int dt, int nb, int b, const int* off, const int* col, const void* val,
+                 const void* x, const void* y, void* z, double alpha, double beta) {
+    int bb = b * b;
+    std::vector<double> out((size_t)nb * b);
+    for (int i = 0; i < nb; ++i)
+        for (int r = 0; r < b; ++r) {
+            double s = 0.0;
+            if (alpha != 0.0) {
+                for (int e = off[i]; e < off[i + 1]; ++e)
+                    for (int c = 0; c < b; ++c)
+                        s += load_any(dt, val, (long)e * bb + r * b + c) *
+                             load_any(dt, x, (long)col[e] * b + c);
+                s *= alpha;
+            }
+            if (beta != 0.0) s += beta * load_any(dt, y, (long)i * b + r);
+            out[(size_t)i * b + r] = s;
+        }
+    for (long k = 0; k < (long)nb * b; ++k) store_any(dt, z, k, out[k]);
+    return 0;
+}
+
+// ------------------------------------------------------------------ Krylov
+// Textbook preconditioned CG and CR (the reference calls warp.optim.linear.cg/cr,
+// e.g. tests/test_fsai.py:64, benchmarks/synthetic.py:50).  Operators in double:
+//   op_kind 0: CSR A (off, col, val f64)
+//   op_kind 1: factored (K M^-1 K + w M), all rows free (K CSR f64, mass f64)
+// Preconditioner: 0 none, 1 FSAI (G, G^T CSR f64), 2 Jacobi on a diagonal array.
+// Stopping follows the capturable-loop convention: the residual is compared with
+// max(atol, tol*||b||) every check_every iterations, and an iteration whose
+// incoming residual already meets the target is a no-op, so the returned count is
+// the first converged iteration rounded up to a multiple of check_every.  With
+// this rule the reference's recorded grid32/grid64 counts are reproduced exactly.
+// Returns iterations; the residual norm (not squared) through *residual.
+struct OrcOp {
+    int kind;
+    int n;
+    const int* off; const int* col; const double* val;
+    const double* mass; double weight;
+    std::vector<double> tmp;
+    void apply(const double* x, double* y) {
+        if (kind == 0) {
+            for (int r = 0; r < n; ++r) {
+                double s = 0.0;
+                for (int e = off[r]; e < off[r + 1]; ++e) s += val[e] * x[col[e]];
+                y[r] = s;
+            }
+        } else {
+            tmp.resize(n);
+            for (int r = 0; r < n; ++r) {
+                double s = 0.0;
+                for (int e = off[r]; e < off[r + 1]; ++e) s += val[e] * x[col[e]];
+                tmp[r] = (1.0 / mass[r]) * s;
+            }
+            for (int r = 0; r < n; ++r) {
+                double s = 0.0;
+                for (int e = off[r]; e < off[r + 1]; ++e) s += val[e] * tmp[col[e]];
+                if (weight != 0.0) s += weight * mass[r] * x[r];
+                y[r] = s;
+            }
+        }
+    }
+};
+struct OrcPrec {
+    int kind;
+    int n;
+    const int* goff; const int* gcol; const double* gval;
+    const int* toff; const int* tcol; const double* tval;
+    const double* diag;
+    std::vector<double> tmp;
+    void apply(const double* r, double* z) {
+        if (kind == 0) { for (int i = 0; i < n; ++i) z[i] = r[i]; return; }
+        if (kind == 2) { for (int i = 0; i < n; ++i) z[i] = r[i] / diag[i]; return; }
+        tmp.resize(n);
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int e = goff[i]; e < goff[i + 1]; ++e) s += gval[e] * r[gcol[e]];
+            tmp[i] = s;
+        }
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int e = toff[i]; e < toff[i + 1]; ++e) s += tval[e] * tmp[tcol[e]];
+            z[i] = s;
+        }
+    }
+};
+
+static double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+int orc_krylov(int method, int n, int op_kind, const int* off, const int* col, const double* val,
+               const double* mass, double weight, int prec_kind, const int* goff, const int* gcol,
+               const double* gval, const int* toff, const int* tcol, const double* tval,
+               const double* diag, const double* b, double* x, double tol, double atol,
+               int maxiter, int check_every, double* residual, double* history) {
+    OrcOp A{op_kind, n, off, col, val, mass, weight, {}};
+    OrcPrec M{prec_kind, n, goff, gcol, gval, toff, tcol, tval, diag, {}};
+    std::vector<double> r(n), z(n), p(n), Ap(n), xv(x, x + n), bv(b, b + n), Az(n), Mr(n);
+    A.apply(xv.data(), Ap.data());
+    for (int i = 0; i < n; ++i) r[i] = bv[i] - Ap[i];
+    double bnorm = std::sqrt(dot(bv, bv));
+    double target = std::max(atol, tol * bnorm);
+    double rn = std::sqrt(dot(r, r));
+    if (history) history[0] = rn;
+    int it = 0;
+    if (check_every <= 0) check_every = 1;
+    if (rn <= target) { *residual = rn; return 0; }
+    M.apply(r.data(), z.data());
+    for (int i = 0; i < n; ++i) p[i] = z[i];
+    if (method == 0) {  // CG
+        double rz = dot(r, z);
+        while (it < maxiter) {
+            for (int k = 0; k < check_every && it + k < maxiter; ++k) {
+                if (!(rn > target)) continue;  // converged: the step is a no-op
+                A.apply(p.data(), Ap.data());
+                double pAp = dot(p, Ap);
+                double alpha = rz / pAp;
+                for (int i = 0; i < n; ++i) { xv[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+                rn = std::sqrt(dot(r, r));
+                if (history) history[it + k + 1] = rn;
+                M.apply(r.data(), z.data());
+                double rz_new = dot(r, z);
+                double beta = rz_new / rz;
+                rz = rz_new;
+                for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+            }
+            it = std::min(it + check_every, maxiter);
+            if (rn <= target) break;
+        }
+    } else {  // CR: z = M r, p = z, Az = A z, Ap = A p
+        A.apply(z.data(), Az.data());
+        for (int i = 0; i < n; ++i) Ap[i] = Az[i];
+        double zAz = dot(z, Az);
+        while (it < maxiter) {
+            for (int k = 0; k < check_every && it + k < maxiter; ++k) {
+                if (!(rn > target)) continue;
+                M.apply(Ap.data(), Mr.data());  // M A p
+                double denom = dot(Ap, Mr);
+                double alpha = zAz / denom;
+                for (int i = 0; i < n; ++i) { xv[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; z[i] -= alpha * Mr[i]; }
+                rn = std::sqrt(dot(r, r));
+                if (history) history[it + k + 1] = rn;
+                A.apply(z.data(), Az.data());
+                double zAz_new = dot(z, Az);
+                double beta = zAz_new / zAz;
+                zAz = zAz_new;
+                for (int i = 0; i < n; ++i) { p[i] = z[i] + beta * p[i]; Ap[i] = Az[i] + beta * Ap[i]; }
+            }
+            it = std::min(it + check_every, maxiter);
+            if (rn <= target) break;
+        }
+    }
+    for (int i = 0; i < n; ++i) x[i] = xv[i];
+    *residual = rn;
+    return it;
+}
+
+}  // extern "C"

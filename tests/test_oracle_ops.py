@@ -1,0 +1,226 @@
+"""Pins the oracle's refit, products, factored operator, block preconditioners and
+Krylov drivers against closed forms, scipy and the reference's recorded results."""
+
+import numpy as np
+import pytest
+import scipy.sparse as ss
+import scipy.sparse.linalg as sla
+
+
+def csr(cpu, a, dtype=np.float64, block=1):
+    return cpu.CSR.from_scipy(ss.csr_matrix(a), dtype, block)
+
+
+@pytest.mark.parametrize("block", [1, 3])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_refit_local_solves_and_rollback(oracle, block, dtype):
+    # tests/test_updates.py:16-56.
+    rng = np.random.default_rng(48)
+    r = rng.normal(size=(12, 12))
+    a = r @ r.T + np.eye(12) * 3
+    P = oracle.fsai(csr(oracle, a, dtype, block), max_row_size=5, factor_dtype=np.float32)
+    b = a + np.diag(np.arange(12))
+    Q = oracle.refit(P, csr(oracle, b, dtype, block))
+    g = Q["G"].to_dense()
+    np.testing.assert_array_equal(Q["GT"].to_dense(), g.T)
+    for i in range(12):
+        support = Q["G"].col[Q["G"].off[i]:Q["G"].off[i + 1]]
+        e = np.zeros(len(support))
+        e[-1] = 1
+        z = np.linalg.solve(b[np.ix_(support, support)], e)
+        np.testing.assert_allclose(g[i, support], z / np.sqrt(z[-1]), rtol=2e-4, atol=2e-7)
+    zero = csr(oracle, b, dtype, block)
+    zero.val[:] = 0
+    with pytest.raises(ValueError, match="positive diagonals"):
+        oracle.refit(P, zero)
+    changed = csr(oracle, b, dtype, block)
+    changed.col = changed.col.copy()
+    changed.col[0] += 1
+    with pytest.raises(ValueError, match="sparsity"):
+        oracle.refit(P, changed)
+    full = oracle.fsai(csr(oracle, a, dtype, block), max_row_size=12, kap_tolerance=0)
+    full = oracle.refit(full, csr(oracle, b, dtype, block))
+    gf = full["G"].to_dense()
+    np.testing.assert_allclose(gf.T @ gf, np.linalg.inv(b), rtol=2e-4, atol=1e-7)
+
+
+def test_refit_zero_inside_block_becomes_nonzero(oracle):
+    # tests/test_updates.py:90-101.
+    a = np.array([[4.0, 1.0, 0.0], [1.0, 4.0, 1.0], [0.0, 1.0, 4.0]])
+    b = a + 0.2 * np.ones((3, 3))
+    P = oracle.fsai(csr(oracle, a, block=3), max_row_size=3, kap_tolerance=0)
+    Q = oracle.refit(P, csr(oracle, b, block=3))
+    g = Q["G"].to_dense()
+    np.testing.assert_allclose(g.T @ g, np.linalg.inv(b), atol=1e-13)
+
+
+def test_source_map_addresses_block_values(oracle):
+    rng = np.random.default_rng(4)
+    a = ss.random(12, 12, density=0.3, random_state=rng).toarray() + np.eye(12)
+    A = csr(oracle, a, block=3)
+    S = oracle.scalarize(A)
+    mapping = oracle.source_map(S, A)
+    np.testing.assert_array_equal(A.val.reshape(-1)[mapping], S.val)
+    np.testing.assert_allclose(S.to_dense(), A.to_dense())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spmv_against_scipy(oracle, dtype):
+    # tests/test_sparse_operator.py:12-41 (empty first row, NaN in the unused operand).
+    rng = np.random.default_rng(91)
+    a = ss.random(67, 67, density=0.12, random_state=rng, format="csr")
+    a.data = rng.normal(size=a.nnz)
+    a = a.tolil()
+    a[0, :] = 0
+    a = a.tocsr()
+    A = csr(oracle, a, dtype)
+    x = rng.normal(size=67).astype(dtype)
+    y = rng.normal(size=67).astype(dtype)
+    rtol = 2e-5 if dtype == np.float32 else 1e-12
+    np.testing.assert_allclose(oracle.spmv(A, x, y, 0.7, -0.3), 0.7 * (a @ x) - 0.3 * y,
+                               rtol=rtol, atol=rtol)
+    nan = np.full(67, np.nan, dtype)
+    assert np.isfinite(oracle.spmv(A, x, nan, 1.0, 0.0)).all()
+    np.testing.assert_array_equal(oracle.spmv(A, nan, np.full(67, 2.0, dtype), 0.0, 0.5), 1.0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("mass_weight", [0.0, 0.15])
+def test_factored_operator_and_rhs(oracle, dtype, mass_weight):
+    # tests/test_squared_laplacian.py:13-58.
+    rng = np.random.default_rng(27)
+    n = 67
+    lap = ss.diags([-np.ones(n - 1), np.r_[1.0, np.full(n - 2, 2.0), 1.0], -np.ones(n - 1)],
+                   [-1, 0, 1]).tocsr()
+    mass = rng.uniform(0.4, 2, size=n).astype(dtype)
+    free = rng.permutation(np.arange(3, n - 4)).astype(np.int32)
+    op = oracle.sqlap_setup(csr(oracle, lap, dtype), mass, free)
+    q = lap @ ss.diags(1 / mass.astype(np.float64)) @ lap + mass_weight * ss.diags(mass)
+    reduced = q[free][:, free]
+    rtol, atol = (3e-5, 3e-6) if dtype == np.float32 else (1e-12, 1e-12)
+    x = rng.normal(size=len(free)).astype(dtype)
+    y = rng.normal(size=len(free)).astype(dtype)
+    np.testing.assert_allclose(oracle.sqlap_apply(op, x, y, 0.7, -0.2, mass_weight),
+                               0.7 * (reduced @ x) - 0.2 * y, rtol=rtol, atol=atol)
+    bc = rng.normal(size=n).astype(dtype)
+    prescribed = bc.copy()
+    prescribed[free] = 0
+    np.testing.assert_allclose(oracle.sqlap_apply(op, bc, None, -1.0, 0.0, boundary=True),
+                               -(q @ prescribed)[free], rtol=rtol, atol=atol)
+
+
+def test_factored_operator_validation(oracle):
+    L = csr(oracle, ss.eye(5))
+    for free in [[0, 0], [0, 5], [-1, 2]]:
+        with pytest.raises(ValueError, match="free_indices"):
+            oracle.sqlap_setup(L, np.ones(5), np.array(free))
+    with pytest.raises(ValueError, match="mass entries"):
+        oracle.sqlap_setup(L, np.zeros(5), np.array([1, 2]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("size", [1, 2, 3, 4])
+def test_block_ilu_exact_and_jacobi(oracle, dtype, size):
+    # tests/test_indefinite.py:12-43: enough sweeps give the exact LU.
+    rng = np.random.default_rng(61)
+    n = 5
+    blocks = rng.normal(size=(n, n, size, size)) * 0.03
+    for i in range(n):
+        blocks[i, i] = np.diag(np.arange(1, size + 1) * np.where(np.arange(size) % 2, -1.0, 1.0))
+    a = blocks.transpose(0, 2, 1, 3).reshape(n * size, n * size)
+    A = csr(oracle, a, dtype, size)
+    factors, inv = oracle.block_ilu0(A, factor_sweeps=2 * n + 2)
+    x = rng.normal(size=n * size).astype(dtype)
+    y = rng.normal(size=n * size).astype(dtype)
+    z = oracle.block_ilu0_apply(A, factors, inv, x, y, 0.7, -0.2, solve_sweeps=n)
+    tol = dict(rtol=2e-5, atol=1e-6) if dtype == np.float32 else dict(rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(z.ravel(), 0.7 * np.linalg.solve(a, x) - 0.2 * y, **tol)
+    jinv = oracle.block_jacobi(A)
+    got = oracle.block_jacobi_apply(jinv, x)
+    expected = np.stack([np.linalg.solve(blocks[i, i], x.reshape(n, size)[i]) for i in range(n)])
+    np.testing.assert_allclose(got, expected, rtol=1e-6, atol=1e-6)
+
+
+def test_bad_block_pivot(oracle):
+    with pytest.raises(ValueError, match="pivot"):
+        oracle.block_jacobi(csr(oracle, np.diag([1.0, 0.0]), block=2))
+
+
+def test_sparse_ilu_matches_restated_factors_and_is_linear(oracle):
+    # tests/test_indefinite.py:148-192.
+    rng = np.random.default_rng(83)
+    n, size = 9, 2
+    blocks = {}
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                blocks[i, j] = np.diag([3.0, -2.0])
+            elif abs(i - j) == 1 or (i + j) % 5 == 0:
+                blocks[i, j] = rng.normal(size=(size, size)) * 0.08
+    expected = {key: value.copy() for key, value in blocks.items()}
+    for i in range(n):
+        for j in sorted(j for row, j in blocks if row == i):
+            value = blocks[i, j].copy()
+            for k in range(min(i, j)):
+                if (i, k) in expected and (k, j) in expected:
+                    value -= expected[i, k] @ expected[k, j]
+            expected[i, j] = value @ np.linalg.inv(expected[j, j]) if j < i else value
+    a = np.zeros((size * n, size * n))
+    for (i, j), value in blocks.items():
+        a[size * i:size * (i + 1), size * j:size * (j + 1)] = value
+    A = csr(oracle, a, block=size)
+    factors, inv = oracle.block_ilu0(A, factor_sweeps=2 * n + 2)
+    for i in range(n):
+        for e in range(A.off[i], A.off[i + 1]):
+            np.testing.assert_allclose(factors[e], expected[i, A.col[e]], rtol=1e-12, atol=1e-12)
+    apply = lambda v: oracle.block_ilu0_apply(A, factors, inv, v, solve_sweeps=3).ravel()  # noqa: E731
+    x, y = rng.normal(size=(2, n * size))
+    np.testing.assert_allclose(apply(0.3 * x + 0.7 * y), 0.3 * apply(x) + 0.7 * apply(y),
+                               rtol=1e-12, atol=1e-12)
+
+
+def grid_problem(size):
+    # benchmarks/synthetic.py:24-30: Q = h^2 I + K^2 / h^2 on a size x size grid, seed 13.
+    d = ss.diags([-np.ones(size - 1), 2 * np.ones(size), -np.ones(size - 1)], [-1, 0, 1])
+    k = ss.kronsum(d, d).tocsr()
+    mass = 1 / (size + 1) ** 2
+    q = (mass * ss.eye(size * size) + k @ k / mass).tocsr()
+    exact = np.random.default_rng(13).normal(size=size * size)
+    return q, q @ exact
+
+
+@pytest.mark.parametrize("size,expected", [
+    (32, {1: 600, 4: 300, 8: 175, 16: 125}),
+    (64, {1: 2175, 4: 975, 8: 600, 16: 400}),
+])
+def test_cg_iteration_counts_match_recorded_grid_results(oracle, size, expected):
+    # results/grid32.json and results/grid64.json: FSAI-CG (and Jacobi-CG for width 1),
+    # float64, tol 1e-11, residual checked every 25 iterations.  Warp's exact CG
+    # recurrences are not in the reference; the no-op-after-convergence convention
+    # (oracle krylov) reproduces every recorded count exactly.
+    q, b = grid_problem(size)
+    Q = csr(oracle, q)
+    for width, count in expected.items():
+        if width == 1:
+            x, it, res = oracle.krylov("cg", b, A=Q, diag=q.diagonal(), tol=1e-11, maxiter=20000,
+                                       check_every=25)
+        else:
+            P = oracle.fsai(Q, max_row_size=width)
+            x, it, res = oracle.krylov("cg", b, A=Q, P=P, tol=1e-11, maxiter=20000,
+                                       check_every=25)
+        assert it == count, (width, it, count)
+        assert np.linalg.norm(b - q @ x) / np.linalg.norm(b) < 2e-11
+
+
+def test_cg_and_cr_converge_to_direct_solve(oracle):
+    n = 23
+    lap = ss.diags([-np.ones(n - 1), np.r_[1.0, np.full(n - 2, 2.0), 1.0], -np.ones(n - 1)],
+                   [-1, 0, 1]).tocsr()
+    mass = np.linspace(0.5, 2.0, n)
+    target = np.sin(np.arange(n))
+    w = 0.03
+    expected = sla.spsolve(lap @ ss.diags(1 / mass) @ lap + ss.diags(w * mass), w * mass * target)
+    for method in ["cg", "cr"]:
+        x, it, _ = oracle.krylov(method, w * mass * target, K=csr(oracle, lap), mass=mass, weight=w,
+                                 x0=target, tol=1e-12, maxiter=300)
+        np.testing.assert_allclose(x, expected, rtol=1e-10, atol=1e-11)
