@@ -1,0 +1,320 @@
+/* wpc.h -- C ABI of the MI355X (gfx950) warp-preconditioners hot path.
+ *
+ * Every entry point is stream-ordered and takes plain device pointers, element
+ * counts and a HIP stream (`stream`, a hipStream_t passed as void*; NULL is the
+ * null stream).  No entry point allocates device memory: the caller owns every
+ * buffer, including the scratch (`workspace`, `tmp`, `partials`) arguments.
+ * Indices are int32 (CSR offsets/columns, as in Warp's BsrMatrix).  Status
+ * counters are int32 device arrays the kernels atomically increment; the caller
+ * zeroes them and reads them back.
+ *
+ * Return value: WPC_OK (0), a negative WPC_E* code for an argument the library
+ * does not support, or a positive hipError_t from the launch.
+ *
+ * Each declaration cites the reference routine it replaces (paths relative to
+ * the root of alecjacobson/warp-preconditioners).  The reference is pure Python
+ * over NVIDIA Warp; these are the kernels a ctypes layer binds in its place.
+ */
+#ifndef WPC_H
+#define WPC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Value / vector storage formats. */
+enum {
+    WPC_F16 = 1,  /* IEEE binary16 */
+    WPC_BF16 = 2, /* bfloat16 (factor storage only) */
+    WPC_F32 = 3,
+    WPC_F64 = 4
+};
+
+enum {
+    WPC_OK = 0,
+    WPC_EDTYPE = -1,  /* unsupported storage/vector format combination */
+    WPC_EARG = -2,    /* size, width, lanes or vector-count out of range */
+    WPC_ESCRATCH = -3 /* workspace too small */
+};
+
+/* Library identity: returns a static string (build id and target). */
+const char* wpc_version(void);
+
+/* ------------------------------------------------------------ sparse products */
+
+/* z = alpha * diag(row_scale) * A * x + beta * y for scalar CSR A.
+ * Replaces sparse_operator.py:90-104 (_matvec: the grouped multi-lane kernel
+ * :10-40, the mixed-storage kernel :64-87 and Warp's bsr_mv).
+ * `lanes` (1,2,4,8,16,32,64; 0 = pick by mean row length) lanes cooperate on
+ * each row.  `nvec` (1,2,3,4,8) vectors are stored interleaved: x[col*nvec+v].
+ * row_scale (nullable, float32[n]) multiplies each row sum before alpha.
+ * alpha == 0 never reads A or x; beta == 0 never reads y.  x must not alias z;
+ * y may alias z.  Accumulation is in the vector format (float for binary16
+ * vectors).  val_dtype may be narrower than vec_dtype (mixed storage). */
+int wpc_csr_spmv(int n, const int* offsets, const int* columns, const void* values, int val_dtype,
+                 const float* row_scale, const void* x, const void* y, void* z, int vec_dtype,
+                 int nvec, double alpha, double beta, int lanes, void* stream);
+
+/* z = alpha * A * x + beta * y for b x b BSR (b = 1..4), same format for values
+ * and vectors (F32/F64).  Replaces Warp's bsr_mv on block matrices (CG on a
+ * BsrMatrix operator; biharmonic.py:103,106).  x must not alias z. */
+int wpc_bsr_spmv(int nb, int b, const int* offsets, const int* columns, const void* values,
+                 int dtype, const void* x, const void* y, void* z, double alpha, double beta,
+                 void* stream);
+
+/* z = alpha * G^T (G x) + beta * y through tmp[n*nvec].  Replaces FSAI._apply
+ * (fsai.py:374-381).  G is lower triangular CSR with its transpose GT stored
+ * explicitly.  With row_scale2 (nullable, float32[n]) the stored factor is
+ * q = G / s row-wise and the product is G^T G = q^T diag(s^2) q: the scale is
+ * applied in the first pass, GT holds q^T unscaled.  x is fully consumed before
+ * z is written, so x, y and z may alias.  alpha == 0 never reads x. */
+int wpc_fsai_apply(int n, const int* g_offsets, const int* g_columns, const void* g_values,
+                   const int* gt_offsets, const int* gt_columns, const void* gt_values,
+                   int factor_dtype, const float* row_scale2, void* tmp, const void* x,
+                   const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
+                   int lanes, void* stream);
+
+/* ------------------------------------------------------------ FSAI setup */
+
+/* scale[i] = 1/sqrt(A_ii); non-positive or non-finite diagonals set 0 and add 1
+ * to status[0].  Replaces fsai.py:43-57 (diagonal). dtype F16/F32/F64. */
+int wpc_fsai_diagonal(int n, const int* offsets, const int* columns, const void* values, int dtype,
+                      void* scale, int* status, void* stream);
+
+/* Adaptive equilibrated FSAI rows, one wavefront per row.  Replaces fsai.py:59-176
+ * (build).  Writes cols_out/vals_out[i*width+p] (-1/0 padded, selection order,
+ * diagonal first), status[1] += truncated rows, status[2] += invalid rows.
+ * width, step in [1,64]; tolerance is kap_tolerance, pivot_floor the local pivot
+ * floor.  Arithmetic is that of `dtype` (F16 as IEEE binary16 operations). */
+int wpc_fsai_build(int n, const int* offsets, const int* columns, const void* values, int dtype,
+                   const void* scale, double tolerance, double pivot_floor, int width, int step,
+                   int* cols_out, void* vals_out, int* status, void* stream);
+
+/* Sort each padded row by column and drop -1 slots; counts[i+1] = row size
+ * (counts[0] is set to 0).  Replaces fsai_pack.py:15-39 (sort_rows). */
+int wpc_fsai_sort_rows(int n, int width, int* columns, void* values, int dtype, int* counts,
+                       void* stream);
+
+/* out[i] = in[0] + ... + in[i] (int32) in one workgroup.  Replaces
+ * wp.utils.array_scan (fsai_pack.py:63). */
+int wpc_scan_inclusive(int count, const int* in, int* out, void* stream);
+
+/* Compact sorted padded rows into CSR.  Replaces fsai_pack.py:41-53 (pack). */
+int wpc_fsai_pack(int n, int width, const int* offsets, const int* columns, const void* values,
+                  int dtype, int* out_columns, void* out_values, void* stream);
+
+/* Expand compact b x b BSR into scalar CSR (row r*b+c), keeping stored zeros.
+ * Replaces fsai_pack.py:74-130 (scalarize / scalarize_compact). */
+int wpc_bsr_scalarize(int nb, int b, const int* offsets, const int* columns, const void* values,
+                      int dtype, int* out_offsets, int* out_columns, void* out_values,
+                      void* stream);
+
+/* Transpose scalar CSR (nrow x ncol, nnz = offsets[nrow]) with ascending
+ * columns in every output row; values move as raw `elem_bytes` elements.
+ * workspace: int32[ncol + 1].  Replaces Warp's bsr_transposed (fsai.py:337). */
+int wpc_csr_transpose(int nrow, int ncol, int nnz, const int* offsets, const int* columns,
+                      const void* values, int elem_bytes, int* t_offsets, int* t_columns,
+                      void* t_values, int* workspace, void* stream);
+
+/* dst[i] = (dst format) src[i].  Replaces bsr_copy(..., scalar_type=...) on
+ * values (fsai.py:325) and wp.utils.array_cast. */
+int wpc_convert(long count, const void* src, int src_dtype, void* dst, int dst_dtype,
+                void* stream);
+
+/* status[0] += rows with a non-finite value or a non-positive diagonal.
+ * Replaces fsai.py:181-196 (_validate_float32_factor). */
+int wpc_fsai_validate_factor(int n, const int* offsets, const int* columns, const void* values,
+                             int dtype, int* status, void* stream);
+
+/* Row-scaled narrow factor: s_i = max_j |G_ij|, q = G / s_i stored in q_dtype
+ * (F16/BF16/F32), row_scale2[i] = s_i^2 (float32).  status[0] += rows with a
+ * non-finite value or a non-positive stored diagonal (diagonal last).  No
+ * counterpart in the reference (its float16 factors are unscaled); it feeds
+ * wpc_fsai_apply's row_scale2 path. */
+int wpc_fsai_compress_scaled(int n, const int* offsets, const int* columns, const void* values,
+                             int dtype, void* q_values, int q_dtype, float* row_scale2,
+                             int* status, void* stream);
+
+/* ------------------------------------------------------------ FSAI refit */
+
+/* bad[0] += number of positions where a != b.  Replaces fsai_reuse.py:10-14. */
+int wpc_compare_i32(long count, const int* a, const int* b, int* bad, void* stream);
+
+/* Scalar nz -> index into the source BSR value array.  Replaces fsai_reuse.py:35-48. */
+int wpc_refit_source_map(int n, const int* offsets, const int* columns, const int* src_offsets,
+                         const int* src_columns, int block, int* mapping, void* stream);
+
+/* G nz -> position in GT.  Replaces fsai_reuse.py:51-61. */
+int wpc_refit_transpose_map(int n, const int* offsets, const int* columns, const int* t_offsets,
+                            const int* t_columns, int* mapping, void* stream);
+
+/* out[e] = src[map[e]] / out[map[e]] = src[e] on `elem_bytes` elements.
+ * Replace fsai_reuse.py:69-74 (gather) and :136-141 (transpose). */
+int wpc_gather(long count, int elem_bytes, const void* src, const int* map, void* out,
+               void* stream);
+int wpc_scatter(long count, int elem_bytes, const void* src, const int* map, void* out,
+                void* stream);
+
+/* Fixed-support refit: per row, dense Cholesky of the equilibrated principal
+ * submatrix on G's stored (sorted, diagonal last) support and
+ * G[i,S] = z/sqrt(z_last)*scale[S] with z = A[S,S]^-1 e_last.  Writes
+ * out[offsets of G] in out_dtype; bad[0] += invalid rows.  Replaces
+ * fsai_reuse.py:76-134 (refit). */
+int wpc_fsai_refit(int n, const int* offsets, const int* columns, const void* values, int dtype,
+                   const int* g_offsets, const int* g_columns, const void* scale,
+                   double pivot_floor, int width, void* out, int out_dtype, int* bad,
+                   void* stream);
+
+/* out[i] = +-1 (Rademacher), deterministic in (seed, i).  Stands in for
+ * fsai_reuse.py:143-147 (probe); the random stream differs from Warp's. */
+int wpc_rademacher(int n, int seed, void* out, int dtype, void* stream);
+
+/* ------------------------------------------------------------ vectors */
+
+/* z = alpha * x + beta * y elementwise (count elements; alpha == 0 never reads
+ * x, beta == 0 never reads y).  Replaces fsai_reuse.py:149-152 (subtract),
+ * block_ilu.py:173-185 (combine) and Warp's array_axpy. */
+int wpc_axpby(long count, double alpha, const void* x, double beta, const void* y, void* z,
+              int dtype, void* stream);
+
+/* out[v] = sum_i x[i*nvec+v]*y[i*nvec+v] in float64, deterministic order
+ * (per-block partials, then one fixed-order pass); `rows` = count / nvec.
+ * partials: double[wpc_dot_partials(rows) * nvec].  Replaces
+ * wp.utils.array_inner (fsai_reuse.py:278). */
+int wpc_dot_partials(long rows);
+int wpc_dot(long rows, int nvec, const void* x, const void* y, int dtype, double* partials,
+            double* out, void* stream);
+
+/* dst[i] = value (count elements of dtype) / int32 fill. */
+int wpc_fill(long count, void* dst, int dtype, double value, void* stream);
+int wpc_fill_i32(long count, int* dst, int value, void* stream);
+
+/* ------------------------------------------------------------ squared Laplacian */
+
+/* index[free[i]] = i with duplicate/range detection (index pre-filled with -1).
+ * status[0] += out-of-range, status[1] += duplicates.  Replaces
+ * squared_laplacian.py:11-20 (_index_free). */
+int wpc_sqlap_index_free(int n_free, const int* free_indices, int n, int* index, int* status,
+                         void* stream);
+
+/* inverse[i] = 1/mass[i]; status[2] += non-positive/non-finite masses.
+ * Replaces squared_laplacian.py:307-314 (mass_inverse). */
+int wpc_sqlap_mass_inverse(int n, const void* mass, void* inverse, int dtype, int* status,
+                           void* stream);
+
+/* Factored (L M^-1 L + w M)[f,f] product.  Replaces squared_laplacian.py:316-377
+ * (forward/backward) as driven by _product :437-474.
+ *   pass 1: tmp[r] = inverse[r] * sum_{e in row r} L_e * x~[col]   (all rows of L)
+ *           x~[col] = x[index[col]] if index[col] >= 0 (boundary = 0), or
+ *                     x[col] if index[col] < 0 (boundary = 1)
+ *   pass 2: z[i] = alpha*(sum_{e in row free[i]} L_e tmp[col] + w*mass[free[i]]*x[i])
+ *                  + beta*y[i]
+ * index/free may be NULL when every row is free (identity maps).  nvec
+ * interleaved vectors.  x, y, z may alias (x is consumed by pass 1 except for
+ * the same-row mass term).  tmp: n_full*nvec. */
+int wpc_sqlap_apply(int n_full, const int* offsets, const int* columns, const void* values,
+                    int dtype, const void* inverse, const int* index, int n_free,
+                    const int* free_indices, const void* mass, double mass_weight, void* tmp,
+                    const void* x, const void* y, void* z, int nvec, double alpha, double beta,
+                    int boundary, int lanes, void* stream);
+
+/* ------------------------------------------------------------ block Jacobi / ILU(0) */
+
+/* rows[e] = block row of e; diagonal[i] = slot of block (i,i) or -1 (bad[0]++).
+ * Replaces block_ilu.py:45-61 (topology). */
+int wpc_block_topology(int nb, const int* offsets, const int* columns, int* rows, int* diagonal,
+                       int* bad, void* stream);
+
+/* inverse[i] = safe_inverse(values[diagonal[i]]) for b = 1..4.  Replaces
+ * block_ilu.py:16-43 (safe_inverse) + :63-73 (invert). */
+int wpc_block_invert(int nb, int b, const void* values, const int* diagonal, void* inverse,
+                     double tolerance, int* bad, int dtype, void* stream);
+
+/* Binary-search each diagonal block and invert it.  Replaces block_ilu.py:75-104
+ * (diagonal_only; compact storage). */
+int wpc_block_diagonal_only(int nb, int b, const int* offsets, const int* columns,
+                            const void* values, void* inverse, double tolerance, int* bad,
+                            int dtype, void* stream);
+
+/* factors[e] = values[e] * inverse[col] below the diagonal, else values[e].
+ * Replaces block_ilu.py:106-118 (initialize). */
+int wpc_ilu_initialize(long nnzb, int b, const int* rows, const int* columns, const void* values,
+                       const void* inverse, void* factors, int dtype, void* stream);
+
+/* One synchronous ILU(0) sweep with relaxation.  Replaces block_ilu.py:120-150. */
+int wpc_ilu_sweep(long nnzb, int b, const int* offsets, const int* columns, const int* rows,
+                  const void* values, const void* old, const void* inverse, void* out,
+                  double relaxation, int dtype, void* stream);
+
+/* One Jacobi step of the unit-lower (lower=1) or upper (lower=0) solve.
+ * Replaces block_ilu.py:152-171 (triangular). */
+int wpc_ilu_triangular(int nb, int b, const int* offsets, const int* columns, const void* factors,
+                       const void* inverse, const void* rhs, const void* old, void* out,
+                       int lower, int dtype, void* stream);
+
+/* z_i = alpha * D_i^-1 x_i + beta * y_i.  Replaces block_ilu.py:187-202. */
+int wpc_block_diagonal_apply(int nb, int b, const void* inverse, const void* x, const void* y,
+                             void* z, double alpha, double beta, int dtype, void* stream);
+
+/* bad[0] += non-finite entries.  Replaces block_ilu.py:204-210 (validate_factors). */
+int wpc_count_nonfinite(long count, const void* values, int dtype, int* bad, void* stream);
+
+/* bad[0] += entries that are not finite and positive.  Replaces biharmonic.py:70-74. */
+int wpc_count_nonpositive(long count, const void* values, int dtype, int* bad, void* stream);
+
+/* ------------------------------------------------------------ Krylov (batched) */
+
+/* Vector steps of warp.optim.linear.cg / cr (the reference's solvers, called e.g.
+ * at tests/test_fsai.py:64 and benchmarks/synthetic.py:50), for nvec right-hand
+ * sides on interleaved [n][nvec] vectors in dtype F32 or F64.  Scalars live on
+ * the device in a float64 array state[WPC_KSTATE * nvec], field-major
+ * (state[field * nvec + v]).  A slot whose incoming r.r already meets its
+ * target takes no step (device-side stopping), so a captured chunk of
+ * iterations may run past convergence without changing the result; ITER counts
+ * the steps actually taken.
+ *
+ * One CG iteration:  Ap = A p;  krylov_dot(p, Ap -> pd);  krylov_update;
+ *                    z = M r;   krylov_dot(r, z -> pz);  krylov_direction.
+ * One CR iteration:  MAp = M Ap; krylov_dot(Ap, MAp -> pd); krylov_update (also
+ *                    z -= alpha MAp);  Az = A z; krylov_dot(z, Az -> pz);
+ *                    krylov_direction (also Ap = Az + beta Ap). */
+enum {
+    WPC_KS_RZ = 0,      /* r.z (CG) or z.Az (CR) of the current direction */
+    WPC_KS_RR = 1,      /* r.r */
+    WPC_KS_TARGET = 2,  /* squared stopping threshold */
+    WPC_KS_ALPHA = 3,
+    WPC_KS_ITER = 4,    /* steps taken */
+    WPC_KS_DENOM = 5,   /* p.Ap (CG) or Ap.MAp (CR) */
+    WPC_KS_RZ_OLD = 6,
+    WPC_KS_ACTIVE = 7,
+    WPC_KSTATE = 8
+};
+
+/* Number of blocks P the Krylov kernels use for n rows; each partials region is
+ * double[P * nvec]. */
+int wpc_krylov_partials(int n);
+
+/* partials[b*nvec+v] = block b's share of sum_i a[i,v]*b[i,v]. */
+int wpc_krylov_dot(int n, int nvec, const void* a, const void* b, int dtype, double* partials,
+                   void* stream);
+
+/* Folds part_denom; alpha = active ? rz/denom : 0; x += alpha p; r -= alpha Ap
+ * (CR: z -= alpha map); writes r.r partials to part_rr.  method 0 = CG, 1 = CR. */
+int wpc_krylov_update(int method, int n, int nvec, void* x, void* r, const void* p,
+                      const void* ap, void* z, const void* map, int dtype, double* state,
+                      const double* part_denom, double* part_rr, void* stream);
+
+/* Folds part_rz and part_rr; for slots that stepped: beta = rz_new/rz_old,
+ * p = z + beta p (CR: ap = az + beta ap), state RZ/RR/ITER updated. */
+int wpc_krylov_direction(int method, int n, int nvec, const void* z, void* p, const void* az,
+                         void* ap, int dtype, double* state, const double* part_rz,
+                         const double* part_rr, void* stream);
+
+/* state[TARGET] holds ||b||^2 on entry; becomes max(atol, tol*||b||)^2; ITER = 0. */
+int wpc_krylov_set_target(int nvec, double* state, double tol, double atol, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WPC_H */

@@ -1,0 +1,242 @@
+// Vector steps of batched preconditioned CG and CR with device-resident scalars.
+//
+// nvec right-hand sides advance together on interleaved [n][nvec] vectors.  Every
+// inner product is reduced deterministically: each of P blocks writes one partial
+// per vector (fixed tree order), and the consuming kernel folds the P partials in
+// a fixed order in every block, so no host round trip and no atomics are needed.
+// A slot whose incoming residual already meets its target takes no step, which
+// lets a captured chunk of iterations run past convergence unchanged.
+
+#include "wpc_common.h"
+
+namespace wpc {
+
+constexpr int kKrylovThreads = 256;
+
+int krylov_blocks(int n) {
+    long b = ((long)n + 511) / 512;
+    if (b > 1024) b = 1024;
+    return (int)(b < 1 ? 1 : b);
+}
+
+// Fold partials[0..P) (stride nvec) for vector v with all threads of the block, in
+// a fixed order; the result lands in red[0].
+template <int NV>
+__device__ void fold(const double* __restrict__ partials, int P, int v, double* red) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < P; b += kKrylovThreads) s += partials[(long)b * NV + v];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kKrylovThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+}
+
+// Block-level sum of per-thread values acc[NV] -> partials[blockIdx.x*NV + v].
+template <int NV>
+__device__ void block_partial(const double* acc, double* red, double* partials) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        red[threadIdx.x] = acc[v];
+        __syncthreads();
+        for (int w = kKrylovThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partials[(long)blockIdx.x * NV + v] = red[0];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void chunk_of(int n, long* begin, long* end) {
+    const long chunk = ((long)n + gridDim.x - 1) / gridDim.x;
+    *begin = (long)blockIdx.x * chunk;
+    *end = *begin + chunk < n ? *begin + chunk : n;
+}
+
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void kdot_kernel(int n, const T* __restrict__ a,
+                                                              const T* __restrict__ b,
+                                                              double* __restrict__ partials) {
+    __shared__ double red[kKrylovThreads];
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += (double)a[i * NV + v] * (double)b[i * NV + v];
+    }
+    block_partial<NV>(acc, red, partials);
+}
+
+// alpha = active ? rz / denom : 0;  x += alpha p;  r -= alpha Ap;  (CR: z -= alpha MAp)
+template <class T, int NV, bool CR>
+__global__ __launch_bounds__(kKrylovThreads) void kupdate_kernel(
+    int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap,
+    T* __restrict__ z, const T* __restrict__ map, double* __restrict__ state,
+    const double* __restrict__ part_denom, double* __restrict__ part_rr) {
+    __shared__ double red[kKrylovThreads];
+    __shared__ double alpha_s[NV];
+    const int P = gridDim.x;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        fold<NV>(part_denom, P, v, red);
+        if (threadIdx.x == 0) {
+            const double denom = red[0];
+            const double rz = state[WPC_KS_RZ * NV + v];
+            const bool active = state[WPC_KS_RR * NV + v] > state[WPC_KS_TARGET * NV + v];
+            const double alpha = active ? rz / denom : 0.0;
+            alpha_s[v] = alpha;
+            if (blockIdx.x == 0) {
+                state[WPC_KS_ALPHA * NV + v] = alpha;
+                state[WPC_KS_DENOM * NV + v] = denom;
+                state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
+                state[WPC_KS_RZ_OLD * NV + v] = rz;
+            }
+        }
+        __syncthreads();
+    }
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long k = i * NV + v;
+            const T a = (T)alpha_s[v];
+            T rk = r[k];
+            if (a != T(0)) {
+                x[k] += a * p[k];
+                rk -= a * ap[k];
+                r[k] = rk;
+                if (CR) z[k] -= a * map[k];
+            }
+            acc[v] += (double)rk * (double)rk;
+        }
+    }
+    block_partial<NV>(acc, red, part_rr);
+}
+
+// beta = rz_new / rz_old; p = z + beta p (CR: Ap = Az + beta Ap) for active slots.
+template <class T, int NV, bool CR>
+__global__ __launch_bounds__(kKrylovThreads) void kdirection_kernel(
+    int n, const T* __restrict__ z, T* __restrict__ p, const T* __restrict__ az,
+    T* __restrict__ ap, double* __restrict__ state, const double* __restrict__ part_rz,
+    const double* __restrict__ part_rr) {
+    __shared__ double red[kKrylovThreads];
+    __shared__ double beta_s[NV];
+    __shared__ int active_s[NV];
+    const int P = gridDim.x;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        fold<NV>(part_rz, P, v, red);
+        double rz_new = red[0];
+        __syncthreads();
+        fold<NV>(part_rr, P, v, red);
+        if (threadIdx.x == 0) {
+            const double rr = red[0];
+            const bool active = state[WPC_KS_ACTIVE * NV + v] != 0.0;
+            const double rz_old = state[WPC_KS_RZ_OLD * NV + v];
+            beta_s[v] = active ? rz_new / rz_old : 0.0;
+            active_s[v] = active;
+            if (blockIdx.x == 0 && active) {
+                state[WPC_KS_RZ * NV + v] = rz_new;
+                state[WPC_KS_RR * NV + v] = rr;
+                state[WPC_KS_ITER * NV + v] += 1.0;
+            }
+        }
+        __syncthreads();
+    }
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (!active_s[v]) continue;
+            const long k = i * NV + v;
+            const T b = (T)beta_s[v];
+            p[k] = z[k] + b * p[k];
+            if (CR) ap[k] = az[k] + b * ap[k];
+        }
+    }
+}
+
+// target = max(atol, tol * sqrt(||b||^2))^2 in place (state[TARGET] holds ||b||^2).
+__global__ void ktarget_kernel(int nvec, double* state, double tol, double atol) {
+    const int v = threadIdx.x;
+    if (v >= nvec) return;
+    const double bnorm = sqrt(state[WPC_KS_TARGET * nvec + v]);
+    const double t = fmax(atol, tol * bnorm);
+    state[WPC_KS_TARGET * nvec + v] = t * t;
+    state[WPC_KS_ITER * nvec + v] = 0.0;
+}
+
+}  // namespace wpc
+
+using namespace wpc;
+
+#define NV_SWITCH(nvec, CALL)                               \
+    switch (nvec) {                                         \
+        case 1: { constexpr int NV = 1; CALL; break; }      \
+        case 2: { constexpr int NV = 2; CALL; break; }      \
+        case 3: { constexpr int NV = 3; CALL; break; }      \
+        case 4: { constexpr int NV = 4; CALL; break; }      \
+        case 8: { constexpr int NV = 8; CALL; break; }      \
+        default: return WPC_EARG;                           \
+    }
+#define VT_SWITCH(dt, CALL)                                 \
+    if ((dt) == WPC_F64) { using T = double; CALL; }        \
+    else if ((dt) == WPC_F32) { using T = float; CALL; }    \
+    else return WPC_EDTYPE;
+
+extern "C" {
+
+int wpc_krylov_partials(int n) { return krylov_blocks(n); }
+
+int wpc_krylov_dot(int n, int nvec, const void* a, const void* b, int dtype, double* partials,
+                   void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((kdot_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)a, (const T*)b, partials)))
+    return launch_status();
+}
+
+int wpc_krylov_update(int method, int n, int nvec, void* x, void* r, const void* p,
+                      const void* ap, void* z, const void* map, int dtype, double* state,
+                      const double* part_denom, double* part_rr, void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    if (method == 0) {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((kupdate_kernel<T, NV, false>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, (T*)z, (const T*)map, state, part_denom, part_rr)))
+    } else {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((kupdate_kernel<T, NV, true>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, (T*)z, (const T*)map, state, part_denom, part_rr)))
+    }
+    return launch_status();
+}
+
+int wpc_krylov_direction(int method, int n, int nvec, const void* z, void* p, const void* az,
+                         void* ap, int dtype, double* state, const double* part_rz,
+                         const double* part_rr, void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    if (method == 0) {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((kdirection_kernel<T, NV, false>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, (const T*)az, (T*)ap, state, part_rz, part_rr)))
+    } else {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((kdirection_kernel<T, NV, true>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, (const T*)az, (T*)ap, state, part_rz, part_rr)))
+    }
+    return launch_status();
+}
+
+int wpc_krylov_set_target(int nvec, double* state, double tol, double atol, void* stream) {
+    hipLaunchKernelGGL(ktarget_kernel, dim3(1), dim3(64), 0, as_stream(stream), nvec, state, tol, atol);
+    return launch_status();
+}
+
+const char* wpc_version(void) { return "wpc 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+"""Linear-operator protocol and device-resident CG / CR drivers.
+
+The reference passes its operators to ``warp.optim.linear`` (LinearOperator,
+aslinearoperator, cg, cr, preconditioner; e.g. tests/test_fsai.py:64,
+benchmarks/synthetic.py:50, fsai_reuse.py:256).  Warp has no ROCm backend, so
+this module provides the same call signatures on torch device buffers, with
+every vector step in the HIP library (include/wpc.h, Krylov section):
+
+* ``matvec(x, y, z, alpha, beta)`` computes ``z = alpha * Op x + beta * y``.
+* ``cg``/``cr`` stop when ``||r||_2 <= max(atol, tol * ||b||_2)``, checked on
+  the host every ``check_every`` iterations; a step whose incoming residual
+  already meets the target changes nothing, so the returned count is the first
+  converged iteration rounded up to a multiple of ``check_every`` (with
+  ``check_every=0`` the exact count is returned as a device tensor, like Warp's
+  capturable loop).  Chunks of ``check_every`` iterations are replayed from a
+  captured HIP graph when ``use_cuda_graph``.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _native as nat
+from .sparse import BlockType, BsrMatrix, _product
+
+
+class LinearOperator:
+    def __init__(self, shape, dtype, device, matvec):
+        self.shape = tuple(shape)
+        self.dtype = dtype
+        self.device = torch.device(device) if device is not None else None
+        self.matvec = matvec
+
+    @property
+    def scalar_type(self):
+        return self.dtype.scalar_type if isinstance(self.dtype, BlockType) else self.dtype
+
+
+class _MatrixOperator(LinearOperator):
+    """BsrMatrix as an operator: Warp's bsr_mv with aliasing handled."""
+
+    def __init__(self, A: BsrMatrix, lanes=8):
+        self.A = A
+        self.lanes = lanes
+        self._alias = None
+        super().__init__(A.shape, A.dtype, A.device, self._apply)
+
+    def _apply(self, x, y, z, alpha, beta):
+        xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+        if xf.data_ptr() == zf.data_ptr():
+            if self._alias is None or self._alias.numel() != xf.numel() or self._alias.dtype != xf.dtype:
+                self._alias = torch.empty_like(xf)
+            self._alias.copy_(xf)
+            xf = self._alias
+        _product(self.A, xf, yf, zf, alpha, beta, self.lanes)
+
+
+def aslinearoperator(A) -> LinearOperator:
+    if A is None or isinstance(A, LinearOperator):
+        return A
+    if isinstance(A, BsrMatrix):
+        return _MatrixOperator(A)
+    raise TypeError("expected a LinearOperator or BsrMatrix")
+
+
+def preconditioner(A: BsrMatrix, ptype: str = "diag") -> LinearOperator:
+    """Scalar Jacobi (``ptype='diag'``) or identity (``'id'``)."""
+    if ptype == "id":
+        return None
+    if ptype != "diag":
+        raise ValueError(f"unsupported preconditioner type {ptype!r}")
+    from .block_ilu import BlockJacobi
+    from .sparse import bsr_copy
+
+    scalar = A if A.block_shape == (1, 1) else bsr_copy(A, block_shape=(1, 1))
+    jac = BlockJacobi(scalar)
+    op = LinearOperator(A.shape, A.dtype, A.device, lambda x, y, z, a, b: jac.matvec(
+        x.reshape(-1), y.reshape(-1), z.reshape(-1), a, b))
+    op.inner = jac
+    return op
+
+
+class KrylovWork:
+    """Device buffers and launch sequence of one CG/CR solve on nvec interleaved RHS."""
+
+    def __init__(self, method, A, M, n, nvec, dtype, device):
+        self.method = 0 if method == "cg" else 1
+        self.A, self.M = A, M
+        self.n, self.nvec, self.dtype = n, nvec, dtype
+        size = n * nvec
+        new = lambda: torch.empty(size, dtype=dtype, device=device)  # noqa: E731
+        self.r, self.p, self.ap = new(), new(), new()
+        self.z = new() if (M is not None or self.method == 1) else self.r
+        self.map = new() if self.method == 1 else None
+        self.az = new() if self.method == 1 else None
+        self.P = nat.lib().wpc_krylov_partials(n)
+        self.parts = torch.zeros(3, self.P * nvec, dtype=torch.float64, device=device)
+        self.state = torch.zeros(nat_state_fields() * nvec, dtype=torch.float64, device=device)
+        self.dot_parts = torch.empty(nat.lib().wpc_dot_partials(n) * nvec, dtype=torch.float64,
+                                     device=device)
+        self.code = nat.code(dtype)
+        self.graph = None
+        self.graph_steps = 0
+
+    # -- helpers
+    def _field(self, k):
+        return self.state[k * self.nvec:(k + 1) * self.nvec]
+
+    def _dot(self, a, b, out):
+        nat.call("wpc_dot", self.n, self.nvec, nat.ptr(a), nat.ptr(b), self.code,
+                 nat.ptr(self.dot_parts), nat.ptr(out), nat.stream())
+
+    def _apply_M(self, src, dst):
+        if self.M is None:
+            if dst.data_ptr() != src.data_ptr():
+                dst.copy_(src)
+        else:
+            self.M.matvec(src, dst, dst, 1.0, 0.0)
+
+    def start(self, b, x, tol, atol):
+        """r = b - A x; initial direction; device-side target."""
+        self.A.matvec(x, b, self.r, -1.0, 1.0)
+        self._dot(b, b, self._field(2))  # ||b||^2 into TARGET
+        nat.call("wpc_krylov_set_target", self.nvec, nat.ptr(self.state), float(tol), float(atol),
+                 nat.stream())
+        self._dot(self.r, self.r, self._field(1))
+        self._apply_M(self.r, self.z)
+        if self.method == 0:
+            self._dot(self.r, self.z, self._field(0))
+            self.p.copy_(self.z)
+        else:
+            self.A.matvec(self.z, self.az, self.az, 1.0, 0.0)
+            self._dot(self.z, self.az, self._field(0))
+            self.p.copy_(self.z)
+            self.ap.copy_(self.az)
+
+    def step(self, x):
+        s = nat.stream()
+        pd, pr, pz = (nat.ptr(self.parts[k]) for k in range(3))
+        st = nat.ptr(self.state)
+        if self.method == 0:
+            self.A.matvec(self.p, self.ap, self.ap, 1.0, 0.0)
+            nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.p), nat.ptr(self.ap),
+                     self.code, pd, s)
+            nat.call("wpc_krylov_update", 0, self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
+                     nat.ptr(self.p), nat.ptr(self.ap), None, None, self.code, st, pd, pr, s)
+            self._apply_M(self.r, self.z)
+            nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.r), nat.ptr(self.z),
+                     self.code, pz, s)
+            nat.call("wpc_krylov_direction", 0, self.n, self.nvec, nat.ptr(self.z),
+                     nat.ptr(self.p), None, None, self.code, st, pz, pr, s)
+        else:
+            self._apply_M(self.ap, self.map)
+            nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.ap), nat.ptr(self.map),
+                     self.code, pd, s)
+            nat.call("wpc_krylov_update", 1, self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
+                     nat.ptr(self.p), nat.ptr(self.ap), nat.ptr(self.z), nat.ptr(self.map),
+                     self.code, st, pd, pr, s)
+            self.A.matvec(self.z, self.az, self.az, 1.0, 0.0)
+            nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.az),
+                     self.code, pz, s)
+            nat.call("wpc_krylov_direction", 1, self.n, self.nvec, nat.ptr(self.z),
+                     nat.ptr(self.p), nat.ptr(self.az), nat.ptr(self.ap), self.code, st, pz, pr, s)
+
+    def run(self, x, steps, use_graph):
+        if not use_graph or steps <= 0:
+            for _ in range(steps):
+                self.step(x)
+            return
+        if self.graph is None or self.graph_steps != steps or self._graph_x != x.data_ptr():
+            # warm once eagerly (first launches load code objects), then capture
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                for _ in range(steps):
+                    self.step(x)
+            self.graph, self.graph_steps, self._graph_x = g, steps, x.data_ptr()
+        self.graph.replay()
+
+    def host_state(self):
+        s = self.state.cpu().view(-1, self.nvec)
+        return s
+
+
+def nat_state_fields() -> int:
+    return 8
+
+
+def _default_tol(dtype):
+    return {torch.float32: 1.0e-6, torch.float64: 1.0e-12}.get(dtype, 1.0e-3)
+
+
+def _solve(method, A, b, x, tol, atol, maxiter, M, callback, check_every, use_cuda_graph):
+    A = aslinearoperator(A)
+    M = aslinearoperator(M)
+    bf = b.reshape(-1)
+    xf = x.reshape(-1)
+    if xf.data_ptr() != x.data_ptr() or not x.is_contiguous():
+        raise ValueError("x must be a contiguous device array")
+    nat.require_device(bf, "b")
+    if bf.dtype not in (torch.float32, torch.float64) or xf.dtype != bf.dtype:
+        raise TypeError("b and x must share a float32 or float64 dtype")
+    n = bf.numel()
+    if maxiter == 0:
+        maxiter = A.shape[0]
+    if tol is None and atol is None:
+        tol = _default_tol(bf.dtype)
+    tol = 0.0 if tol is None else float(tol)
+    atol = 0.0 if atol is None else float(atol)
+    work = KrylovWork(method, A, M, n, 1, bf.dtype, bf.device)
+    work.start(bf, xf, tol, atol)
+    st = work.host_state()
+    rr, target = float(st[1, 0]), float(st[2, 0])
+    abs_tol = math.sqrt(target)
+    exact = check_every == 0
+    chunk = check_every if check_every > 0 else 32
+    iters = 0
+    if callback is not None:
+        callback(0, math.sqrt(rr), abs_tol)
+    if rr > target:
+        while iters < maxiter:
+            steps = min(chunk, maxiter - iters)
+            work.run(xf, steps, use_cuda_graph and steps == chunk and iters > 0)
+            iters += steps
+            st = work.host_state()
+            rr = float(st[1, 0])
+            if callback is not None:
+                callback(iters, math.sqrt(rr), abs_tol)
+            if rr <= target:
+                break
+    if exact:
+        taken = int(st[4, 0])
+        return (torch.tensor([taken], dtype=torch.int32, device=bf.device),
+                torch.tensor([rr], dtype=torch.float64, device=bf.device), abs_tol)
+    return iters, math.sqrt(rr), abs_tol
+
+
+def cg(A, b, x, tol=None, atol=None, maxiter=0, M=None, callback=None, check_every=10,
+       use_cuda_graph=True):
+    """Preconditioned conjugate gradients; returns (iterations, residual, atol)."""
+    return _solve("cg", A, b, x, tol, atol, maxiter, M, callback, check_every, use_cuda_graph)
+
+
+def cr(A, b, x, tol=None, atol=None, maxiter=0, M=None, callback=None, check_every=10,
+       use_cuda_graph=True):
+    """Preconditioned conjugate residuals; returns (iterations, residual, atol)."""
+    return _solve("cr", A, b, x, tol, atol, maxiter, M, callback, check_every, use_cuda_graph)

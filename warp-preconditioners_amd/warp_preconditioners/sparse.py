@@ -1,0 +1,232 @@
+"""Block-sparse-row matrices over torch device buffers.
+
+The reference package stores matrices in Warp's ``warp.sparse.BsrMatrix`` and uses
+Warp's ``bsr_from_triplets``, ``bsr_copy``, ``bsr_zeros``, ``bsr_transposed`` and
+``bsr_mv`` (call sites: fsai.py:279,325,337; sparse_operator.py:104,127;
+biharmonic.py:97-106,135,169-170).  ``BsrMatrix`` here exposes the attributes
+the package reads (offsets, columns, values, nrow, ncol, shape, block_shape,
+scalar_type, dtype, device, nnz, nnz_sync(), row_counts) with int32 index
+buffers and values of shape [nnz] (1x1 blocks) or [nnz, b, b].  Storage is
+always compact and canonical: sorted columns, no duplicate blocks.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+@dataclass(frozen=True)
+class BlockType:
+    """dtype of a b x b block (Warp's matrix(shape=(b, b)) value type)."""
+
+    rows: int
+    cols: int
+    scalar_type: torch.dtype
+
+
+class BsrMatrix:
+    def __init__(self, nrow, ncol, block_shape, offsets, columns, values):
+        self.nrow = int(nrow)
+        self.ncol = int(ncol)
+        self.block_shape = tuple(block_shape)
+        self.offsets = offsets
+        self.columns = columns
+        self.values = values
+        self.row_counts = None  # always compact
+
+    @property
+    def scalar_type(self) -> torch.dtype:
+        return self.values.dtype
+
+    @property
+    def dtype(self):
+        if self.block_shape == (1, 1):
+            return self.values.dtype
+        return BlockType(self.block_shape[0], self.block_shape[1], self.values.dtype)
+
+    @property
+    def device(self) -> torch.device:
+        return self.values.device
+
+    @property
+    def shape(self):
+        return (self.nrow * self.block_shape[0], self.ncol * self.block_shape[1])
+
+    @property
+    def nnz(self) -> int:
+        return int(self.columns.numel())
+
+    def nnz_sync(self) -> int:
+        return self.nnz
+
+    def notify_nnz_changed(self, nnz=None):
+        pass
+
+    # host helpers (tests, diagnostics) ------------------------------------
+    def to_scipy(self):
+        import scipy.sparse as ss
+
+        b0, b1 = self.block_shape
+        off = self.offsets.cpu().numpy()
+        col = self.columns.cpu().numpy()
+        val = self.values.detach().to(torch.float64).cpu().numpy()
+        if (b0, b1) == (1, 1):
+            return ss.csr_matrix((val.reshape(-1), col, off), shape=self.shape)
+        return ss.bsr_matrix((val.reshape(-1, b0, b1), col, off), shape=self.shape).tocsr()
+
+    def to_dense(self) -> np.ndarray:
+        return self.to_scipy().toarray()
+
+
+def _device(device):
+    return torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+
+
+def bsr_zeros(nrow, ncol, dtype, device=None, block_shape=(1, 1)) -> BsrMatrix:
+    device = _device(device)
+    shape = (0,) if tuple(block_shape) == (1, 1) else (0, *block_shape)
+    return BsrMatrix(
+        nrow, ncol, block_shape,
+        torch.zeros(nrow + 1, dtype=torch.int32, device=device),
+        torch.zeros(0, dtype=torch.int32, device=device),
+        torch.zeros(shape, dtype=dtype, device=device),
+    )
+
+
+def bsr_from_triplets(nrow, ncol, rows, cols, values, prune_numerical_zeros=False) -> BsrMatrix:
+    """Sum duplicate (row, col) blocks and sort (Warp's bsr_from_triplets).
+
+    ``values`` is [nnz] for scalars or [nnz, b, b] for blocks.  Stored zeros are
+    kept unless ``prune_numerical_zeros``.
+    """
+    rows = torch.as_tensor(rows).to(torch.int64)
+    cols = torch.as_tensor(cols).to(torch.int64)
+    values = torch.as_tensor(values)
+    device = values.device
+    rows, cols = rows.to(device), cols.to(device)
+    block_shape = (1, 1) if values.dim() == 1 else tuple(values.shape[1:])
+    key = rows * int(ncol) + cols
+    order = torch.argsort(key, stable=True)
+    key = key[order]
+    uniq, inverse = torch.unique_consecutive(key, return_inverse=True)
+    summed = torch.zeros((uniq.numel(), *values.shape[1:]), dtype=values.dtype, device=device)
+    summed.index_add_(0, inverse, values[order])
+    if prune_numerical_zeros:
+        keep = summed.reshape(uniq.numel(), -1).ne(0).any(dim=1)
+        uniq, summed = uniq[keep], summed[keep]
+    r = uniq // int(ncol)
+    c = (uniq % int(ncol)).to(torch.int32)
+    counts = torch.bincount(r, minlength=int(nrow))
+    offsets = torch.zeros(int(nrow) + 1, dtype=torch.int32, device=device)
+    offsets[1:] = torch.cumsum(counts, 0).to(torch.int32)
+    return BsrMatrix(nrow, ncol, block_shape, offsets, c.contiguous(), summed.contiguous())
+
+
+def from_scipy(a, device=None, dtype=torch.float64, block_shape=(1, 1)) -> BsrMatrix:
+    """Upload a scipy sparse matrix (canonical BSR with the given block shape)."""
+    import scipy.sparse as ss
+
+    device = _device(device)
+    b0, b1 = block_shape
+    if (b0, b1) == (1, 1):
+        m = ss.csr_matrix(a, copy=True)
+        m.sum_duplicates()
+        m.sort_indices()
+        vals = np.asarray(m.data)
+        nrow, ncol = m.shape
+    else:
+        m = ss.bsr_matrix(a, blocksize=(b0, b1))
+        m.sum_duplicates()
+        m.sort_indices()
+        vals = np.asarray(m.data)
+        nrow, ncol = m.shape[0] // b0, m.shape[1] // b1
+    np_dtype = {torch.float16: np.float16, torch.float32: np.float32, torch.float64: np.float64}[dtype]
+    return BsrMatrix(
+        nrow, ncol, (b0, b1),
+        torch.from_numpy(np.ascontiguousarray(m.indptr, dtype=np.int32)).to(device),
+        torch.from_numpy(np.ascontiguousarray(m.indices, dtype=np.int32)).to(device),
+        torch.from_numpy(np.ascontiguousarray(vals.astype(np_dtype))).to(device),
+    )
+
+
+def scalarize(A: BsrMatrix) -> BsrMatrix:
+    """Expand square b x b blocks into scalar CSR on the device, keeping stored zeros
+    (fsai_pack.py:128-149)."""
+    b = A.block_shape[0]
+    if A.block_shape != (b, b):
+        raise ValueError("scalarize requires square blocks")
+    nat.require_device(A.values, "matrix")
+    n = A.nrow * b
+    count = A.nnz * b * b
+    dev = A.device
+    off = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    col = torch.empty(count, dtype=torch.int32, device=dev)
+    val = torch.empty(count, dtype=A.scalar_type, device=dev)
+    nat.call("wpc_bsr_scalarize", A.nrow, b, nat.ptr(A.offsets), nat.ptr(A.columns),
+             nat.ptr(A.values), nat.code(A.scalar_type), nat.ptr(off), nat.ptr(col), nat.ptr(val),
+             nat.stream())
+    return BsrMatrix(n, A.ncol * b, (1, 1), off, col, val)
+
+
+def bsr_copy(A: BsrMatrix, block_shape=None, scalar_type=None) -> BsrMatrix:
+    """Copy with optional reblocking to 1x1 and/or value conversion (Warp's bsr_copy)."""
+    out = A
+    if block_shape is not None and tuple(block_shape) != A.block_shape:
+        if tuple(block_shape) != (1, 1):
+            raise NotImplementedError("bsr_copy reblocks to scalar (1, 1) blocks only")
+        out = scalarize(A)
+    else:
+        out = BsrMatrix(A.nrow, A.ncol, A.block_shape, A.offsets.clone(), A.columns.clone(),
+                        A.values.clone())
+    if scalar_type is not None and scalar_type != out.scalar_type:
+        vals = torch.empty(out.values.shape, dtype=scalar_type, device=out.device)
+        nat.call("wpc_convert", out.values.numel(), nat.ptr(out.values), nat.code(out.scalar_type),
+                 nat.ptr(vals), nat.code(scalar_type), nat.stream())
+        out = BsrMatrix(out.nrow, out.ncol, out.block_shape, out.offsets, out.columns, vals)
+    return out
+
+
+def bsr_transposed(A: BsrMatrix) -> BsrMatrix:
+    """Transpose of a scalar CSR matrix with sorted rows (Warp's bsr_transposed)."""
+    if A.block_shape != (1, 1):
+        raise NotImplementedError("bsr_transposed supports scalar matrices")
+    dev = A.device
+    nnz = A.nnz
+    toff = torch.empty(A.ncol + 1, dtype=torch.int32, device=dev)
+    tcol = torch.empty(nnz, dtype=torch.int32, device=dev)
+    tval = torch.empty(nnz, dtype=A.scalar_type, device=dev)
+    work = torch.empty(A.ncol + 1, dtype=torch.int32, device=dev)
+    nat.call("wpc_csr_transpose", A.nrow, A.ncol, nnz, nat.ptr(A.offsets), nat.ptr(A.columns),
+             nat.ptr(A.values), A.values.element_size(), nat.ptr(toff), nat.ptr(tcol),
+             nat.ptr(tval), nat.ptr(work), nat.stream())
+    return BsrMatrix(A.ncol, A.nrow, (1, 1), toff, tcol, tval)
+
+
+def bsr_mv(A: BsrMatrix, x, y, alpha=1.0, beta=0.0, lanes=8):
+    """y = alpha * A x + beta * y (Warp's bsr_mv semantics; y updated in place)."""
+    xf = x.reshape(-1)
+    yf = y.reshape(-1)
+    if xf.data_ptr() == yf.data_ptr():
+        xf = xf.clone()
+    _product(A, xf, yf, yf, alpha, beta, lanes)
+    return y
+
+
+def _product(A: BsrMatrix, x, y, z, alpha, beta, lanes=8):
+    """z = alpha A x + beta y on flat vectors (x must not alias z)."""
+    b = A.block_shape[0]
+    if A.block_shape == (1, 1):
+        nat.call("wpc_csr_spmv", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(A.values),
+                 nat.code(A.scalar_type), None, nat.ptr(x), nat.ptr(y), nat.ptr(z),
+                 nat.code(x.dtype), 1, float(alpha), float(beta), lanes, nat.stream())
+    else:
+        if x.dtype != A.scalar_type:
+            raise TypeError("block products require vectors of the matrix scalar type")
+        nat.call("wpc_bsr_spmv", A.nrow, b, nat.ptr(A.offsets), nat.ptr(A.columns),
+                 nat.ptr(A.values), nat.code(A.scalar_type), nat.ptr(x), nat.ptr(y), nat.ptr(z),
+                 float(alpha), float(beta), nat.stream())

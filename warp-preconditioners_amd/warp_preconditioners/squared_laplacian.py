@@ -1,0 +1,112 @@
+"""Factored squared-Laplacian operator (drop-in for squared_laplacian.py).
+
+``SquaredLaplacianOperator(L, mass, free_indices, row_lanes, mass_weight)`` applies
+``(L diag(1/mass) L + mass_weight diag(mass))[free, free]`` as two sparse passes
+and builds the Dirichlet right-hand side with the same arithmetic
+(squared_laplacian.py:382-497).  All passes are ``wpc_sqlap_apply``.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _native as nat
+from .linear import LinearOperator
+from .sparse import BsrMatrix, bsr_copy
+
+LANES = (1, 2, 4, 8, 16, 32)
+
+
+class SquaredLaplacianOperator(LinearOperator):
+    """Apply ``(L diag(1/mass) L + mass_weight diag(mass))[free,free]``.
+
+    ``L`` is a symmetric scalar BSR matrix, ``mass`` positive lumped masses,
+    ``free_indices`` distinct free scalar indices (the rest are constrained).
+    The energy includes every row of L.  Instances own scratch; do not use one
+    instance on concurrent streams.
+    """
+
+    def __init__(self, L, mass, free_indices, row_lanes=1, mass_weight=0.0):
+        if not isinstance(L, BsrMatrix) or L.nrow != L.ncol or L.block_shape != (1, 1):
+            raise ValueError("L must be a square scalar BSR matrix")
+        if L.scalar_type not in (torch.float32, torch.float64):
+            raise TypeError("L must use float32 or float64")
+        if mass.device != L.device or free_indices.device != L.device:
+            raise ValueError("L, mass, and free_indices must be on the same device")
+        if mass.dtype != L.scalar_type or mass.dim() != 1 or mass.numel() != L.nrow:
+            raise ValueError("mass must match the scalar type and size of L")
+        if free_indices.dtype != torch.int32 or free_indices.dim() != 1:
+            raise ValueError("free_indices must be a one-dimensional int32 array")
+        if not isinstance(row_lanes, int) or row_lanes not in LANES:
+            raise ValueError("row_lanes must be one of 1, 2, 4, 8, 16, 32")
+        if not math.isfinite(mass_weight) or mass_weight < 0:
+            raise ValueError("mass_weight must be finite and nonnegative")
+        nat.require_device(L.values, "L")
+        self.mass_weight = float(mass_weight)
+        self.L = bsr_copy(L)
+        self.mass = mass
+        self.free = free_indices.clone()
+        n = L.nrow
+        dev = L.device
+        s = nat.stream()
+        self.index = torch.empty(n, dtype=torch.int32, device=dev)
+        nat.call("wpc_fill_i32", n, nat.ptr(self.index), -1, s)
+        self.inverse_mass = torch.empty_like(mass)
+        self.row_lanes = row_lanes
+        status = torch.zeros(3, dtype=torch.int32, device=dev)
+        nat.call("wpc_sqlap_index_free", self.free.numel(), nat.ptr(self.free), n,
+                 nat.ptr(self.index), nat.ptr(status), s)
+        nat.call("wpc_sqlap_mass_inverse", n, nat.ptr(mass), nat.ptr(self.inverse_mass),
+                 nat.code(L.scalar_type), nat.ptr(status), s)
+        errors = status.cpu()
+        if errors[0] or errors[1]:
+            raise ValueError("free_indices must be distinct and in bounds")
+        if errors[2]:
+            raise ValueError("mass entries must be finite and positive")
+        # Every row free in natural order: the index maps are the identity.
+        self.identity = self.free.numel() == n and bool(
+            torch.equal(self.free, torch.arange(n, dtype=torch.int32, device=dev)))
+        self._tmp = {}
+        nf = self.free.numel()
+        super().__init__((nf, nf), L.dtype, dev, self._apply)
+
+    def _scratch(self, dtype, count):
+        t = self._tmp.get(dtype)
+        if t is None or t.numel() < count:
+            t = torch.empty(count, dtype=dtype, device=self.device)
+            self._tmp[dtype] = t
+        return t
+
+    def product(self, x, y, z, alpha, beta, boundary=False, nvec=1):
+        L = self.L
+        x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+        tmp = self._scratch(x.dtype, L.nrow * nvec)
+        ident = self.identity and not boundary
+        nat.call("wpc_sqlap_apply", L.nrow, nat.ptr(L.offsets), nat.ptr(L.columns),
+                 nat.ptr(L.values), nat.code(L.scalar_type), nat.ptr(self.inverse_mass),
+                 None if ident else nat.ptr(self.index), self.free.numel(),
+                 None if ident else nat.ptr(self.free), nat.ptr(self.mass),
+                 0.0 if boundary else self.mass_weight, nat.ptr(tmp), nat.ptr(x), nat.ptr(y),
+                 nat.ptr(z), nvec, float(alpha), float(beta), int(boundary), self.row_lanes,
+                 nat.stream())
+
+    def _apply(self, x, y, z, alpha, beta):
+        self.product(x, y, z, alpha, beta, False)
+
+    def apply_batch(self, x, y, z, alpha, beta, nvec):
+        self.product(x, y, z, alpha, beta, False, nvec)
+
+    def rhs(self, boundary_values, out=None):
+        if (boundary_values.dtype != self.scalar_type or boundary_values.device != self.device
+                or boundary_values.dim() != 1 or boundary_values.numel() != self.L.nrow):
+            raise ValueError("boundary_values must match the scalar type, device, and size of L")
+        nf = self.free.numel()
+        if out is None:
+            out = torch.empty(nf, dtype=self.scalar_type, device=self.device)
+        elif (out.dtype != self.scalar_type or out.device != self.device or out.dim() != 1
+              or out.numel() != nf):
+            raise ValueError("out must match the reduced operator")
+        self.product(boundary_values, out, out, -1.0, 0.0, True)
+        return out
