@@ -1,0 +1,377 @@
+"""Headline benchmark: FSAI-preconditioned CG solves/sec at 1e-8 relative error.
+
+Workload (the factored-biharmonic configuration): a 64^3-vertex jittered Kuhn tet
+mesh of the unit cube (6 tets per cube, interior vertices jittered by +-0.15h,
+seed 0), P1 stiffness K and lumped mass M, operator alpha*M + K M^-1 K (alpha = 1)
+applied factored and never assembled, FSAI width 8 built on the assembled
+sparsity, three right-hand sides alpha*M*f (f = x, y, 0.5x + sin 3y + z^2 + xz,
+plus N(0, 1e-2^2) noise, seed 0), x0 = 0.  One step = one solve of all three
+right-hand sides to relative mass-weighted field error <= 1e-8 against a float64
+reference solve (computed once, untimed, and refined once); the solver stops on
+its recursive relative residual (--rtol, default 1e-5, which leaves a 10x margin
+on this workload) and every timed solve's field error is checked -- a step that
+misses 1e-8 makes the reported value 0.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode MODE]
+
+Prints one JSON line (rank 0).  The inputs are resident in HBM before the timed
+region starts; the timed region is K whole solves bracketed by device syncs.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (str(ROOT), str(ROOT / "warp-preconditioners_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(side=64, alpha=1.0):
+    from warp_preconditioners import workloads
+
+    t0 = time.perf_counter()
+    vertices, tets, _ = workloads.kuhn_grid(side, 0.15, seed=0)
+    K, mass = workloads.p1_laplacian(vertices, tets)
+    Q = workloads.squared_operator(K, mass, alpha)
+    B = workloads.biharmonic_rhs(vertices, mass, alpha, noise=1e-2, seed=0)
+    log(f"workload: n={len(vertices)} tets={len(tets)} nnzK={K.nnz} nnzQ={Q.nnz} "
+        f"({time.perf_counter() - t0:.1f}s host assembly)")
+    return dict(vertices=vertices, tets=tets, K=K, mass=mass, Q=Q, B=B, alpha=alpha)
+
+
+class Solver:
+    """Device-resident operator, preconditioner and solve policy for one mode."""
+
+    def __init__(self, wl, mode, width=8, lanes=8):
+        import torch
+
+        import warp_preconditioners as wp
+        from warp_preconditioners.sparse import from_scipy
+
+        dev = torch.device("cuda", 0)
+        self.torch = torch
+        self.mode = mode
+        self.n = wl["K"].shape[0]
+        self.k = wl["B"].shape[1]
+        alpha = wl["alpha"]
+        ident = torch.arange(self.n, dtype=torch.int32, device=dev)
+        self.K64 = from_scipy(wl["K"], dev, torch.float64)
+        self.m64 = torch.tensor(wl["mass"], dtype=torch.float64, device=dev)
+        self.op64 = wp.SquaredLaplacianOperator(self.K64, self.m64, ident, row_lanes=lanes,
+                                                mass_weight=alpha)
+        Q64 = from_scipy(wl["Q"], dev, torch.float64)
+        factor = {"plain-f64": torch.float32, "batched-f64": torch.float32,
+                  "plain-f32": torch.float32, "batched-f32": torch.float32,
+                  "batched-mixed": torch.float32, "batched-mixed-bf16": torch.bfloat16}[mode]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.P = wp.FSAI(Q64, max_row_size=width, factor_dtype=factor, apply_lanes=lanes)
+        torch.cuda.synchronize()
+        self.setup_s = time.perf_counter() - t0
+        del Q64
+        self.nnzG = self.P.G.nnz
+        if "mixed" in mode or "f32" in mode:
+            self.K32 = from_scipy(wl["K"], dev, torch.float32)
+            self.m32 = self.m64.float()
+            self.op32 = wp.SquaredLaplacianOperator(self.K32, self.m32, ident, row_lanes=lanes,
+                                                    mass_weight=alpha)
+        self.B = torch.tensor(np.ascontiguousarray(wl["B"]), dtype=torch.float64, device=dev)
+        self.X = torch.zeros_like(self.B)
+        self.work = None
+        self.inner_work = None
+        self.iterations = None
+
+    def solve(self, tol=1e-8, maxiter=40000, check_every=25):
+        from warp_preconditioners import linear
+
+        torch = self.torch
+        self.X.zero_()
+        if self.mode in ("plain-f64", "plain-f32"):
+            f32 = self.mode == "plain-f32"
+            op = self.op32 if f32 else self.op64
+            its = []
+            if self.work is None:
+                self.work = [None] * self.k
+                self.Bs = [self.B[:, v].contiguous().to(torch.float32 if f32 else torch.float64)
+                           for v in range(self.k)]
+                self.Xs = [torch.zeros_like(b) for b in self.Bs]
+            for v in range(self.k):
+                x = self.Xs[v]
+                x.zero_()
+                out = linear.solve_batched("cg", op, self.Bs[v][:, None], x[:, None], M=self.P,
+                                           tol=tol, maxiter=maxiter, check_every=check_every,
+                                           work=self.work[v])
+                self.work[v] = out["work"]
+                self.X[:, v] = x.double()
+                its += out["iterations"]
+            self.iterations = its
+            return
+        if self.mode in ("batched-f64", "batched-f32"):
+            f32 = self.mode == "batched-f32"
+            if f32:
+                if self.work is None:
+                    self.B32 = self.B.float().contiguous()
+                    self.X32 = torch.zeros_like(self.B32)
+                self.X32.zero_()
+                out = linear.solve_batched("cg", self.op32, self.B32, self.X32, M=self.P, tol=tol,
+                                           maxiter=maxiter, check_every=check_every, work=self.work)
+                self.X.copy_(self.X32)
+            else:
+                out = linear.solve_batched("cg", self.op64, self.B, self.X, M=self.P, tol=tol,
+                                           maxiter=maxiter, check_every=check_every, work=self.work)
+            self.work = out["work"]
+            self.iterations = out["iterations"]
+            return
+        # mixed: float32 Krylov on the residual, float64 true residual and update
+        bnorm = torch.linalg.vector_norm(self.B, dim=0)
+        R = self.B.clone()
+        its = [0] * self.k
+        C = torch.zeros(self.B.shape, dtype=torch.float32, device=self.B.device)
+        for outer in range(30):
+            self.op64.apply_batch(self.X, self.B, R, -1.0, 1.0, self.k)
+            rn = torch.linalg.vector_norm(R, dim=0)
+            if bool((rn <= tol * bnorm).all()):
+                break
+            scale = torch.where(rn > 0, rn, torch.ones_like(rn))
+            R32 = (R / scale).float().contiguous()
+            C.zero_()
+            out = linear.solve_batched("cg", self.op32, R32, C, M=self.P, tol=self.inner_tol,
+                                       maxiter=maxiter, check_every=check_every,
+                                       work=self.inner_work)
+            self.inner_work = out["work"]
+            its = [a + b for a, b in zip(its, out["iterations"])]
+            active = (rn > tol * bnorm).double()
+            self.X += C.double() * (scale * active)
+        self.iterations = its
+
+    inner_tol = 1e-3
+
+
+def true_residual(s, X=None):
+    torch = s.torch
+    X = s.X if X is None else X
+    R = torch.empty_like(s.B)
+    s.op64.apply_batch(X, s.B, R, -1.0, 1.0, s.k)
+    return (torch.linalg.vector_norm(R, dim=0) / torch.linalg.vector_norm(s.B, dim=0)).cpu().numpy()
+
+
+def reference_solution(s, tol=1e-12):
+    """Float64 reference: batched CG to a tight recursive tolerance, then one
+    restarted correction from the true residual; returns (X_ref, change) where
+    change is the mass-weighted size of that correction (a bound on its error)."""
+    from warp_preconditioners import linear
+
+    torch = s.torch
+    X = torch.zeros_like(s.B)
+    out = linear.solve_batched("cg", s.op64, s.B, X, M=s.P, tol=tol, maxiter=200000,
+                               check_every=100)
+    R = torch.empty_like(s.B)
+    s.op64.apply_batch(X, s.B, R, -1.0, 1.0, s.k)
+    C = torch.zeros_like(R)
+    linear.solve_batched("cg", s.op64, R, C, M=s.P, tol=1e-6, maxiter=200000, check_every=100)
+    X += C
+    change = field_error(s, X - C, X)
+    return X, change, out["iterations"]
+
+
+def field_error(s, X, Xref):
+    """Relative mass-weighted field error per right-hand side."""
+    torch = s.torch
+    m = s.m64[:, None]
+    num = torch.sqrt((m * (X - Xref) ** 2).sum(0))
+    den = torch.sqrt((m * Xref ** 2).sum(0))
+    return (num / den).cpu().numpy()
+
+
+def kernel_times(s, reps=30):
+    """Average duration of each apply kernel (HIP events on the launch stream)."""
+    import torch
+
+    from warp_preconditioners import _native as nat
+
+    k, n = s.k, s.n
+    dt = torch.float32 if "mixed" in s.mode else torch.float64
+    X = torch.randn(n * k, dtype=dt, device="cuda:0")
+    Y = torch.empty_like(X)
+    T = torch.empty_like(X)
+    P = s.P
+    op = s.op32 if "mixed" in s.mode else s.op64
+    L = op.L
+    code = nat.code(dt)
+    stream = torch.cuda.current_stream()
+
+    def g_pass():
+        nat.call("wpc_csr_spmv", n, nat.ptr(P.G.offsets), nat.ptr(P.G.columns),
+                 nat.ptr(P.G.values), nat.code(P.G.scalar_type), nat.ptr(P.row_scale2),
+                 nat.ptr(X), None, nat.ptr(T), code, k, 1.0, 0.0, P.apply_lanes, stream.cuda_stream)
+
+    def gt_pass():
+        nat.call("wpc_csr_spmv", n, nat.ptr(P.GT.offsets), nat.ptr(P.GT.columns),
+                 nat.ptr(P.GT.values), nat.code(P.GT.scalar_type), None, nat.ptr(T), None,
+                 nat.ptr(Y), code, k, 1.0, 0.0, P.apply_lanes, stream.cuda_stream)
+
+    def sq_forward():
+        nat.call("wpc_sqlap_apply", n, nat.ptr(L.offsets), nat.ptr(L.columns), nat.ptr(L.values),
+                 nat.code(L.scalar_type), nat.ptr(op.inverse_mass), None, 0, None,
+                 nat.ptr(op.mass), op.mass_weight, nat.ptr(T), nat.ptr(X), nat.ptr(Y), nat.ptr(Y),
+                 k, 1.0, 0.0, 0, op.row_lanes, stream.cuda_stream)
+
+    def sq_backward():
+        nat.call("wpc_sqlap_apply", 0, nat.ptr(L.offsets), nat.ptr(L.columns), nat.ptr(L.values),
+                 nat.code(L.scalar_type), nat.ptr(op.inverse_mass), None, n, None,
+                 nat.ptr(op.mass), op.mass_weight, nat.ptr(T), nat.ptr(X), nat.ptr(Y), nat.ptr(Y),
+                 k, 1.0, 0.0, 0, op.row_lanes, stream.cuda_stream)
+
+    out = {}
+    for name, fn in [("fsai_G_pass", g_pass), ("fsai_GT_pass", gt_pass),
+                     ("sqlap_forward", sq_forward), ("sqlap_backward", sq_backward)]:
+        for _ in range(3):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps * 1e-3  # seconds
+    xb = X.element_size()
+    vb_g = P.G.values.element_size()
+    vb_k = L.values.element_size()
+    nnzG, nnzK = P.G.nnz, L.nnz
+    scale_b = 4 * n if P.row_scale2 is not None else 0
+    bytes_ = {
+        "fsai_G_pass": nnzG * (vb_g + 4) + 4 * (n + 1) + 2 * n * k * xb + scale_b,
+        "fsai_GT_pass": nnzG * (vb_g + 4) + 4 * (n + 1) + 2 * n * k * xb,
+        "sqlap_forward": nnzK * (vb_k + 4) + 4 * (n + 1) + 2 * n * k * xb + n * vb_k,
+        "sqlap_backward": nnzK * (vb_k + 4) + 4 * (n + 1) + 3 * n * k * xb + n * vb_k,
+    }
+    return out, bytes_
+
+
+def cpu_baseline(wl, s, iters_needed, sample_iters=40):
+    """Oracle PCG (float64, factored operator, same FSAI factor) on the host, one
+    thread, timed over a bounded number of iterations and scaled to solves/s."""
+    from oracle import cpu
+
+    K = cpu.CSR.from_scipy(wl["K"])
+    G = cpu.CSR(s.P.G.offsets.cpu().numpy(), s.P.G.columns.cpu().numpy(),
+                s.P.G.values.double().cpu().numpy(), s.n, s.n)
+    GT = cpu.CSR(s.P.GT.offsets.cpu().numpy(), s.P.GT.columns.cpu().numpy(),
+                 s.P.GT.values.double().cpu().numpy(), s.n, s.n)
+    b = np.ascontiguousarray(wl["B"][:, 2])
+    t0 = time.perf_counter()
+    cpu.krylov("cg", b, K=K, mass=wl["mass"], weight=wl["alpha"], P=dict(G=G, GT=GT), tol=0.0,
+               maxiter=sample_iters, check_every=sample_iters)
+    dt = time.perf_counter() - t0
+    per_iter = dt / sample_iters
+    return dict(value=1.0 / (per_iter * iters_needed), unit="solves/s", cores=1, kind="port",
+                sample=f"{sample_iters} PCG iterations of one RHS ({dt:.1f}s; {per_iter * 1e3:.1f} ms/iter), "
+                       f"scaled by {iters_needed} iterations per solve; same FSAI factor")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="batched-f64",
+                    choices=["plain-f64", "plain-f32", "batched-f64", "batched-f32",
+                             "batched-mixed", "batched-mixed-bf16"])
+    ap.add_argument("--rtol", type=float, default=1e-5,
+                    help="recursive relative-residual stopping tolerance")
+    ap.add_argument("--compare", action="store_true", help="also time the plain one-RHS path")
+    ap.add_argument("--side", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every run is on one "
+              "MI355X); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    wl = build_workload(args.side)
+    s = Solver(wl, args.mode)
+    log(f"mode {args.mode}: FSAI setup {s.setup_s:.3f}s nnzG={s.nnzG}")
+    t0 = time.perf_counter()
+    Xref, ref_change, ref_iters = reference_solution(s)
+    log(f"float64 reference: {ref_iters} iterations, correction {ref_change} "
+        f"({time.perf_counter() - t0:.1f}s)")
+    for w in range(args.warmup):
+        s.solve(tol=args.rtol)
+        log(f"warmup {w}: iterations {s.iterations} field error {field_error(s, s.X, Xref)}")
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        s.solve(tol=args.rtol)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+        log(f"step {k}: {lat[-1] * 1e3:.1f} ms iterations {s.iterations}")
+    elapsed = time.perf_counter() - t0
+    rel = true_residual(s)
+    ferr = field_error(s, s.X, Xref)
+    ok = bool((ferr <= 1e-8).all())
+    value = s.k * args.steps / elapsed
+    times, bytes_ = kernel_times(s)
+    dominant = max(times, key=lambda k: times[k])
+    achieved = bytes_[dominant] / times[dominant] / 1e9
+    record = {
+        "metric": "CG solves/sec at 1e-8 rel. error",
+        "value": value if ok else 0.0,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if "f64" in args.mode else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"factored biharmonic (M + K M^-1 K), {args.side}^3 jittered Kuhn "
+                               f"tet mesh, FSAI W8 on assembled Q, {s.k} RHS, mode {args.mode}",
+                   "n": s.n, "nnzK": int(wl["K"].nnz), "nnzQ": int(wl["Q"].nnz), "nnzG": s.nnzG},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes": bytes_[dominant],
+                     "avg_us": times[dominant] * 1e6},
+        "kernels_us": {k: v * 1e6 for k, v in times.items()},
+        "kernels_GBs": {k: bytes_[k] / times[k] / 1e9 for k, v in times.items()},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "iterations": s.iterations,
+        "rtol": args.rtol,
+        "field_error": [float(e) for e in ferr],
+        "true_relative_residual": [float(r) for r in rel],
+        "reference_correction": [float(e) for e in ref_change],
+        "converged": ok,
+        "setup_s": s.setup_s,
+    }
+    if not args.no_cpu:
+        try:
+            record["cpu_baseline"] = cpu_baseline(wl, s, max(s.iterations))
+        except Exception as exc:  # the baseline is reported, never required
+            record["cpu_baseline"] = {"error": str(exc)}
+    print(json.dumps(record), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -249,7 +249,8 @@ def test_row_scaled_narrow_factor(wp, factor, vdt):
     gq = pre.dense_factor()
     np.testing.assert_array_equal(host(pre.G.columns), host(ref.G.columns))
     rel = 4e-3 if factor == torch.float16 else 2e-2
-    np.testing.assert_allclose(gq, g, rtol=rel, atol=rel * np.abs(g).max(axis=1, keepdims=True))
+    row_max = np.abs(g).max(axis=1, keepdims=True)
+    assert np.all(np.abs(gq - g) <= rel * row_max)
     x = torch.tensor(rng.normal(size=300), dtype=vdt, device=dev())
     z = torch.empty_like(x)
     pre.matvec(x, x, z, 1.0, 0.0)

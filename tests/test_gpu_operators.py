@@ -133,10 +133,11 @@ def test_factored_dirichlet_operator_and_rhs(wp, oracle, dtype, lanes, mass_weig
     z = torch.empty_like(x)
     A.matvec(x, y, z, 1.0, 0.0)
     np.testing.assert_allclose(host(z), reduced @ np.ones(len(free)), rtol=rtol, atol=atol)
+    bcw = vec(bc, dtype)
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         A.matvec(x, y, z, 1.0, 0.0)
-        A.rhs(vec(bc, dtype), out=result)
+        A.rhs(bcw, out=result)
     for _ in range(2):
         z.fill_(0.0)
         result.fill_(0.0)

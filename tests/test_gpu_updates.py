@@ -145,5 +145,4 @@ def test_scaled_factor_refit(wp, factor):
     full.update(upload(b))
     g = dense(full.G)
     rel = 4e-3 if factor == torch.float16 else 2e-2
-    np.testing.assert_allclose(pre.dense_factor(), g, rtol=rel,
-                               atol=rel * np.abs(g).max(axis=1, keepdims=True))
+    assert np.all(np.abs(pre.dense_factor() - g) <= rel * np.abs(g).max(axis=1, keepdims=True))

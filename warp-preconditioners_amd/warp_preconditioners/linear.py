@@ -56,6 +56,17 @@ class _MatrixOperator(LinearOperator):
             xf = self._alias
         _product(self.A, xf, yf, zf, alpha, beta, self.lanes)
 
+    def apply_batch(self, x, y, z, alpha, beta, nvec):
+        if self.A.block_shape != (1, 1):
+            raise NotImplementedError("batched products need a scalar matrix")
+        xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+        if xf.data_ptr() == zf.data_ptr():
+            xf = xf.clone()
+        A = self.A
+        nat.call("wpc_csr_spmv", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(A.values),
+                 nat.code(A.scalar_type), None, nat.ptr(xf), nat.ptr(yf), nat.ptr(zf),
+                 nat.code(xf.dtype), nvec, float(alpha), float(beta), self.lanes, nat.stream())
+
 
 def aslinearoperator(A) -> LinearOperator:
     if A is None or isinstance(A, LinearOperator):
@@ -112,16 +123,22 @@ class KrylovWork:
         nat.call("wpc_dot", self.n, self.nvec, nat.ptr(a), nat.ptr(b), self.code,
                  nat.ptr(self.dot_parts), nat.ptr(out), nat.stream())
 
+    def _op(self, op, x, y, z, alpha, beta):
+        if self.nvec == 1:
+            op.matvec(x, y, z, alpha, beta)
+        else:
+            op.apply_batch(x, y, z, alpha, beta, self.nvec)
+
     def _apply_M(self, src, dst):
         if self.M is None:
             if dst.data_ptr() != src.data_ptr():
                 dst.copy_(src)
         else:
-            self.M.matvec(src, dst, dst, 1.0, 0.0)
+            self._op(self.M, src, dst, dst, 1.0, 0.0)
 
     def start(self, b, x, tol, atol):
         """r = b - A x; initial direction; device-side target."""
-        self.A.matvec(x, b, self.r, -1.0, 1.0)
+        self._op(self.A, x, b, self.r, -1.0, 1.0)
         self._dot(b, b, self._field(2))  # ||b||^2 into TARGET
         nat.call("wpc_krylov_set_target", self.nvec, nat.ptr(self.state), float(tol), float(atol),
                  nat.stream())
@@ -131,7 +148,7 @@ class KrylovWork:
             self._dot(self.r, self.z, self._field(0))
             self.p.copy_(self.z)
         else:
-            self.A.matvec(self.z, self.az, self.az, 1.0, 0.0)
+            self._op(self.A, self.z, self.az, self.az, 1.0, 0.0)
             self._dot(self.z, self.az, self._field(0))
             self.p.copy_(self.z)
             self.ap.copy_(self.az)
@@ -141,7 +158,7 @@ class KrylovWork:
         pd, pr, pz = (nat.ptr(self.parts[k]) for k in range(3))
         st = nat.ptr(self.state)
         if self.method == 0:
-            self.A.matvec(self.p, self.ap, self.ap, 1.0, 0.0)
+            self._op(self.A, self.p, self.ap, self.ap, 1.0, 0.0)
             nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.p), nat.ptr(self.ap),
                      self.code, pd, s)
             nat.call("wpc_krylov_update", 0, self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
@@ -158,7 +175,7 @@ class KrylovWork:
             nat.call("wpc_krylov_update", 1, self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
                      nat.ptr(self.p), nat.ptr(self.ap), nat.ptr(self.z), nat.ptr(self.map),
                      self.code, st, pd, pr, s)
-            self.A.matvec(self.z, self.az, self.az, 1.0, 0.0)
+            self._op(self.A, self.z, self.az, self.az, 1.0, 0.0)
             nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.az),
                      self.code, pz, s)
             nat.call("wpc_krylov_direction", 1, self.n, self.nvec, nat.ptr(self.z),
@@ -246,3 +263,33 @@ def cr(A, b, x, tol=None, atol=None, maxiter=0, M=None, callback=None, check_eve
        use_cuda_graph=True):
     """Preconditioned conjugate residuals; returns (iterations, residual, atol)."""
     return _solve("cr", A, b, x, tol, atol, maxiter, M, callback, check_every, use_cuda_graph)
+
+
+def solve_batched(method, A, B, X, M=None, tol=1.0e-8, atol=0.0, maxiter=10000, check_every=25,
+                  use_cuda_graph=True, work=None):
+    """CG/CR on k right-hand sides at once: B and X are contiguous [n, k] device arrays.
+
+    Every product streams the matrix once for all k vectors.  Each RHS stops on
+    its own target (a converged slot takes no further steps).  Returns a dict with
+    per-RHS ``iterations`` (steps taken), ``residuals`` (recursive ||r||) and the
+    number of ``chunks`` of ``check_every`` steps that ran.
+    """
+    A = aslinearoperator(A)
+    M = aslinearoperator(M)
+    n, k = B.shape
+    if tuple(X.shape) != (n, k) or not X.is_contiguous() or not B.is_contiguous():
+        raise ValueError("B and X must be contiguous [n, k] arrays")
+    if work is None or work.nvec != k or work.n != n:
+        work = KrylovWork(method, A, M, n, k, B.dtype, B.device)
+    work.start(B.reshape(-1), X.reshape(-1), tol, atol)
+    st = work.host_state()
+    chunks = 0
+    iters = 0
+    while iters < maxiter and bool((st[1] > st[2]).any()):
+        steps = min(check_every, maxiter - iters)
+        work.run(X.reshape(-1), steps, use_cuda_graph and steps == check_every and chunks > 0)
+        iters += steps
+        chunks += 1
+        st = work.host_state()
+    return dict(iterations=[int(v) for v in st[4]], residuals=[float(v) ** 0.5 for v in st[1]],
+                targets=[float(v) ** 0.5 for v in st[2]], chunks=chunks, work=work)

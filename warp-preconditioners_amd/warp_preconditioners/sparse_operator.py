@@ -54,3 +54,10 @@ class SparseOperator(LinearOperator):
             self._alias.copy_(x)
             x = self._alias
         _matvec(self.scalar, x, y, z, alpha, beta, self.row_lanes)
+
+    def apply_batch(self, x, y, z, alpha, beta, nvec):
+        """z = alpha A x + beta y on nvec interleaved vectors."""
+        x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+        if x.data_ptr() == z.data_ptr():
+            x = x.clone()
+        _matvec(self.scalar, x, y, z, alpha, beta, self.row_lanes, nvec=nvec)
