@@ -58,7 +58,7 @@ def build_workload(side=64, alpha=1.0):
 class Solver:
     """Device-resident operator, preconditioner and solve policy for one mode."""
 
-    def __init__(self, wl, mode, width=8, lanes=8):
+    def __init__(self, wl, mode, width=8, lanes=1):
         import torch
 
         import warp_preconditioners as wp
@@ -200,47 +200,45 @@ def field_error(s, X, Xref):
 
 
 def kernel_times(s, reps=30):
-    """Average duration of each apply kernel (HIP events on the launch stream)."""
+    """Average duration of each kernel of one CG iteration, timed with HIP events on
+    the stream the kernels are launched on (torch's current stream)."""
     import torch
 
     from warp_preconditioners import _native as nat
 
     k, n = s.k, s.n
-    dt = torch.float32 if "mixed" in s.mode else torch.float64
+    dt = torch.float32 if ("mixed" in s.mode or "f32" in s.mode) else torch.float64
     X = torch.randn(n * k, dtype=dt, device="cuda:0")
     Y = torch.empty_like(X)
     T = torch.empty_like(X)
     P = s.P
-    op = s.op32 if "mixed" in s.mode else s.op64
+    op = s.op32 if dt == torch.float32 else s.op64
     L = op.L
+    pg, pgt = P.plans()
+    pl = op._stream()
+    state = torch.ones(12 * k, dtype=torch.float64, device="cuda:0")
+    npart = nat.lib().wpc_krylov_partials(n)
+    parts = torch.zeros(npart * k, dtype=torch.float64, device="cuda:0")
     code = nat.code(dt)
     stream = torch.cuda.current_stream()
-
-    def g_pass():
-        nat.call("wpc_csr_spmv", n, nat.ptr(P.G.offsets), nat.ptr(P.G.columns),
-                 nat.ptr(P.G.values), nat.code(P.G.scalar_type), nat.ptr(P.row_scale2),
-                 nat.ptr(X), None, nat.ptr(T), code, k, 1.0, 0.0, P.apply_lanes, stream.cuda_stream)
-
-    def gt_pass():
-        nat.call("wpc_csr_spmv", n, nat.ptr(P.GT.offsets), nat.ptr(P.GT.columns),
-                 nat.ptr(P.GT.values), nat.code(P.GT.scalar_type), None, nat.ptr(T), None,
-                 nat.ptr(Y), code, k, 1.0, 0.0, P.apply_lanes, stream.cuda_stream)
-
-    def sq_forward():
-        nat.call("wpc_sqlap_apply", n, nat.ptr(L.offsets), nat.ptr(L.columns), nat.ptr(L.values),
-                 nat.code(L.scalar_type), nat.ptr(op.inverse_mass), None, 0, None,
-                 nat.ptr(op.mass), op.mass_weight, nat.ptr(T), nat.ptr(X), nat.ptr(Y), nat.ptr(Y),
-                 k, 1.0, 0.0, 0, op.row_lanes, stream.cuda_stream)
-
-    def sq_backward():
-        nat.call("wpc_sqlap_apply", 0, nat.ptr(L.offsets), nat.ptr(L.columns), nat.ptr(L.values),
-                 nat.code(L.scalar_type), nat.ptr(op.inverse_mass), None, n, None,
-                 nat.ptr(op.mass), op.mass_weight, nat.ptr(T), nat.ptr(X), nat.ptr(Y), nat.ptr(Y),
-                 k, 1.0, 0.0, 0, op.row_lanes, stream.cuda_stream)
-
+    kernels = {
+        "fsai_G_pass": lambda: pg.run(X, None, T, 1.0, 0.0, k, s=P.row_scale2),
+        "fsai_GT_pass_dot": lambda: pgt.run(T, None, Y, 1.0, 0.0, k, w=X),
+        "sqlap_forward": lambda: pl.run(X, None, T, 1.0, 0.0, k, s=op.inverse_mass),
+        "sqlap_backward_dot": lambda: pl.run(T, None, Y, 1.0, 0.0, k, d=op._wm, u=X, w=X),
+        "cg_alpha": lambda: nat.call("wpc_cg_alpha", k, pl.nblocks, nat.ptr(pl.partials(k)),
+                                     nat.ptr(state), stream.cuda_stream),
+        "cg_update": lambda: nat.call("wpc_cg_update", n, k, nat.ptr(X), nat.ptr(Y), nat.ptr(T),
+                                      nat.ptr(T), code, nat.ptr(state), nat.ptr(parts),
+                                      stream.cuda_stream),
+        "cg_beta": lambda: nat.call("wpc_cg_beta", k, pgt.nblocks, nat.ptr(pgt.partials(k)),
+                                    npart, nat.ptr(parts), nat.ptr(state), stream.cuda_stream),
+        "cg_direction": lambda: nat.call("wpc_cg_direction", n, k, nat.ptr(T), nat.ptr(Y), code,
+                                         nat.ptr(state), stream.cuda_stream),
+    }
     out = {}
-    for name, fn in [("fsai_G_pass", g_pass), ("fsai_GT_pass", gt_pass),
-                     ("sqlap_forward", sq_forward), ("sqlap_backward", sq_backward)]:
+    tuned = {"G": pg.mode(k, dt), "GT": pgt.mode(k, dt), "K": pl.mode(k, dt)}
+    for name, fn in kernels.items():
         for _ in range(3):
             fn()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -256,13 +254,19 @@ def kernel_times(s, reps=30):
     vb_k = L.values.element_size()
     nnzG, nnzK = P.G.nnz, L.nnz
     scale_b = 4 * n if P.row_scale2 is not None else 0
+    idx = 4 * (n + 1) + 4 * (pg.nblocks + 1)  # offsets + row partition
     bytes_ = {
-        "fsai_G_pass": nnzG * (vb_g + 4) + 4 * (n + 1) + 2 * n * k * xb + scale_b,
-        "fsai_GT_pass": nnzG * (vb_g + 4) + 4 * (n + 1) + 2 * n * k * xb,
-        "sqlap_forward": nnzK * (vb_k + 4) + 4 * (n + 1) + 2 * n * k * xb + n * vb_k,
-        "sqlap_backward": nnzK * (vb_k + 4) + 4 * (n + 1) + 3 * n * k * xb + n * vb_k,
+        # values + columns streamed once, x gathered once, z written once (+ extras)
+        "fsai_G_pass": nnzG * (vb_g + 4) + idx + 2 * n * k * xb + scale_b,
+        "fsai_GT_pass_dot": nnzG * (vb_g + 4) + idx + 3 * n * k * xb,
+        "sqlap_forward": nnzK * (vb_k + 4) + idx + 2 * n * k * xb + n * vb_k,
+        "sqlap_backward_dot": nnzK * (vb_k + 4) + idx + 3 * n * k * xb + n * vb_k,
+        "cg_alpha": 8 * pl.nblocks * k,
+        "cg_update": 6 * n * k * xb,
+        "cg_beta": 8 * (pgt.nblocks + npart) * k,
+        "cg_direction": 3 * n * k * xb,
     }
-    return out, bytes_
+    return out, bytes_, tuned
 
 
 def cpu_baseline(wl, s, iters_needed, sample_iters=40):
@@ -330,7 +334,7 @@ def main():
     ferr = field_error(s, s.X, Xref)
     ok = bool((ferr <= 1e-8).all())
     value = s.k * args.steps / elapsed
-    times, bytes_ = kernel_times(s)
+    times, bytes_, tuned = kernel_times(s)
     dominant = max(times, key=lambda k: times[k])
     achieved = bytes_[dominant] / times[dominant] / 1e9
     record = {
@@ -355,6 +359,7 @@ def main():
                      "avg_us": times[dominant] * 1e6},
         "kernels_us": {k: v * 1e6 for k, v in times.items()},
         "kernels_GBs": {k: bytes_[k] / times[k] / 1e9 for k, v in times.items()},
+        "row_mapping": {k: ("streamed" if m == 0 else f"{m} lanes/row") for k, m in tuned.items()},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "iterations": s.iterations,

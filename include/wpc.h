@@ -74,6 +74,35 @@ int wpc_fsai_apply(int n, const int* g_offsets, const int* g_columns, const void
                    const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
                    int lanes, void* stream);
 
+/* Row partition for wpc_csr_stream (host function, no device work): rows are cut
+ * into blocks of at most 256 rows and 2048 stored entries, a longer row gets a
+ * block of its own.  host_blocks receives count+1 row boundaries; returns count
+ * (or WPC_ESCRATCH if capacity is too small). */
+int wpc_csr_row_blocks(int n, const int* host_offsets, int* host_blocks, int capacity);
+
+/* Sparse product with a fused epilogue, in one of two row mappings:
+ *   lanes == 0  streamed: each workgroup loads its row block's (from `blocks`,
+ *               see wpc_csr_row_blocks) column/value range into LDS with
+ *               coalesced loads and one lane per row sums the row serially in
+ *               storage order (the order of sparse_operator.py:64-87 and of
+ *               Warp's CSR bsr_mv);
+ *   lanes = 2^k (1..64) lanes own one row and stride over it (the grouped
+ *               kernel of sparse_operator.py:10-40); blocks may be NULL.
+ * Computes  z = alpha * (s .* (A x) + d .* u) + beta * y  with optional row
+ * scale s (float32 or vec_dtype), optional diagonal term d .* u (d: vec_dtype[n],
+ * u: like x); when w is non-null, also one partial per (workgroup, vector) of
+ * sum_i w[i,v] z[i,v] into partials (double[workgroups * nvec]; workgroups =
+ * nblocks when streamed, ceil(n*lanes/256) otherwise), folded in workgroup order
+ * into out[v] by a single-workgroup launch when out is non-null.  alpha == 0
+ * never reads A, x or u; beta == 0 never reads y; x must not alias z.  Replaces
+ * sparse_operator.py:90-104 (_matvec), the two passes of fsai.py:374-381 and of
+ * squared_laplacian.py:316-377 (identity free map). */
+int wpc_csr_stream(int nblocks, const int* blocks, int n, int lanes, const int* offsets,
+                   const int* columns, const void* values, int val_dtype, const void* s,
+                   int s_dtype, const void* d, const void* u, const void* x, const void* y,
+                   void* z, int vec_dtype, int nvec, double alpha, double beta, const void* w,
+                   double* partials, double* out, void* stream);
+
 /* ------------------------------------------------------------ FSAI setup */
 
 /* scale[i] = 1/sqrt(A_ii); non-positive or non-finite diagonals set 0 and add 1
@@ -286,8 +315,11 @@ enum {
     WPC_KS_ITER = 4,    /* steps taken */
     WPC_KS_DENOM = 5,   /* p.Ap (CG) or Ap.MAp (CR) */
     WPC_KS_RZ_OLD = 6,
-    WPC_KS_ACTIVE = 7,
-    WPC_KSTATE = 8
+    WPC_KS_ACTIVE = 7,  /* the current step moves this slot */
+    WPC_KS_RR_NEW = 8,  /* r.r after the update (fused CG) */
+    WPC_KS_RZ_NEW = 9,  /* r.z of the new preconditioned residual (fused CG) */
+    WPC_KS_BETA = 10,
+    WPC_KSTATE = 12
 };
 
 /* Number of blocks P the Krylov kernels use for n rows; each partials region is
@@ -312,6 +344,24 @@ int wpc_krylov_direction(int method, int n, int nvec, const void* z, void* p, co
 
 /* state[TARGET] holds ||b||^2 on entry; becomes max(atol, tol*||b||)^2; ITER = 0. */
 int wpc_krylov_set_target(int nvec, double* state, double tol, double atol, void* stream);
+
+/* Fused CG (no host round trip; every reduction folded in a fixed order):
+ *   A p              operator whose last pass leaves p.Ap partials
+ *                    (wpc_csr_stream with w = p, out = NULL)
+ *   wpc_cg_alpha     DENOM = fold, ACTIVE = RR > TARGET, ALPHA = RZ/DENOM or 0
+ *   wpc_cg_update    x += ALPHA p, r -= ALPHA Ap, r.r partials
+ *   M r              preconditioner whose last pass leaves r.z partials
+ *   wpc_cg_beta      RZ_NEW, RR_NEW folded; stepping slots: BETA = RZ_NEW/RZ,
+ *                    RZ <- RZ_NEW, RR <- RR_NEW, ITER += 1
+ *   wpc_cg_direction p = z + BETA p for stepping slots.
+ * update partials: double[wpc_krylov_partials(n) * nvec]. */
+int wpc_cg_alpha(int nvec, int nparts, const double* part_denom, double* state, void* stream);
+int wpc_cg_update(int n, int nvec, void* x, void* r, const void* p, const void* ap, int dtype,
+                  const double* state, double* partials, void* stream);
+int wpc_cg_beta(int nvec, int nparts_rz, const double* part_rz, int nparts_rr,
+                const double* part_rr, double* state, void* stream);
+int wpc_cg_direction(int n, int nvec, const void* z, void* p, int dtype, const double* state,
+                     void* stream);
 
 #ifdef __cplusplus
 }

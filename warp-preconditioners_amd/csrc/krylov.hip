@@ -7,11 +7,11 @@
 // A slot whose incoming residual already meets its target takes no step, which
 // lets a captured chunk of iterations run past convergence unchanged.
 
-#include "wpc_common.h"
+#include "reduce.h"
 
 namespace wpc {
 
-constexpr int kKrylovThreads = 256;
+constexpr int kKrylovThreads = kStreamRows;
 
 int krylov_blocks(int n) {
     long b = ((long)n + 511) / 512;
@@ -166,6 +166,114 @@ __global__ __launch_bounds__(kKrylovThreads) void kdirection_kernel(
     }
 }
 
+
+// ---------------------------------------------------------------- fused CG steps
+// The operator's last pass leaves p.Ap partials and the preconditioner's last
+// pass r.z partials (wpc_csr_stream with a dot and out = NULL); two
+// single-workgroup scalar launches fold them, so a CG iteration is
+//   A p (+partials) | scalars_alpha | update (+r.r partials) | M r (+partials) |
+//   scalars_beta | direction.
+
+// DENOM = fold(p.Ap); ACTIVE = RR > TARGET; ALPHA = ACTIVE ? RZ / DENOM : 0.
+__global__ __launch_bounds__(1024) void cg_alpha_kernel(
+    int nvec, int P, const double* __restrict__ part_denom, double* __restrict__ state) {
+    __shared__ double sums[8];
+    fold_all<1024>(part_denom, P, nvec, sums);
+    const int v = threadIdx.x;
+    {
+        if (v < nvec) {
+            const double denom = sums[v];
+            const bool active = state[WPC_KS_RR * nvec + v] > state[WPC_KS_TARGET * nvec + v];
+            state[WPC_KS_DENOM * nvec + v] = denom;
+            state[WPC_KS_ACTIVE * nvec + v] = active ? 1.0 : 0.0;
+            state[WPC_KS_ALPHA * nvec + v] = active ? state[WPC_KS_RZ * nvec + v] / denom : 0.0;
+        }
+    }
+}
+
+// x += ALPHA p;  r -= ALPHA Ap;  partials of r.r.
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void cg_update_kernel(
+    int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+    const T* __restrict__ ap, const double* __restrict__ state, double* __restrict__ partials) {
+    __shared__ double red[kKrylovThreads];
+    T alpha[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) alpha[v] = (T)state[WPC_KS_ALPHA * NV + v];
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long k = i * NV + v;
+            T rk = r[k];
+            if (alpha[v] != T(0)) {
+                x[k] += alpha[v] * p[k];
+                rk -= alpha[v] * ap[k];
+                r[k] = rk;
+            }
+            acc[v] += (double)rk * (double)rk;
+        }
+    }
+    block_sum<NV>(acc, red);
+    write_partials<NV>(acc, partials);
+}
+
+// RZ_NEW = fold(r.z), RR_NEW = fold(r.r); for stepping slots BETA = RZ_NEW / RZ,
+// then RZ <- RZ_NEW, RR <- RR_NEW, ITER += 1 (BETA = 0 and nothing committed
+// for slots that did not step).
+__global__ __launch_bounds__(1024) void cg_beta_kernel(
+    int nvec, int P_rz, const double* __restrict__ part_rz, int P_rr,
+    const double* __restrict__ part_rr, double* __restrict__ state) {
+    __shared__ double s_rz[8], s_rr[8];
+    fold_all<1024>(part_rz, P_rz, nvec, s_rz);
+    fold_all<1024>(part_rr, P_rr, nvec, s_rr);
+    const int v = threadIdx.x;
+    {
+        if (v < nvec) {
+            const double rz_new = s_rz[v];
+            const double rr = s_rr[v];
+            const bool active = state[WPC_KS_ACTIVE * nvec + v] != 0.0;
+            state[WPC_KS_RZ_NEW * nvec + v] = rz_new;
+            state[WPC_KS_RR_NEW * nvec + v] = rr;
+            if (active) {
+                state[WPC_KS_BETA * nvec + v] = rz_new / state[WPC_KS_RZ * nvec + v];
+                state[WPC_KS_RZ * nvec + v] = rz_new;
+                state[WPC_KS_RR * nvec + v] = rr;
+                state[WPC_KS_ITER * nvec + v] += 1.0;
+            } else {
+                state[WPC_KS_BETA * nvec + v] = 0.0;
+            }
+        }
+    }
+}
+
+// p = z + BETA p for slots that stepped.
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void cg_direction_kernel(
+    int n, const T* __restrict__ z, T* __restrict__ p, const double* __restrict__ state) {
+    T beta[NV];
+    bool act[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        act[v] = state[WPC_KS_ACTIVE * NV + v] != 0.0;
+        beta[v] = (T)state[WPC_KS_BETA * NV + v];
+    }
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (!act[v]) continue;
+            const long k = i * NV + v;
+            p[k] = z[k] + beta[v] * p[k];
+        }
+    }
+}
+
 // target = max(atol, tol * sqrt(||b||^2))^2 in place (state[TARGET] holds ||b||^2).
 __global__ void ktarget_kernel(int nvec, double* state, double tol, double atol) {
     const int v = threadIdx.x;
@@ -234,6 +342,35 @@ int wpc_krylov_direction(int method, int n, int nvec, const void* z, void* p, co
 
 int wpc_krylov_set_target(int nvec, double* state, double tol, double atol, void* stream) {
     hipLaunchKernelGGL(ktarget_kernel, dim3(1), dim3(64), 0, as_stream(stream), nvec, state, tol, atol);
+    return launch_status();
+}
+
+int wpc_cg_alpha(int nvec, int nparts, const double* part_denom, double* state, void* stream) {
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(1024), 0, as_stream(stream), nvec,
+                       nparts, part_denom, state);
+    return launch_status();
+}
+
+int wpc_cg_update(int n, int nvec, void* x, void* r, const void* p, const void* ap, int dtype,
+                  const double* state, double* partials, void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_update_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, state, partials)))
+    return launch_status();
+}
+
+int wpc_cg_beta(int nvec, int nparts_rz, const double* part_rz, int nparts_rr,
+                const double* part_rr, double* state, void* stream) {
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(1024), 0, as_stream(stream), nvec,
+                       nparts_rz, part_rz, nparts_rr, part_rr, state);
+    return launch_status();
+}
+
+int wpc_cg_direction(int n, int nvec, const void* z, void* p, int dtype, const double* state,
+                     void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, state)))
     return launch_status();
 }
 

@@ -1,0 +1,82 @@
+// Deterministic block reductions shared by the streamed products and the
+// Krylov kernels (256-thread workgroups).
+#pragma once
+
+#include "wpc_common.h"
+
+namespace wpc {
+
+constexpr int kStreamRows = 256;
+
+// Deterministic block sum of v[NV] (all threads) into red[0..NV) (thread 0 reads).
+template <int NV>
+__device__ void block_sum(double* v, double* red) {
+    // red: kStreamRows doubles of scratch
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = v[q];
+        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
+        __syncthreads();
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < kStreamRows / kWave; ++w) s += red[w];
+            v[q] = s;
+        }
+    }
+}
+
+// Block partials are folded by a later launch (kernel boundaries order them): a
+// device-scope fence inside a kernel writes back the whole L2 of the XCD on
+// gfx950, which costs far more than one extra single-workgroup launch.
+template <int NV>
+__device__ __forceinline__ void write_partials(const double* v, double* partials) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) partials[(long)blockIdx.x * NV + q] = v[q];
+    }
+}
+
+// Fixed-order fold of partials[0..P) (stride nvec, vector v) by one workgroup;
+// the result is returned to every thread.
+__device__ inline double fold_block(const double* __restrict__ partials, int P, int nvec, int v,
+                                    double* red) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < P; b += kStreamRows) s += partials[(long)b * nvec + v];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kStreamRows / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double out = red[0];
+    __syncthreads();
+    return out;
+}
+
+// Fixed-order fold of partials[0..P) for all nvec (<= 8) vectors at once by a
+// workgroup of NT threads; results land in out_s[0..nvec) (shared memory).
+template <int NT>
+__device__ void fold_all(const double* __restrict__ partials, int P, int nvec, double* out_s) {
+    __shared__ double red[NT / kWave][8];
+    double acc[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+    for (int b = threadIdx.x; b < P; b += NT)
+        for (int v = 0; v < nvec; ++v) acc[v] += partials[(long)b * nvec + v];
+    for (int v = 0; v < nvec; ++v) {
+        double x = acc[v];
+        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < nvec) {
+        double t = 0.0;
+        for (int w = 0; w < NT / kWave; ++w) t += red[w][threadIdx.x];
+        out_s[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+}  // namespace wpc

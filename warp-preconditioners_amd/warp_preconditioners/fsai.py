@@ -20,7 +20,7 @@ import torch
 
 from . import _native as nat
 from .linear import LinearOperator
-from .sparse import BsrMatrix, bsr_transposed, scalarize
+from .sparse import BsrMatrix, StreamPlan, bsr_transposed, scalarize
 
 LANES = (1, 2, 4, 8, 16, 32)
 _FLOATS = (torch.float16, torch.float32, torch.float64)
@@ -154,6 +154,7 @@ class FSAI(LinearOperator):
                     "factor_dtype conversion lost finite entries or a positive diagonal"
                 )
         self.GT = bsr_transposed(self.G)
+        self._plans = None
         self._tmp = {}
         super().__init__(A.shape, A.dtype, device, self._apply)
         self._scalar_n = n
@@ -196,11 +197,30 @@ class FSAI(LinearOperator):
             self._tmp[dtype] = t
         return t
 
-    def apply_batch(self, x, y, z, alpha, beta, nvec):
-        """z = alpha G^T G x + beta y on nvec interleaved right-hand sides."""
+    # float32/float64 vectors go through wpc_csr_stream: apply_lanes == 1 (the
+    # default) lets the plan's autotuner pick the row mapping, other values force
+    # that many lanes per row; binary16 vectors use the multi-lane kernel.
+    fused = True
+
+    def plans(self):
+        if self._plans is None:
+            lanes = None if self.apply_lanes == 1 else self.apply_lanes
+            self._plans = (StreamPlan(self.G, lanes), StreamPlan(self.GT, lanes))
+        return self._plans
+
+    def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
+        """z = alpha G^T G x + beta y on nvec interleaved right-hand sides; with
+        ``dot_w`` also dot_out[v] = sum_i dot_w[i,v] z[i,v] (device, float64)."""
         x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
         n = self._scalar_n
         tmp = self._scratch(x.dtype, n * nvec)
+        streamed = x.dtype in (torch.float32, torch.float64)
+        if streamed:
+            pg, pgt = self.plans()
+            if alpha != 0.0:
+                pg.run(x, None, tmp, 1.0, 0.0, nvec, s=self.row_scale2)
+            return pgt.run(tmp, y, z, alpha, beta, nvec,
+                           w=None if dot_w is None else dot_w.reshape(-1), out=dot_out)
         nat.call("wpc_fsai_apply", n, nat.ptr(self.G.offsets), nat.ptr(self.G.columns),
                  nat.ptr(self.G.values), nat.ptr(self.GT.offsets), nat.ptr(self.GT.columns),
                  nat.ptr(self.GT.values), nat.code(self.G.scalar_type), nat.ptr(self.row_scale2),

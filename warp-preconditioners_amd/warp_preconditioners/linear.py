@@ -23,7 +23,7 @@ import math
 import torch
 
 from . import _native as nat
-from .sparse import BlockType, BsrMatrix, _product
+from .sparse import BlockType, BsrMatrix, StreamPlan, _product
 
 
 class LinearOperator:
@@ -41,11 +41,16 @@ class LinearOperator:
 class _MatrixOperator(LinearOperator):
     """BsrMatrix as an operator: Warp's bsr_mv with aliasing handled."""
 
-    def __init__(self, A: BsrMatrix, lanes=8):
+    def __init__(self, A: BsrMatrix, lanes=1):
         self.A = A
         self.lanes = lanes
         self._alias = None
+        self._plan = None
         super().__init__(A.shape, A.dtype, A.device, self._apply)
+
+    @property
+    def fused(self):
+        return self.A.block_shape == (1, 1)
 
     def _apply(self, x, y, z, alpha, beta):
         xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
@@ -54,15 +59,23 @@ class _MatrixOperator(LinearOperator):
                 self._alias = torch.empty_like(xf)
             self._alias.copy_(xf)
             xf = self._alias
-        _product(self.A, xf, yf, zf, alpha, beta, self.lanes)
+        if self.A.block_shape == (1, 1) and xf.dtype in (torch.float32, torch.float64):
+            self.apply_batch(xf, yf, zf, alpha, beta, 1)
+        else:
+            _product(self.A, xf, yf, zf, alpha, beta, 8)
 
-    def apply_batch(self, x, y, z, alpha, beta, nvec):
+    def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
         if self.A.block_shape != (1, 1):
             raise NotImplementedError("batched products need a scalar matrix")
         xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
         if xf.data_ptr() == zf.data_ptr():
             xf = xf.clone()
         A = self.A
+        if xf.dtype in (torch.float32, torch.float64):
+            if self._plan is None:
+                self._plan = StreamPlan(A, None if self.lanes == 1 else self.lanes)
+            return self._plan.run(xf, yf, zf, alpha, beta, nvec,
+                                  w=None if dot_w is None else dot_w.reshape(-1), out=dot_out)
         nat.call("wpc_csr_spmv", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(A.values),
                  nat.code(A.scalar_type), None, nat.ptr(xf), nat.ptr(yf), nat.ptr(zf),
                  nat.code(xf.dtype), nvec, float(alpha), float(beta), self.lanes, nat.stream())
@@ -114,6 +127,9 @@ class KrylovWork:
         self.code = nat.code(dtype)
         self.graph = None
         self.graph_steps = 0
+        self.use_fused = (self.method == 0 and M is not None and bool(getattr(A, "fused", False))
+                          and bool(getattr(M, "fused", False)) and hasattr(A, "apply_batch")
+                          and hasattr(M, "apply_batch"))
 
     # -- helpers
     def _field(self, k):
@@ -157,6 +173,20 @@ class KrylovWork:
         s = nat.stream()
         pd, pr, pz = (nat.ptr(self.parts[k]) for k in range(3))
         st = nat.ptr(self.state)
+        if self.use_fused:
+            # A p (+p.Ap partials) | alpha | update (+r.r partials) | M r (+r.z partials)
+            # | beta | direction
+            part, nparts = self.A.apply_batch(self.p, self.ap, self.ap, 1.0, 0.0, self.nvec,
+                                              dot_w=self.p)
+            nat.call("wpc_cg_alpha", self.nvec, nparts, nat.ptr(part), st, s)
+            nat.call("wpc_cg_update", self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
+                     nat.ptr(self.p), nat.ptr(self.ap), self.code, st, pr, s)
+            part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
+                                              dot_w=self.r)
+            nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), self.P, pr, st, s)
+            nat.call("wpc_cg_direction", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
+                     self.code, st, s)
+            return
         if self.method == 0:
             self._op(self.A, self.p, self.ap, self.ap, 1.0, 0.0)
             nat.call("wpc_krylov_dot", self.n, self.nvec, nat.ptr(self.p), nat.ptr(self.ap),
@@ -201,7 +231,7 @@ class KrylovWork:
 
 
 def nat_state_fields() -> int:
-    return 8
+    return 12
 
 
 def _default_tol(dtype):

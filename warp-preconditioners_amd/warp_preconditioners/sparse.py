@@ -83,6 +83,111 @@ class BsrMatrix:
         return self.to_scipy().toarray()
 
 
+class StreamPlan:
+    """Launch plan for ``wpc_csr_stream`` on one scalar CSR matrix.
+
+    Two row mappings exist: streamed (mode 0: row blocks of <= 256 rows / 2048
+    entries staged through LDS, computed once on the host from the offsets) and
+    multi-lane (mode L: L lanes per row).  ``lanes=None`` autotunes: on the first
+    product for a given (vector count, vector dtype) outside graph capture, every
+    mode is timed with HIP events on the current stream and the fastest is kept;
+    under capture a row-length heuristic decides.  The fused inner-product
+    partials belong to the plan, so one plan must not run on two streams at once.
+    """
+
+    CANDIDATES = (0, 2, 4, 8, 16)
+    # Choices are shared by every plan over the same index buffers in this process,
+    # so re-planning one matrix never changes its summation order.
+    _chosen = {}
+
+    def __init__(self, A: "BsrMatrix", lanes=None):
+        if A.block_shape != (1, 1):
+            raise ValueError("streamed products need a scalar CSR matrix")
+        off = np.ascontiguousarray(A.offsets.cpu().numpy(), dtype=np.int32)
+        cap = A.nrow + 2
+        blocks = np.empty(cap, dtype=np.int32)
+        count = nat.lib().wpc_csr_row_blocks(A.nrow, off.ctypes.data, blocks.ctypes.data, cap)
+        if count < 0:
+            raise nat.NativeError("wpc_csr_row_blocks failed")
+        self.nblocks = int(count)
+        self.blocks = torch.from_numpy(blocks[: count + 1].copy()).to(A.device)
+        self.fixed = lanes
+        self.modes = {}
+        self.timings = {}
+        self._partials = None
+        self.A = A
+        self.mean_row = float(off[-1]) / max(A.nrow, 1)
+
+    def workgroups(self, mode):
+        return self.nblocks if mode == 0 else (self.A.nrow * mode + 255) // 256
+
+    def partials(self, nvec, mode=None):
+        need = max(self.workgroups(0), self.workgroups(16), 1) * nvec
+        if self._partials is None or self._partials.numel() < need:
+            self._partials = torch.zeros(need, dtype=torch.float64, device=self.A.device)
+        return self._partials
+
+    def mode(self, nvec, dtype):
+        if self.fixed is not None:
+            return self.fixed
+        key = (nvec, dtype)
+        m = self.modes.get(key)
+        if m is None:
+            A = self.A
+            shared = (A.offsets.data_ptr(), A.columns.data_ptr(), A.nrow, A.nnz, nvec, dtype)
+            m = StreamPlan._chosen.get(shared)
+            if m is None:
+                if torch.cuda.is_current_stream_capturing():
+                    return 0 if self.mean_row <= 10 else 8
+                m = self._tune(nvec, dtype)
+                StreamPlan._chosen[shared] = m
+            self.modes[key] = m
+        return m
+
+    def _tune(self, nvec, dtype, reps=5):
+        n = self.A.nrow
+        x = torch.ones(n * nvec, dtype=dtype, device=self.A.device)
+        z = torch.empty_like(x)
+        stream = torch.cuda.current_stream()
+        best, best_t = 0, float("inf")
+        for m in self.CANDIDATES:
+            for _ in range(2):
+                self._launch(m, x, None, z, 1.0, 0.0, nvec)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                self._launch(m, x, None, z, 1.0, 0.0, nvec)
+            e1.record(stream)
+            e1.synchronize()
+            t = e0.elapsed_time(e1) / reps
+            self.timings[(nvec, dtype, m)] = t
+            if t < best_t:
+                best, best_t = m, t
+        return best
+
+    def _launch(self, mode, x, y, z, alpha, beta, nvec, s=None, d=None, u=None, w=None, out=None,
+                values=None, part=None):
+        A = self.A
+        vals = A.values if values is None else values
+        nat.call("wpc_csr_stream", self.nblocks, nat.ptr(self.blocks), A.nrow, mode,
+                 nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(vals), nat.code(vals.dtype),
+                 nat.ptr(s), nat.code(s.dtype) if s is not None else 0, nat.ptr(d), nat.ptr(u),
+                 nat.ptr(x), nat.ptr(y), nat.ptr(z), nat.code(x.dtype), nvec, float(alpha),
+                 float(beta), nat.ptr(w), nat.ptr(part), nat.ptr(out), nat.stream())
+
+    def run(self, x, y, z, alpha, beta, nvec=1, s=None, d=None, u=None, w=None, out=None,
+            values=None):
+        """z = alpha*(s .* (A x) + d .* u) + beta*y.
+
+        With ``w``: per-workgroup partials of w . z are left in the plan's buffer
+        (and folded into ``out`` when given); returns (partials, count)."""
+        mode = self.mode(nvec, x.dtype)
+        part = self.partials(nvec) if w is not None else None
+        self._launch(mode, x, y, z, alpha, beta, nvec, s, d, u, w, out, values, part)
+        return part, self.workgroups(mode)
+
+
 def _device(device):
     return torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
 

@@ -11,7 +11,7 @@ import torch
 
 from . import _native as nat
 from .linear import LinearOperator
-from .sparse import BsrMatrix, bsr_copy
+from .sparse import BsrMatrix, StreamPlan, bsr_copy
 
 LANES = (1, 2, 4, 8, 16, 32)
 
@@ -46,18 +46,27 @@ class SparseOperator(LinearOperator):
         self.scalar = bsr_copy(A, block_shape=(1, 1)) if A.block_shape != (1, 1) else bsr_copy(A)
         self.row_lanes = row_lanes
         self._alias = torch.empty(self.scalar.ncol, dtype=A.scalar_type, device=A.device)
+        self._plan = None
         super().__init__(A.shape, A.dtype, A.device, self._apply)
+
+    fused = True
+
+    def _stream(self):
+        if self._plan is None:
+            self._plan = StreamPlan(self.scalar, None if self.row_lanes == 1 else self.row_lanes)
+        return self._plan
 
     def _apply(self, x, y, z, alpha, beta):
         x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
         if x.data_ptr() == z.data_ptr():
             self._alias.copy_(x)
             x = self._alias
-        _matvec(self.scalar, x, y, z, alpha, beta, self.row_lanes)
+        self._stream().run(x, y, z, alpha, beta)
 
-    def apply_batch(self, x, y, z, alpha, beta, nvec):
-        """z = alpha A x + beta y on nvec interleaved vectors."""
+    def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
+        """z = alpha A x + beta y on nvec interleaved vectors (+ fused dot)."""
         x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
         if x.data_ptr() == z.data_ptr():
             x = x.clone()
-        _matvec(self.scalar, x, y, z, alpha, beta, self.row_lanes, nvec=nvec)
+        return self._stream().run(x, y, z, alpha, beta, nvec,
+                                  w=None if dot_w is None else dot_w.reshape(-1), out=dot_out)

@@ -14,7 +14,7 @@ import torch
 
 from . import _native as nat
 from .linear import LinearOperator
-from .sparse import BsrMatrix, bsr_copy
+from .sparse import BsrMatrix, StreamPlan, bsr_copy
 
 LANES = (1, 2, 4, 8, 16, 32)
 
@@ -69,6 +69,8 @@ class SquaredLaplacianOperator(LinearOperator):
         self.identity = self.free.numel() == n and bool(
             torch.equal(self.free, torch.arange(n, dtype=torch.int32, device=dev)))
         self._tmp = {}
+        self._plan = None
+        self._wm = None
         nf = self.free.numel()
         super().__init__((nf, nf), L.dtype, dev, self._apply)
 
@@ -79,11 +81,33 @@ class SquaredLaplacianOperator(LinearOperator):
             self._tmp[dtype] = t
         return t
 
-    def product(self, x, y, z, alpha, beta, boundary=False, nvec=1):
+    @property
+    def fused(self):
+        return self.identity
+
+    def _stream(self):
+        if self._plan is None:
+            self._plan = StreamPlan(self.L, None if self.row_lanes == 1 else self.row_lanes)
+            self._wm = torch.empty_like(self.mass)
+            nat.call("wpc_axpby", self.mass.numel(), self.mass_weight, nat.ptr(self.mass), 0.0,
+                     None, nat.ptr(self._wm), nat.code(self.mass.dtype), nat.stream())
+        return self._plan
+
+    def product(self, x, y, z, alpha, beta, boundary=False, nvec=1, dot_w=None, dot_out=None):
         L = self.L
         x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
         tmp = self._scratch(x.dtype, L.nrow * nvec)
         ident = self.identity and not boundary
+        if ident and x.dtype in (torch.float32, torch.float64):
+            # pass 1: tmp = M^-1 (L x); pass 2: z = alpha (L tmp + w M x) + beta y
+            plan = self._stream()
+            if alpha != 0.0:
+                plan.run(x, None, tmp, 1.0, 0.0, nvec, s=self.inverse_mass)
+            return plan.run(tmp, y, z, alpha, beta, nvec,
+                            d=self._wm if self.mass_weight != 0.0 else None, u=x,
+                            w=None if dot_w is None else dot_w.reshape(-1), out=dot_out)
+        if dot_w is not None:
+            raise ValueError("fused inner products need the identity free map")
         nat.call("wpc_sqlap_apply", L.nrow, nat.ptr(L.offsets), nat.ptr(L.columns),
                  nat.ptr(L.values), nat.code(L.scalar_type), nat.ptr(self.inverse_mass),
                  None if ident else nat.ptr(self.index), self.free.numel(),
@@ -95,8 +119,8 @@ class SquaredLaplacianOperator(LinearOperator):
     def _apply(self, x, y, z, alpha, beta):
         self.product(x, y, z, alpha, beta, False)
 
-    def apply_batch(self, x, y, z, alpha, beta, nvec):
-        self.product(x, y, z, alpha, beta, False, nvec)
+    def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
+        return self.product(x, y, z, alpha, beta, False, nvec, dot_w, dot_out)
 
     def rhs(self, boundary_values, out=None):
         if (boundary_values.dtype != self.scalar_type or boundary_values.device != self.device
