@@ -1,0 +1,108 @@
+"""Writes the golden FSAI fixtures from the CPU oracle (run in the build container).
+
+    python tests/golden/make_golden.py [--threads 8]
+
+* config1_fsai.npz -- full factor of the CPU-runnable configuration: 64x64 interior
+  five-point Laplacian (diag 4, off -1), float32, FSAI width 8, step 1,
+  kap_tolerance 1e-3 (offsets, columns, values), the oracle's truncated-row
+  count, and the float64 PCG iteration count to 1e-8 for b ~ N(0,1) seed 0.
+* patterns.json + rows_*.npz -- for the full-size configurations: SHA-256 of the
+  factor pattern (offsets then columns as little-endian int32) and of the values,
+  plus 1000 sampled rows (seed 1) with their columns and values.
+"""
+
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "warp-preconditioners_amd"))
+
+from oracle import cpu  # noqa: E402
+from warp_preconditioners import workloads  # noqa: E402
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).astype(a.dtype.newbyteorder("<")).tobytes())
+    return h.hexdigest()
+
+
+def sample_rows(G, count=1000, seed=1):
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(G.nrow, size=min(count, G.nrow), replace=False)).astype(np.int32)
+    off = [0]
+    cols, vals = [], []
+    for r in rows:
+        a, b = G.off[r], G.off[r + 1]
+        cols.append(G.col[a:b])
+        vals.append(G.val[a:b])
+        off.append(off[-1] + (b - a))
+    return dict(rows=rows, off=np.array(off, np.int32), cols=np.concatenate(cols).astype(np.int32),
+                vals=np.concatenate(vals))
+
+
+def config_matrices():
+    """(name, CSR on the host, FSAI kwargs) for the full-size configurations."""
+    out = []
+    v, t, _ = workloads.kuhn_grid(64, 0.15, seed=0)
+    K, m = workloads.p1_laplacian(v, t)
+    Q = workloads.squared_operator(K, m, 1.0)
+    out.append(("config2_Q_w8", cpu.CSR.from_scipy(Q, np.float64),
+                dict(max_row_size=8, factor_dtype=np.float32)))
+    off, col, val, _, _ = workloads.kron_coupling(48, 1.0e6, seed=0)
+    A3 = cpu.CSR(off, col, np.ascontiguousarray(val), len(off) - 1, len(off) - 1, 3)
+    out.append(("config3_KxC_w16", A3, dict(max_row_size=16)))
+    vb, tb, _ = workloads.kuhn_grid(None, 0.15, seed=0, shape=(160, 40, 40))
+    Kb, _, _, _ = workloads.p1_elasticity(vb, tb, vb[:, 0] == 0.0, young=1.0, poisson=0.45)
+    A4 = cpu.CSR(Kb.indptr.astype(np.int32), Kb.indices.astype(np.int32),
+                 np.ascontiguousarray(Kb.data), Kb.shape[0] // 3, Kb.shape[1] // 3, 3)
+    out.append(("config4_elasticity_w8", A4, dict(max_row_size=8, factor_dtype=np.float32)))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--skip-large", action="store_true")
+    args = ap.parse_args()
+    cpu.build()
+    lap = workloads.laplacian_2d(64)
+    A1 = cpu.CSR.from_scipy(lap, np.float32)
+    P = cpu.fsai(A1, max_row_size=8, max_step_size=1, kap_tolerance=1e-3)
+    b = np.random.default_rng(0).standard_normal(4096)
+    P64 = dict(G=cpu.CSR(P["G"].off, P["G"].col, P["G"].val.astype(np.float64), 4096, 4096),
+               GT=cpu.CSR(P["GT"].off, P["GT"].col, P["GT"].val.astype(np.float64), 4096, 4096))
+    _, iters, _ = cpu.krylov("cg", b, A=cpu.CSR.from_scipy(lap), P=P64, tol=1e-8, maxiter=4096)
+    np.savez_compressed(HERE / "config1_fsai.npz", offsets=P["G"].off, columns=P["G"].col,
+                        values=P["G"].val, truncated=np.int32(P["truncated_rows"]),
+                        cg_iterations=np.int32(iters))
+    print("config1: nnz", P["G"].nnz, "truncated", P["truncated_rows"], "CG iterations", iters)
+    if args.skip_large:
+        return
+    meta = {}
+    for name, A, kw in config_matrices():
+        t0 = time.perf_counter()
+        P = cpu.fsai(A, nthreads=args.threads, **kw)
+        G = P["G"]
+        meta[name] = dict(n=int(G.nrow), nnz=int(G.nnz), truncated_rows=int(P["truncated_rows"]),
+                          pattern_sha256=digest(G.off, G.col), values_sha256=digest(G.val),
+                          values_dtype=str(G.val.dtype), fsai=dict(kw, factor_dtype=str(
+                              np.dtype(kw.get("factor_dtype", A.val.dtype)))))
+        np.savez_compressed(HERE / f"rows_{name}.npz", **sample_rows(G))
+        print(name, meta[name], f"{time.perf_counter() - t0:.1f}s", flush=True)
+    (HERE / "patterns.json").write_text(json.dumps(meta, indent=2) + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -280,3 +280,44 @@ def test_lifted_biharmonic(wp, tau):
     b = system.rhs(vec(rng.normal(size=10)))
     assert tuple(b.shape) == (10, 2)
     np.testing.assert_array_equal(host(system.solution(b)), host(b)[:, 0])
+
+
+@pytest.mark.parametrize("tau", [0.02, 2.0])
+def test_lifted_biharmonic_gmres(wp, tau):
+    # tests/test_biharmonic.py:13-43: GMRES on the lifted system recovers the Q solve.
+    from warp_preconditioners import linear
+
+    rng = np.random.default_rng(13)
+    k = rng.normal(size=(10, 10))
+    k = k @ k.T
+    mass = rng.uniform(0.4, 2.0, size=10)
+    b_np = rng.normal(size=10)
+    system = wp.BiharmonicSystem(upload(k), vec(mass), tau)
+    Q = np.diag(mass) + tau * (k / mass) @ k
+    b = system.rhs(vec(b_np))
+    x = torch.zeros_like(b)
+    pre = system.preconditioner(max_row_size=10, kap_tolerance=0)
+    linear.gmres(system.matrix, b, x, M=pre, tol=1e-12, maxiter=100, restart=20)
+    u = host(system.solution(x))
+    np.testing.assert_allclose(u, np.linalg.solve(Q, b_np), rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(Q @ u, b_np, rtol=1e-9, atol=1e-9)
+
+
+def test_block_ilu_gmres_on_indefinite_blocks(wp):
+    # tests/test_indefinite.py:127-139 pattern: GMRES + BlockILU0 on an indefinite
+    # 2x2-block system with a manufactured solution.
+    from warp_preconditioners import linear
+
+    rng = np.random.default_rng(7)
+    n = 12
+    blocks = rng.normal(size=(n, n, 2, 2)) * 0.05
+    for i in range(n):
+        blocks[i, i] = np.array([[2.0, 0.3], [0.3, -1.5]])
+    a = blocks.transpose(0, 2, 1, 3).reshape(2 * n, 2 * n)
+    truth = rng.normal(size=(n, 2))
+    A = upload(a, block=2)
+    b = torch.tensor((a @ truth.ravel()).reshape(n, 2), device=dev())
+    x = torch.zeros_like(b)
+    linear.gmres(A, b, x, M=wp.BlockILU0(A, factor_sweeps=20, solve_sweeps=n), tol=1e-11,
+                 maxiter=60, restart=15)
+    np.testing.assert_allclose(host(x), truth, rtol=1e-8, atol=1e-8)
