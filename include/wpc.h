@@ -247,6 +247,34 @@ int wpc_sqlap_apply(int n_full, const int* offsets, const int* columns, const vo
                     const void* x, const void* y, void* z, int nvec, double alpha, double beta,
                     int boundary, int lanes, void* stream);
 
+/* ------------------------------------------------------------ assembly */
+
+/* P1 tetrahedra: local[16*t + 4a + b] = vol_t * grad(phi_a) . grad(phi_b) and
+ * quarter_volume[t] = vol_t / 4 (vertices float64 [nv,3], tets int32 [ntet,4]).
+ * Replaces the host stiffness/mass assembly the reference's benchmarks run
+ * (benchmarks/elasticity_problem.py:33-61, scalar term; lumped mass :44). */
+int wpc_p1_local(int ntet, const double* vertices, const int* tets, double* local,
+                 double* quarter_volume, void* stream);
+
+/* out[s] = sum_{k = seg[s]}^{seg[s+1]-1} values[order[k]] summed in k order
+ * (order may be NULL = identity): deterministic triplet assembly after a stable
+ * key sort (Warp's bsr_from_triplets duplicate summation, biharmonic.py:97,169). */
+int wpc_segment_sum(int nseg, const long* segments, const long* order, const double* values,
+                    double* out, void* stream);
+
+/* Q = K diag(1/m) K + alpha diag(m) on the device (float64): per row, the products
+ * (K[i,k] * inverse_mass[k]) * K[k,j] accumulate per column in the order of k
+ * along row i, then alpha*m_i joins the diagonal -- the term order of the host
+ * SpGEMM the reference uses for its assembled squared operator
+ * (visualization/solve_dirichlet.py:150-153).  count: counts[i] = row size of Q
+ * (-1 if a row exceeds 512 distinct columns); scan to q_offsets, then fill
+ * (columns sorted). */
+int wpc_spgemm_kmk_count(int n, const int* offsets, const int* columns, int* counts,
+                         void* stream);
+int wpc_spgemm_kmk_fill(int n, const int* offsets, const int* columns, const double* values,
+                        const double* inverse_mass, const double* mass, double alpha,
+                        const int* q_offsets, int* q_columns, double* q_values, void* stream);
+
 /* ------------------------------------------------------------ block Jacobi / ILU(0) */
 
 /* rows[e] = block row of e; diagonal[i] = slot of block (i,i) or -1 (bad[0]++).
