@@ -3,11 +3,13 @@
 // the sparse product Q = K diag(1/m) K + alpha diag(m).
 //
 // The product follows the host pipeline it replaces term for term: for row i the
-// contributions (K[i,k] * (1/m_k)) * K[k,j] are accumulated per output column in
-// the order of k along row i, then alpha*m_i is added to the diagonal, so Q is
-// bit-identical to ((K @ diag(1/m)) @ K + alpha*diag(m)) formed row by row with
-// sequential accumulation.  One wavefront owns one output row; the row's columns
-// live in an LDS open-addressing table.
+// contributions (K[i,k] * (1/m_k)) * K[k,j] are accumulated per output column over
+// the stored k of row i in DESCENDING column order, then alpha*m_i is added to the
+// diagonal.  That is the order of scipy's ((K @ diag(1/m)) @ K + alpha*diag(m)):
+// its intermediate K @ diag(1/m) comes out of the row-wise linked-list product with
+// each row's columns reversed, and the second product accumulates along that
+// stored order.  One wavefront owns one output row; the row's columns live in an
+// LDS open-addressing table.
 
 #include "wpc_common.h"
 
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void spgemm_fill_kernel(
         sums[s] = 0.0;
     }
     wave_sync();
-    for (int e = off[i]; e < off[i + 1]; ++e) {
+    for (int e = off[i + 1] - 1; e >= off[i]; --e) {
         const int k = col[e];
         const double a = val[e] * inv_m[k];  // entry of K @ diag(1/m)
         // distinct j within row k: each lane owns different slots in this step

@@ -5,7 +5,8 @@ keys and allocates).
 The reference assembles these on the host (benchmarks/elasticity_problem.py:33-61,
 visualization/solve_dirichlet.py:150-153); the serving loop needs them on the device
 so that setup latency does not scale with host SpGEMM time.  ``squared_operator``
-reproduces the host product's term order, so for the same K every entry is
+reproduces the host product's term order (descending k along each row, the order
+scipy's intermediate K @ diag(1/m) stores), so for the same K every entry is
 bit-identical to ``(K @ diag(1/m)) @ K + alpha diag(m)`` formed with sequential
 accumulation.  Its pattern is structural (every column reachable through a stored
 entry of K), so it can hold exact zeros that a value-dropping host product omits;
