@@ -269,6 +269,42 @@ def kernel_times(s, reps=30):
     return out, bytes_, tuned
 
 
+COPY_PROBE_BYTES = 256 << 20
+
+
+def profile_kernels(s, reps=10):
+    """Per-kernel launches only (no solves), then a plain device copy of
+    COPY_PROBE_BYTES: the copy's FETCH_SIZE/WRITE_SIZE per dispatch calibrate the
+    counters to bytes."""
+    import torch
+
+    times, bytes_, tuned = kernel_times(s, reps=reps)
+    src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_us": {k: v * 1e6 for k, v in times.items()},
+                      "bytes": bytes_, "row_mapping": tuned,
+                      "copy_probe_bytes": COPY_PROBE_BYTES}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (a
+    markdown table row `| kernel | fetch KiB | write KiB | bytes |`), or None."""
+    path = ROOT / "profiles" / "factored_biharmonic_64cube_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == kernel:
+            try:
+                return float(cells[3])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(wl, s, iters_needed, sample_iters=40):
     """Oracle PCG (float64, factored operator, same FSAI factor) on the host, one
     thread, timed over a bounded number of iterations and scaled to solves/s."""
@@ -303,6 +339,12 @@ def main():
     ap.add_argument("--compare", action="store_true", help="also time the plain one-RHS path")
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mapping", default=None,
+                    help="force the row mapping of the G, GT and K products, e.g. 4,0,8 "
+                         "(0 = streamed, L = L lanes per row); default autotunes")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="time only the per-iteration kernels (and a 256 MiB copy that "
+                         "calibrates HBM byte counters); for counter runs under rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every run is on one "
@@ -313,6 +355,16 @@ def main():
     wl = build_workload(args.side)
     s = Solver(wl, args.mode)
     log(f"mode {args.mode}: FSAI setup {s.setup_s:.3f}s nnzG={s.nnzG}")
+    if args.mapping:
+        g, gt, k = (int(v) for v in args.mapping.split(","))
+        pg, pgt = s.P.plans()
+        pg.fixed, pgt.fixed = g, gt
+        for op in (getattr(s, "op64", None), getattr(s, "op32", None)):
+            if op is not None:
+                op._stream().fixed = k
+    if args.profile_kernels:
+        profile_kernels(s)
+        return
     t0 = time.perf_counter()
     Xref, ref_change, ref_iters = reference_solution(s)
     log(f"float64 reference: {ref_iters} iterations, correction {ref_change} "
@@ -355,7 +407,7 @@ def main():
                    "n": s.n, "nnzK": int(wl["K"].nnz), "nnzQ": int(wl["Q"].nnz), "nnzG": s.nnzG},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes": bytes_[dominant],
+                     "traffic": measured_traffic(dominant), "bytes": bytes_[dominant],
                      "avg_us": times[dominant] * 1e6},
         "kernels_us": {k: v * 1e6 for k, v in times.items()},
         "kernels_GBs": {k: bytes_[k] / times[k] / 1e9 for k, v in times.items()},

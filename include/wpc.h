@@ -391,6 +391,51 @@ int wpc_cg_beta(int nvec, int nparts_rz, const double* part_rz, int nparts_rr,
 int wpc_cg_direction(int n, int nvec, const void* z, void* p, int dtype, const double* state,
                      void* stream);
 
+/* ------------------------------------------------------------------ certified residuals
+ * No single reference routine: these supply the float64 true residual b - A x
+ * that the reference's solves check through Warp's cg/cr (`bsr_mv` at
+ * sparse_operator.py:104, squared_laplacian.py:437-474), accumulated with the
+ * compensated Dot2 product so that the residual itself carries no cancellation
+ * noise.  Vectors are float64 [n][nvec] (nvec 1..8).
+ *   assembled operator (t_hi = t_lo = NULL):  r = b - A x        (values = A)
+ *   factored operator:  t = inverse_mass .* (K x) kept as t_hi + t_lo, then
+ *                       r = b - alpha*mass .* x - K t            (values = K)
+ * norms[0..nvec) = sum r^2/mass, norms[nvec..2nvec) = sum mass*x^2 (fixed-order
+ * fold of double[wpc_residual_partials(n) * 2 * nvec] block partials). */
+int wpc_residual_partials(int n);
+int wpc_dd_residual(int n, const int* offsets, const int* columns, const double* values,
+                    const double* mass, const double* inverse_mass, double alpha, const double* x,
+                    const double* b, double* t_hi, double* t_lo, double* r, int nvec,
+                    double* partials, double* norms, void* stream);
+
+/* Mixed-precision refinement steps, per right-hand side v with bit v of mask set
+ * (s_v = sqrt(norms[v]), or 1 when zero):
+ *   prepare: out = r / s_v (others 0), stored as out_dtype (f32 or f64)
+ *   update:  x += s_v * c. */
+int wpc_refine_prepare(int n, int nvec, const double* r, const double* norms, int mask, void* out,
+                       int out_dtype, void* stream);
+int wpc_refine_update(int n, int nvec, double* x, const void* c, int c_dtype, const double* norms,
+                      int mask, void* stream);
+
+/* state[TARGET][v] *= factor (tightens one slot's stopping threshold; a slot
+ * frozen by the old threshold resumes stepping). */
+int wpc_krylov_scale_target(int nvec, double* state, int v, double factor, void* stream);
+
+/* values[diag(i)] += alpha * d[i] on a canonical CSR matrix (forms K + alpha*M);
+ * status[0] counts rows without a stored diagonal. */
+int wpc_add_diagonal(int n, const int* offsets, const int* columns, double* values,
+                     const double* d, double alpha, int* status, void* stream);
+
+/* ------------------------------------------------------------------ graphs
+ * No reference counterpart (Warp captures with its own graph API around
+ * cg/cr): capture the library launches issued on `stream` between begin and end
+ * into an executable HIP graph (thread-local capture mode, no device
+ * synchronization); launch it on any stream; destroy it when done. */
+int wpc_graph_begin(void* stream);
+int wpc_graph_end(void* stream, void** graph_exec);
+int wpc_graph_launch(void* graph_exec, void* stream);
+int wpc_graph_destroy(void* graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,18 @@
 namespace wpc {
 
 constexpr int kStreamRows = 256;
+constexpr int kXcds = 8;
+
+// The dispatcher hands workgroups to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2.  Renumbering the workgroups so that
+// every XCD owns one contiguous range of logical blocks keeps the gathers of
+// neighbouring rows (which share most of their columns) inside one L2.
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int x = b % kXcds, q = b / kXcds;
+    const int per = nb / kXcds, extra = nb % kXcds;
+    return x * per + min(x, extra) + q;
+}
 
 // Deterministic block sum of v[NV] (all threads) into red[0..NV) (thread 0 reads).
 template <int NV>
@@ -31,10 +43,12 @@ __device__ void block_sum(double* v, double* red) {
 // device-scope fence inside a kernel writes back the whole L2 of the XCD on
 // gfx950, which costs far more than one extra single-workgroup launch.
 template <int NV>
-__device__ __forceinline__ void write_partials(const double* v, double* partials) {
+__device__ __forceinline__ void write_partials(const double* v, double* partials,
+                                               int block = -1) {
     if (threadIdx.x == 0) {
+        const long b = block < 0 ? blockIdx.x : block;
 #pragma unroll
-        for (int q = 0; q < NV; ++q) partials[(long)blockIdx.x * NV + q] = v[q];
+        for (int q = 0; q < NV; ++q) partials[b * NV + q] = v[q];
     }
 }
 

@@ -53,7 +53,8 @@ __global__ __launch_bounds__(kStreamRows) void stream_kernel(
     __shared__ int s_col[kStreamNnz];
     __shared__ TL s_val[kStreamNnz];
     __shared__ double red[kStreamRows];
-    const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+    const int blk = xcd_block();
+    const int r0 = blocks[blk], r1 = blocks[blk + 1];
     const int e0 = off[r0];
     const int nnz = off[r1] - e0;
     TA acc[NV];
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(kStreamRows) void stream_kernel(
     }
     if (DOT) {
         block_sum<NV>(dot, red);
-        write_partials<NV>(dot, partials);
+        write_partials<NV>(dot, partials, blk);
     }
 }
 
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
     using TA = typename AccOf<TX>::type;
     __shared__ double red[kStreamRows];
     const int L = 1 << lg;
-    const long row = ((long)blockIdx.x * kStreamRows + threadIdx.x) >> lg;
+    const int blk = xcd_block();
+    const long row = ((long)blk * kStreamRows + threadIdx.x) >> lg;
     const int sub = threadIdx.x & (L - 1);
     const bool live = row < n;
     const bool use_a = alpha != TA(0);
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
     }
     if (DOT) {
         block_sum<NV>(dot, red);
-        write_partials<NV>(dot, partials);
+        write_partials<NV>(dot, partials, blk);
     }
 }
 

@@ -116,3 +116,30 @@ def squared_operator(K: BsrMatrix, mass: torch.Tensor, alpha: float = 1.0) -> Bs
              nat.ptr(inverse), nat.ptr(mass), float(alpha), nat.ptr(offsets), nat.ptr(cols),
              nat.ptr(vals), s)
     return BsrMatrix(n, n, (1, 1), offsets, cols, vals)
+
+
+def add_diagonal(A: BsrMatrix, d: torch.Tensor, alpha: float) -> BsrMatrix:
+    """A + alpha*diag(d) as a new matrix sharing A's index buffers (float64)."""
+    values = A.values.clone()
+    status = torch.zeros(1, dtype=torch.int32, device=A.device)
+    nat.call("wpc_add_diagonal", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(values),
+             nat.ptr(d), float(alpha), nat.ptr(status), nat.stream())
+    if int(status.item()):
+        raise ValueError("add_diagonal requires a stored diagonal in every row")
+    return BsrMatrix(A.nrow, A.ncol, (1, 1), A.offsets, A.columns, values)
+
+
+def jittered_grid(side: int, seed: int, device, jitter: float = 0.15):
+    """Vertices (float64 [side^3, 3]) of the unit-cube grid with interior vertices
+    moved uniformly by +-jitter*h (a device generator seeded with `seed`), and the
+    Kuhn tetrahedra of that grid."""
+    h = 1.0 / (side - 1)
+    ax = torch.arange(side, device=device)
+    idx = torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), dim=-1).reshape(-1, 3)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    noise = (torch.rand(idx.shape, generator=gen, dtype=torch.float64, device=device) * 2 - 1) * (
+        jitter * h)
+    interior = ((idx > 0) & (idx < side - 1)).all(dim=1, keepdim=True)
+    vertices = idx.to(torch.float64) * h + torch.where(interior, noise, torch.zeros_like(noise))
+    return vertices.contiguous(), kuhn_tets((side, side, side), device)

@@ -171,9 +171,15 @@ class FSAI(LinearOperator):
             raise ValueError("Construct FSAI with reuse_pattern=True before updating")
         A = self.source if A is None else A
         if not isinstance(A, BsrMatrix):
-            raise ValueError("update requires a Warp BSR matrix")
+            raise ValueError("update requires a Warp BSR matrix (pass A after release_source)")
         self._refit.update(self, A)
         self.source = A
+        return self
+
+    def release_source(self):
+        """Drop the reference to the source matrix (frees it when nothing else holds
+        it); update() and quality() then need an explicit A."""
+        self.source = None
         return self
 
     def quality(self, A=None, probes=4, seed=17):

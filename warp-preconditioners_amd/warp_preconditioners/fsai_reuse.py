@@ -128,3 +128,85 @@ def quality(owner, A, probes=4, seed=17):
                  nat.ptr(acc), s)
         total += float(acc.item())
     return math.sqrt(total / (probes * max(n, 1)))
+
+
+def refit_clone(template, A: BsrMatrix, topology=None):
+    """A new FSAI for A on `template`'s supports: the reuse path of
+    fsai_reuse.py:195-245 applied to a second matrix of the same topology.
+
+    The clone shares the template's index buffers (G/GT offsets and columns, row
+    partitions) and owns its values, so the template keeps serving its own
+    matrix.  ``topology`` = (offsets, columns) of the template's source (default:
+    the template's source matrix).  Synchronizes once to read the refit status.
+    """
+    from .fsai import FSAI
+    from .sparse import StreamPlan
+
+    if not isinstance(A, BsrMatrix) or A.block_shape != (1, 1) or A.row_counts is not None:
+        raise ValueError("refit_clone requires a compact scalar CSR matrix")
+    n = template._scalar_n
+    if A.shape != (n, n):
+        raise ValueError("FSAI update requires the original shape, dtype, device and topology")
+    off0, col0 = topology if topology is not None else RefitPlan.arrays(template.source)
+    inc = RefitPlan.arrays(A)
+    if inc[0].shape != off0.shape or inc[1].shape != col0.shape:
+        raise ValueError("FSAI update requires identical topology storage")
+    dev = A.device
+    s = nat.stream()
+    bad = torch.zeros(3, dtype=torch.int32, device=dev)
+    nat.call("wpc_compare_i32", off0.numel(), nat.ptr(inc[0]), nat.ptr(off0), nat.ptr(bad), s)
+    nat.call("wpc_compare_i32", col0.numel(), nat.ptr(inc[1]), nat.ptr(col0), nat.ptr(bad), s)
+    G, GT = template.G, template.GT
+    dt = nat.code(A.scalar_type)
+    scale = torch.empty(n, dtype=A.scalar_type, device=dev)
+    nat.call("wpc_fsai_diagonal", n, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(A.values), dt,
+             nat.ptr(scale), nat.ptr(bad[1:]), s)
+    scaled = template.row_scale2 is not None
+    values = torch.empty(G.nnz, dtype=G.scalar_type, device=dev)
+    exact = torch.empty(G.nnz, dtype=A.scalar_type, device=dev) if scaled else values
+    nat.call("wpc_fsai_refit", n, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(A.values), dt,
+             nat.ptr(G.offsets), nat.ptr(G.columns), nat.ptr(scale), float(template.pivot_floor),
+             template.max_row_size, nat.ptr(exact), nat.code(exact.dtype), nat.ptr(bad[1:]), s)
+    row_scale2 = None
+    if scaled:
+        row_scale2 = torch.empty_like(template.row_scale2)
+        nat.call("wpc_fsai_compress_scaled", n, nat.ptr(G.offsets), nat.ptr(G.columns),
+                 nat.ptr(exact), dt, nat.ptr(values), nat.code(G.scalar_type), nat.ptr(row_scale2),
+                 nat.ptr(bad[1:]), s)
+    status = bad.cpu()
+    if int(status[0]):
+        raise ValueError("FSAI update requires identical sparsity; rebuild to change it")
+    if int(status[1]):
+        raise ValueError("FSAI refit requires finite positive diagonals and SPD local supports")
+    transpose = getattr(template, "_transpose_map", None)
+    if transpose is None:
+        transpose = torch.empty(G.nnz, dtype=torch.int32, device=dev)
+        nat.call("wpc_refit_transpose_map", n, nat.ptr(G.offsets), nat.ptr(G.columns),
+                 nat.ptr(GT.offsets), nat.ptr(GT.columns), nat.ptr(transpose), s)
+        template._transpose_map = transpose
+    gt_values = torch.empty_like(values)
+    nat.call("wpc_scatter", G.nnz, values.element_size(), nat.ptr(values), nat.ptr(transpose),
+             nat.ptr(gt_values), s)
+    clone = FSAI.__new__(FSAI)
+    for name in ("max_row_size", "max_step_size", "factor_dtype", "apply_lanes", "kap_tolerance",
+                 "pivot_floor", "truncated_rows", "_scalar_n"):
+        setattr(clone, name, getattr(template, name))
+    clone.source = A
+    clone.G = BsrMatrix(n, n, (1, 1), G.offsets, G.columns, values)
+    clone.GT = BsrMatrix(n, n, (1, 1), GT.offsets, GT.columns, gt_values)
+    clone.row_scale2 = row_scale2
+    clone._transpose_map = transpose
+    clone._tmp = {}
+    clone._refit = None
+    clone._plans = None
+    if template._plans is not None:
+        clone._plans = (StreamPlan.like(template._plans[0], clone.G),
+                        StreamPlan.like(template._plans[1], clone.GT))
+    LinearOperator_init(clone, A)
+    return clone
+
+
+def LinearOperator_init(op, A):
+    from .linear import LinearOperator
+
+    LinearOperator.__init__(op, A.shape, A.dtype, A.device, op._apply)

@@ -118,6 +118,24 @@ class StreamPlan:
         self.A = A
         self.mean_row = float(off[-1]) / max(A.nrow, 1)
 
+    @classmethod
+    def like(cls, other: "StreamPlan", A: "BsrMatrix") -> "StreamPlan":
+        """Plan for a matrix with `other`'s index buffers (same row partition and
+        row-mapping choices, no host work); owns its own inner-product partials."""
+        if A.offsets.data_ptr() != other.A.offsets.data_ptr():
+            raise ValueError("StreamPlan.like requires the same offsets buffer")
+        plan = cls.__new__(cls)
+        plan.nblocks, plan.blocks, plan.fixed = other.nblocks, other.blocks, other.fixed
+        plan.modes, plan.timings = dict(other.modes), {}
+        plan._partials = None
+        plan.A = A
+        plan.mean_row = other.mean_row
+        return plan
+
+    def heuristic(self):
+        """Row mapping chosen without timing: streamed for short rows, else 8 lanes."""
+        return 0 if self.mean_row <= 10 else 8
+
     def workgroups(self, mode):
         return self.nblocks if mode == 0 else (self.A.nrow * mode + 255) // 256
 
@@ -138,7 +156,7 @@ class StreamPlan:
             m = StreamPlan._chosen.get(shared)
             if m is None:
                 if torch.cuda.is_current_stream_capturing():
-                    return 0 if self.mean_row <= 10 else 8
+                    return self.heuristic()
                 m = self._tune(nvec, dtype)
                 StreamPlan._chosen[shared] = m
             self.modes[key] = m
