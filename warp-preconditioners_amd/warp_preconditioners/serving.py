@@ -1,0 +1,596 @@
+"""Continuous-batching solve service on one GPU.
+
+A request names a mesh (grid side and jitter seed of the unit-cube Kuhn mesh), a
+system kind and 1..8 right-hand sides:
+
+* ``"laplacian"`` -- (K + alpha*M) x = b, assembled, FSAI width 8 on it;
+* ``"biharmonic"`` -- (alpha*M + K M^-1 K) x = b, applied factored (the reference's
+  SquaredLaplacianOperator path), FSAI width 8 on the assembled Q.
+
+The engine keeps *systems* (mesh, operator, FSAI factor) resident in HBM under a
+byte budget.  A repeated mesh reuses its system as is; a new mesh whose topology
+is already cached refreshes a cached factor numerically on the stored supports
+(``refit_clone``, the reuse path of fsai_reuse.py) instead of rerunning the
+adaptive search; an unseen topology builds a factor.  Systems live on a pool of
+HIP streams (default priority); every admitted request owns a *slot* (its Krylov
+workspace, captured as a HIP graph of ``chunk`` CG steps) on its system's stream.
+The loop admits arriving requests whenever a slot and budget are free, then
+round-robins chunk launches over all admitted requests, so requests that arrive
+while others are on the GPU join at chunk boundaries and their work overlaps on
+the device; host reads of solver state are asynchronous (pinned buffers + events).
+
+Stopping is certified (refinement.field_bounds): a request finishes when, for
+every right-hand side, ||b - A x||_{M^-1} / (alpha ||x||_M) <= tol with the
+residual recomputed in compensated float64.  Two precision policies:
+
+* ``"f64"``   -- CG in float64 on the float64 operator with a float32 factor; when
+  the recursive residual meets its target the bound is checked and, where it
+  fails, that right-hand side's target is tightened and CG resumes;
+* ``"mixed"`` -- iterative refinement: float32 CG on a float32 copy of the operator
+  with a row-scaled bfloat16 factor (built in float64) solves for corrections
+  of the float64 iterate against compensated float64 residuals.
+
+Results do not depend on what else is in flight: every kernel reduces in a fixed
+order, row mappings are chosen without timing, and a converged right-hand side's
+further steps are no-ops, so a request's solution is bit-identical alone and in
+any mix given the same factor provenance (built / cached / refit).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import assembly
+from .fsai import FSAI
+from .fsai_reuse import refit_clone
+from .linear import KrylovWork
+from .refinement import TrueResidual, field_bounds
+from .sparse import BsrMatrix
+from .sparse_operator import SparseOperator
+from .squared_laplacian import SquaredLaplacianOperator
+
+KINDS = ("laplacian", "biharmonic")
+_ITER, _RR, _TARGET = 4, 1, 2  # Krylov state fields (include/wpc.h WPC_KS_*)
+
+
+@dataclass(frozen=True)
+class MeshSpec:
+    side: int  # vertices per edge of the unit cube
+    seed: int  # jitter seed
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the stream starts
+    mesh: MeshSpec
+    kind: str
+    nrhs: int
+    rhs_seed: int
+    alpha: float = 1.0
+
+
+def request_stream(count=512, rate=200.0, seed=7, sides=(16, 96), repeat=0.5, max_rhs=8,
+                   kinds=KINDS):
+    """Poisson arrivals (mean `rate`/s); each request is, with probability `repeat`,
+    a uniformly chosen earlier (mesh, kind) system, otherwise a new mesh of side
+    U{sides} with a uniformly chosen kind; 1..max_rhs right-hand sides."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    seen, out = [], []
+    for rid in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        if seen and rng.random() < repeat:
+            mesh, kind = seen[int(rng.integers(len(seen)))]
+        else:
+            mesh = MeshSpec(int(rng.integers(sides[0], sides[1] + 1)), int(rng.integers(1 << 31)))
+            kind = kinds[int(rng.integers(len(kinds)))]
+            seen.append((mesh, kind))
+        out.append(Request(rid, t, mesh, kind, int(rng.integers(1, max_rhs + 1)),
+                           int(rng.integers(1 << 31))))
+    return out
+
+
+@dataclass
+class ServingConfig:
+    budget_bytes: float = 200e9
+    streams: int = 4
+    max_active: int = 64
+    width: int = 8
+    tol: float = 1e-8
+    precision: dict = field(default_factory=lambda: {"laplacian": "mixed", "biharmonic": "f64"})
+    factor_dtype: dict = field(default_factory=lambda: {"mixed": torch.bfloat16,
+                                                        "f64": torch.float32})
+    inner_tol: float = 1e-4
+    chunk: int = 25
+    maxiter: int = 200000
+    max_outer: int = 16
+    keep_solutions: bool = True
+
+
+@dataclass
+class Result:
+    rid: int
+    kind: str
+    side: int
+    n: int
+    nrhs: int
+    arrival: float
+    admitted: float
+    finished: float
+    iterations: list
+    outer: int
+    field_bound: list
+    relative_residual: list
+    converged: bool
+    factor: str  # "built", "refit" or "cached"
+    x: np.ndarray | None = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def _convert(t: torch.Tensor, dtype) -> torch.Tensor:
+    out = torch.empty(t.shape, dtype=dtype, device=t.device)
+    nat.call("wpc_convert", t.numel(), nat.ptr(t), nat.code(t.dtype), nat.ptr(out), nat.code(dtype),
+             nat.stream())
+    return out
+
+
+class System:
+    """Device-resident mesh operator and FSAI factor (built on the setup stream)."""
+
+    def __init__(self, cfg: ServingConfig, kind, spec: MeshSpec, alpha, stream, template=None):
+        if kind not in KINDS:
+            raise ValueError(f"unknown system kind {kind!r}")
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.kind, self.spec, self.alpha, self.stream = kind, spec, float(alpha), stream
+        self.policy = cfg.precision[kind]
+        self.slots = {}
+        self.active = 0
+        self.bytes = 0
+        vertices, tets = assembly.jittered_grid(spec.side, spec.seed, dev)
+        K, mass = assembly.p1_laplacian(vertices, tets)
+        del vertices, tets
+        n = K.nrow
+        self.n, self.mass = n, mass
+        ident = torch.arange(n, dtype=torch.int32, device=dev)
+        if kind == "laplacian":
+            A = assembly.add_diagonal(K, mass, self.alpha)
+            del K
+            self.op64 = SparseOperator(A, row_lanes=1)
+            self.op64.source = self.op64.scalar
+            del A
+            source = self.op64.scalar
+        else:
+            self.op64 = SquaredLaplacianOperator(K, mass, ident, row_lanes=1,
+                                                 mass_weight=self.alpha)
+            del K
+            source = assembly.squared_operator(self.op64.L, mass, self.alpha)
+        if template is not None:
+            self.P = refit_clone(template.P, source, topology=template.topology)
+            self.topology = template.topology
+            self.provenance = "refit"
+        else:
+            self.P = FSAI(source, max_row_size=cfg.width, factor_dtype=cfg.factor_dtype[self.policy])
+            self.topology = (source.offsets, source.columns)
+            self.provenance = "built"
+        self.P.release_source()
+        del source
+        if self.policy == "mixed":
+            if kind == "laplacian":
+                S = self.op64.scalar
+                self.inner = SparseOperator(
+                    BsrMatrix(n, n, (1, 1), S.offsets, S.columns, _convert(S.values, torch.float32)),
+                    row_lanes=1)
+            else:
+                L = self.op64.L
+                self.inner = SquaredLaplacianOperator(
+                    BsrMatrix(n, n, (1, 1), L.offsets, L.columns, _convert(L.values, torch.float32)),
+                    _convert(mass, torch.float32), ident, row_lanes=1, mass_weight=self.alpha)
+            self.vdtype = torch.float32
+        else:
+            self.inner = self.op64
+            self.vdtype = torch.float64
+        # Row mappings chosen without timing (results independent of load), and
+        # every scratch buffer sized for 8 right-hand sides up front, so captured
+        # graphs never see a reallocation.
+        plans = list(self.P.plans()) + [self.inner._stream()]
+        for p in plans:
+            if p.fixed is None:
+                p.fixed = p.heuristic()
+            p.partials(8)
+        self.P._scratch(self.vdtype, n * 8)
+        if isinstance(self.inner, SquaredLaplacianOperator):
+            self.inner._scratch(self.vdtype, n * 8)
+
+    def slot(self, nvec):
+        for s in self.slots.setdefault(nvec, []):
+            if not s.busy:
+                return s
+        s = Slot(self, nvec)
+        self.slots[nvec].append(s)
+        return s
+
+    def release(self):
+        for slots in self.slots.values():
+            for s in slots:
+                s.release()
+        self.slots = {}
+
+
+class Slot:
+    """One request's device workspace on a system (reused by later requests)."""
+
+    def __init__(self, system: System, nvec: int):
+        self.system, self.nvec = system, nvec
+        n = system.n
+        dev = system.mass.device
+        before = torch.cuda.memory_allocated()
+        with torch.cuda.stream(system.stream):
+            size = n * nvec
+            self.X = torch.zeros(size, dtype=torch.float64, device=dev)
+            self.B = torch.empty(size, dtype=torch.float64, device=dev)
+            self.res = TrueResidual(system.op64, system.mass, nvec)
+            if system.policy == "mixed":
+                self.Rin = torch.empty(size, dtype=torch.float32, device=dev)
+                self.C = torch.empty(size, dtype=torch.float32, device=dev)
+                self.work = KrylovWork("cg", system.inner, system.P, n, nvec, torch.float32, dev)
+                self.x_iter = self.C
+            else:
+                self.work = KrylovWork("cg", system.op64, system.P, n, nvec, torch.float64, dev)
+                self.x_iter = self.X
+        self.bytes = torch.cuda.memory_allocated() - before
+        self.state_host = torch.empty(12 * nvec, dtype=torch.float64, pin_memory=True)
+        self.norms_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
+        self.x_host = torch.empty(n * nvec, dtype=torch.float64, pin_memory=True)
+        self.graph = None
+        self.graph_steps = 0
+        self.warm = False
+        self.busy = False
+
+    def chunk(self, steps):
+        """`steps` CG steps on the current stream: eager the first time (loads
+        code objects), then captured once into a HIP graph and replayed."""
+        if self.graph is not None and self.graph_steps == steps:
+            nat.call("wpc_graph_launch", self.graph, nat.stream())
+            return
+        if not self.warm:
+            for _ in range(steps):
+                self.work.step(self.x_iter)
+            self.warm = True
+            return
+        handle = ctypes.c_void_p()
+        nat.call("wpc_graph_begin", nat.stream())
+        try:
+            for _ in range(steps):
+                self.work.step(self.x_iter)
+        finally:
+            rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
+        if rc != 0:
+            raise nat.NativeError(f"wpc_graph_end failed: {rc}")
+        if self.graph is not None:
+            nat.lib().wpc_graph_destroy(self.graph)
+        self.graph, self.graph_steps = handle.value, steps
+        nat.call("wpc_graph_launch", self.graph, nat.stream())
+
+    def release(self):
+        if self.graph is not None:
+            nat.lib().wpc_graph_destroy(self.graph)
+            self.graph = None
+
+
+class Job:
+    """State machine of one admitted request (all device work on the system stream)."""
+
+    def __init__(self, eng: "ServingEngine", req: Request, system: System, slot: Slot,
+                 provenance: str, t_admit: float):
+        self.eng, self.req, self.system, self.slot = eng, req, system, slot
+        self.cfg = eng.cfg
+        self.provenance = provenance
+        self.t_admit = t_admit
+        self.pending = deque()  # (event, what)
+        self.chunks_out = 0
+        self.phase = "init"
+        self.outer = 0
+        self.mask = (1 << req.nrhs) - 1
+        self.iterations = np.zeros(req.nrhs, dtype=np.int64)
+        self.inner_iters = np.zeros(req.nrhs, dtype=np.int64)
+        self.bnorm = None
+        self.bounds = np.full(req.nrhs, np.inf)
+        self.rel = np.ones(req.nrhs)
+        self.far = True
+        self.best = np.full(req.nrhs, np.inf)
+        self.stalls = 0
+        self.want_check = False
+        self.finished = False
+        self.result = None
+        slot.busy = True
+        system.active += 1
+
+    # ------------------------------------------------------------------ helpers
+    def _record(self, what):
+        ev = torch.cuda.Event()
+        ev.record(self.system.stream)
+        self.pending.append((ev, what))
+
+    def _start_solve(self, b, x, tol):
+        x.zero_()
+        self.slot.work.start(b, x, tol, 0.0)
+
+    def _residual(self):
+        s = self.slot
+        s.res.run(s.X, s.B)
+        s.norms_host.copy_(s.res.norms, non_blocking=True)
+        self._record("residual")
+
+    def _finish(self):
+        s = self.slot
+        s.x_host[: s.X.numel()].copy_(s.X, non_blocking=True)
+        self._record("result")
+        self.phase = "result"
+
+    # ------------------------------------------------------------------ driving
+    def advance(self) -> bool:
+        did = False
+        while self.pending and self.pending[0][0].query():
+            _, what = self.pending.popleft()
+            self._on(what)
+            did = True
+        if not self.finished and self._enqueue():
+            did = True
+        return did
+
+    def _enqueue(self) -> bool:
+        s = self.slot
+        st = self.system.stream
+        with torch.cuda.stream(st):
+            if self.phase == "init":
+                gen = torch.Generator(device=s.X.device)
+                gen.manual_seed(self.req.rhs_seed)
+                torch.randn((self.system.n, self.req.nrhs), generator=gen,
+                            out=s.B.view(self.system.n, self.req.nrhs))
+                s.X.zero_()
+                self._residual()  # x = 0: r = b gives ||b||_{M^-1}
+                self.phase = "wait_residual"
+                return True
+            if self.phase == "solve":
+                if self.want_check:
+                    if self.chunks_out == 0:
+                        self._after_solve()
+                        return True
+                    return False
+                depth = 2 if self.far else 1
+                if self.chunks_out < depth:
+                    s.chunk(self.cfg.chunk)
+                    s.state_host.copy_(s.work.state, non_blocking=True)
+                    self._record("state")
+                    self.chunks_out += 1
+                    return True
+        return False
+
+    def _after_solve(self):
+        """Recursive targets met: enqueue the certification step."""
+        s = self.slot
+        self.want_check = False
+        iters = s.state_host.view(12, -1)[_ITER].numpy()[: self.req.nrhs]
+        if self.system.policy == "mixed":
+            self.iterations += iters.astype(np.int64)
+            nat.call("wpc_refine_update", self.system.n, self.req.nrhs, nat.ptr(s.X), nat.ptr(s.C),
+                     nat.code(torch.float32), nat.ptr(s.res.norms), self.mask, nat.stream())
+        else:
+            self.iterations = iters.astype(np.int64)
+        self._residual()
+        self.phase = "wait_residual"
+
+    def _on(self, what):
+        s = self.slot
+        k = self.req.nrhs
+        if what == "state":
+            self.chunks_out -= 1
+            st = s.state_host.view(12, -1).numpy()[:, :k]
+            rr, target = st[_RR], st[_TARGET]
+            self.far = bool(np.all(rr > 1e4 * target))
+            total = self.iterations + st[_ITER] if self.system.policy == "mixed" else st[_ITER]
+            if np.all(rr <= target) or np.max(total) >= self.cfg.maxiter:
+                self.want_check = True
+            return
+        if what == "residual":
+            norms = s.norms_host.numpy()[: 2 * k].copy()
+            if self.bnorm is None:
+                self.bnorm = np.sqrt(norms[:k])
+                self._begin_solve(first=True, norms=norms)
+                return
+            self.bounds = field_bounds(norms, k, self.system.alpha)
+            self.rel = np.sqrt(norms[:k]) / np.where(self.bnorm > 0, self.bnorm, 1.0)
+            self._begin_solve(first=False, norms=norms)
+            return
+        if what == "result":
+            self.finished = True
+            eng = self.eng
+            x = None
+            if self.cfg.keep_solutions:
+                x = s.x_host[: self.system.n * k].numpy().reshape(self.system.n, k).copy()
+            self.result = Result(
+                self.req.rid, self.req.kind, self.system.spec.side, self.system.n, k,
+                self.req.arrival, self.t_admit, eng.clock(), [int(v) for v in self.iterations],
+                self.outer, [float(v) for v in self.bounds], [float(v) for v in self.rel],
+                bool(np.all(self.bounds <= self.cfg.tol)), self.provenance, x)
+
+    def _begin_solve(self, first, norms):
+        """After a residual: finish, or start/continue the Krylov phase."""
+        k = self.req.nrhs
+        tol = self.cfg.tol
+        s = self.slot
+        if first:
+            self.rel = np.ones(k)
+            fail = self.bnorm > 0
+        else:
+            fail = self.bounds > tol
+        if not np.any(fail):
+            with torch.cuda.stream(self.system.stream):
+                self._finish()
+            return
+        if not first:
+            improved = self.bounds < 0.5 * self.best
+            self.best = np.minimum(self.best, self.bounds)
+            self.stalls = 0 if np.any(improved & fail) else self.stalls + 1
+            total = int(np.max(self.iterations))
+            if self.stalls >= 3 or total >= self.cfg.maxiter or self.outer >= self.cfg.max_outer:
+                with torch.cuda.stream(self.system.stream):
+                    self._finish()
+                return
+        self.outer += 1
+        self.far = True
+        with torch.cuda.stream(self.system.stream):
+            if self.system.policy == "mixed":
+                self.mask = int(sum(1 << v for v in range(k) if fail[v]))
+                nat.call("wpc_refine_prepare", self.system.n, k, nat.ptr(s.res.r),
+                         nat.ptr(s.res.norms), self.mask, nat.ptr(s.Rin), nat.code(torch.float32),
+                         nat.stream())
+                self._start_solve(s.Rin, s.C, self.cfg.inner_tol)
+            elif first:
+                # recursive target ~ tol / sqrt(n) relative to ||b||_2 (the bound
+                # scales like sqrt(n) times the relative residual on these meshes);
+                # certification then tightens individual right-hand sides
+                self._start_solve(s.B, s.X, tol / math.sqrt(self.system.n))
+            else:
+                for v in range(k):
+                    if fail[v]:
+                        factor = min(0.25, (0.5 * tol / self.bounds[v]) ** 2)
+                        nat.call("wpc_krylov_scale_target", k, nat.ptr(s.work.state), v,
+                                 float(factor), nat.stream())
+        self.phase = "solve"
+
+    def release(self):
+        self.slot.busy = False
+        self.system.active -= 1
+
+
+class ServingEngine:
+    """Admission, factor cache and the chunk scheduler (one GPU, one host thread)."""
+
+    def __init__(self, config: ServingConfig | None = None, device=None):
+        self.cfg = config or ServingConfig()
+        self.device = torch.device("cuda", 0) if device is None else torch.device(device)
+        torch.cuda.set_device(self.device)
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(self.cfg.streams)]
+        self.setup_stream = torch.cuda.Stream(self.device)
+        self._next = 0
+        self.systems = OrderedDict()  # (kind, mesh, alpha) -> System, LRU order
+        self.templates = {}  # (kind, side, alpha, policy) -> System whose factor is refit
+        self.t0 = None
+        self.stats = dict(built=0, refit=0, cached=0, evicted=0, over_budget=0, setup_s=0.0,
+                          peak_bytes=0)
+
+    def clock(self):
+        return time.perf_counter() - self.t0
+
+    def used_bytes(self):
+        return sum(s.bytes + sum(x.bytes for v in s.slots.values() for x in v)
+                   for s in self.systems.values())
+
+    @staticmethod
+    def estimate(req: Request, policy: str):
+        n = req.mesh.side ** 3
+        system = (1500 if req.kind == "biharmonic" else 700) * n
+        slot = (80 if policy == "f64" else 56) * n * req.nrhs
+        return system, slot
+
+    def _make_room(self, need, keep=None):
+        while self.used_bytes() + need > self.cfg.budget_bytes:
+            victim = next((k for k, s in self.systems.items() if s.active == 0 and s is not keep),
+                          None)
+            if victim is None:
+                return False
+            s = self.systems.pop(victim)
+            for key, t in list(self.templates.items()):
+                if t is s:
+                    del self.templates[key]
+            s.release()
+            self.stats["evicted"] += 1
+        return True
+
+    def _system(self, req: Request):
+        key = (req.kind, req.mesh, req.alpha)
+        s = self.systems.get(key)
+        if s is not None:
+            self.systems.move_to_end(key)
+            self.stats["cached"] += 1
+            return s, "cached"
+        policy = self.cfg.precision[req.kind]
+        tkey = (req.kind, req.mesh.side, req.alpha, policy)
+        template = self.templates.get(tkey)
+        stream = self.streams[self._next % len(self.streams)]
+        self._next += 1
+        t = time.perf_counter()
+        before = torch.cuda.memory_allocated()
+        with torch.cuda.stream(self.setup_stream):
+            s = System(self.cfg, req.kind, req.mesh, req.alpha, stream, template)
+            ready = torch.cuda.Event()
+            ready.record(self.setup_stream)
+        stream.wait_event(ready)
+        s.bytes = torch.cuda.memory_allocated() - before
+        self.stats["setup_s"] += time.perf_counter() - t
+        self.systems[key] = s
+        if template is None:
+            self.templates[tkey] = s
+        self.stats[s.provenance] += 1
+        return s, s.provenance
+
+    def _admit(self, req: Request, active) -> bool:
+        if len(active) >= self.cfg.max_active:
+            return False
+        key = (req.kind, req.mesh, req.alpha)
+        s = self.systems.get(key)
+        sys_b, slot_b = self.estimate(req, self.cfg.precision[req.kind])
+        need = (0 if s is not None else sys_b) + slot_b
+        if not self._make_room(need, keep=s):
+            if active:
+                return False
+            # nothing in flight and still over budget: admit alone rather than stall
+            self.stats["over_budget"] += 1
+        system, provenance = self._system(req)
+        slot = system.slot(req.nrhs)
+        active.append(Job(self, req, system, slot, provenance, self.clock()))
+        self.stats["peak_bytes"] = max(self.stats["peak_bytes"], self.used_bytes())
+        return True
+
+    def serve(self, requests, realtime=True, on_result=None):
+        """Run the request list to completion; returns Results in request order.
+        With ``realtime`` each request becomes visible at its arrival time; otherwise
+        all are available at once (admission is still limited by slots and budget)."""
+        self.t0 = time.perf_counter()
+        queue = deque(sorted(requests, key=lambda r: r.arrival))
+        active, results = [], {}
+        while queue or active:
+            now = self.clock()
+            while queue and (not realtime or queue[0].arrival <= now):
+                if not self._admit(queue[0], active):
+                    break
+                queue.popleft()
+            progressed = False
+            for job in active:
+                progressed |= job.advance()
+            done = [j for j in active if j.finished]
+            for j in done:
+                active.remove(j)
+                j.release()
+                results[j.req.rid] = j.result
+                if on_result is not None:
+                    on_result(j.result)
+            if not progressed and not done:
+                if active:
+                    time.sleep(50e-6)
+                elif queue and realtime:
+                    time.sleep(max(0.0, min(queue[0].arrival - self.clock(), 0.01)))
+        return [results[r.rid] for r in requests]
