@@ -408,18 +408,16 @@ int wpc_dd_residual(int n, const int* offsets, const int* columns, const double*
                     const double* b, double* t_hi, double* t_lo, double* r, int nvec,
                     double* partials, double* norms, void* stream);
 
-/* Mixed-precision refinement steps, per right-hand side v with bit v of mask set
+/* Iterative-refinement steps, per right-hand side v with bit v of mask set
  * (s_v = sqrt(norms[v]), or 1 when zero):
  *   prepare: out = r / s_v (others 0), stored as out_dtype (f32 or f64)
- *   update:  x += s_v * c. */
+ *   update:  d = s_v * c (others 0); x += d; c_out = d when c_out is not NULL
+ * (c_out lets the caller certify x + d with wpc_dd_residual(b = r, x = c_out)).
+ * b and r of wpc_dd_residual may be the same array. */
 int wpc_refine_prepare(int n, int nvec, const double* r, const double* norms, int mask, void* out,
                        int out_dtype, void* stream);
 int wpc_refine_update(int n, int nvec, double* x, const void* c, int c_dtype, const double* norms,
-                      int mask, void* stream);
-
-/* state[TARGET][v] *= factor (tightens one slot's stopping threshold; a slot
- * frozen by the old threshold resumes stepping). */
-int wpc_krylov_scale_target(int nvec, double* state, int v, double factor, void* stream);
+                      int mask, double* c_out, void* stream);
 
 /* values[diag(i)] += alpha * d[i] on a canonical CSR matrix (forms K + alpha*M);
  * status[0] counts rows without a stored diagonal. */

@@ -139,6 +139,43 @@ def test_field_bound_is_an_upper_bound():
     assert np.all(actual <= bound * (1 + 1e-6)), (actual, bound)
 
 
+def test_witness_bound_is_an_upper_bound():
+    # Stiff factored biharmonic with a manufactured solution: b = A x* is formed in
+    # compensated arithmetic, so x* is exact to ~1e-14.  For x' = fl(x + c) with an
+    # accurate correction c, the witness certificate dominates the true field error
+    # and stays near 2 eps, while the lambda_min bound of x' is held up by the
+    # rounding of x' itself.
+    from warp_preconditioners import SquaredLaplacianOperator, workloads
+    from warp_preconditioners.refinement import TrueResidual, field_bounds, witness_bounds
+
+    v, _, K, m = _mesh(14, 4)
+    n = K.nrow
+    op = SquaredLaplacianOperator(K, m, torch.arange(n, dtype=torch.int32, device=dev()),
+                                  mass_weight=1.0)
+    vh = host(v)
+    xs = np.cos(np.pi * vh[:, 0]) + vh[:, 1] * vh[:, 2]
+    res = TrueResidual(op, m, 1)
+    zero = torch.zeros(n, dtype=torch.float64, device=dev())
+    res.run(torch.tensor(-xs, device=dev()), zero)
+    b = res.r.clone()  # = A x*
+    rng = np.random.default_rng(8)
+    x = xs + 1e-6 * np.sin(3 * vh[:, 1]) + 1e-9 * rng.standard_normal(n)
+    prev = host(res.run(torch.tensor(x, device=dev()), b)).copy()
+    r = host(res.r).copy()
+    Qh = workloads.squared_operator(_csr(op.L), host(m), 1.0).tocsc()
+    c = sla.splu(Qh).solve(r)
+    w = torch.empty(2, dtype=torch.float64, device=dev())
+    res.run(torch.tensor(c, device=dev()), res.r, norms=w)
+    x1 = x + c
+    nxt = host(res.run(torch.tensor(x1, device=dev()), b)).copy()
+    wb = witness_bounds(nxt, host(w), prev, 1, 1.0)[0]
+    pb = field_bounds(nxt, 1, 1.0)[0]
+    mh = host(m)
+    actual = np.sqrt(np.sum(mh * (x1 - xs) ** 2) / np.sum(mh * xs ** 2))
+    assert actual <= wb + 1e-13, (actual, wb)
+    assert wb < 1e-11 and pb > 10 * wb, (wb, pb)
+
+
 def test_refine_prepare_update():
     from warp_preconditioners import _native as nat
 
@@ -158,11 +195,13 @@ def test_refine_prepare_update():
     np.testing.assert_array_equal(host(out).reshape(n, k), want)
     x = torch.tensor(rng.standard_normal(n * k), device=dev())
     x0 = host(x).reshape(n, k).copy()
+    c64 = torch.full((n * k,), 7.0, dtype=torch.float64, device=dev())
     nat.call("wpc_refine_update", n, k, nat.ptr(x), nat.ptr(out), nat.code(torch.float32),
-             nat.ptr(nd), mask, nat.stream())
-    want_x = x0 + scale * want.astype(np.float64)
-    want_x[:, 1] = x0[:, 1]
-    np.testing.assert_array_equal(host(x).reshape(n, k), want_x)
+             nat.ptr(nd), mask, nat.ptr(c64), nat.stream())
+    d = scale * want.astype(np.float64)
+    d[:, 1] = 0.0
+    np.testing.assert_array_equal(host(c64).reshape(n, k), d)
+    np.testing.assert_array_equal(host(x).reshape(n, k), x0 + d)
 
 
 def test_native_graph_capture_replays():

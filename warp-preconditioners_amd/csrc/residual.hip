@@ -92,8 +92,10 @@ template <int NV>
 __global__ __launch_bounds__(kResidualThreads) void dd_residual_kernel(
     int n, const int* __restrict__ off, const int* __restrict__ col, const double* __restrict__ val,
     const double* __restrict__ mass, double alpha, const double* __restrict__ x,
-    const double* __restrict__ t_hi, const double* __restrict__ t_lo, const double* __restrict__ b,
-    double* __restrict__ r, double* __restrict__ partials) {
+    const double* __restrict__ t_hi, const double* __restrict__ t_lo, const double* b, double* r,
+    double* __restrict__ partials) {
+    // b and r may alias (r overwrites b in place): each thread reads b only at its
+    // own row before writing r there.
     __shared__ double red[kResidualThreads];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double q[2 * NV];
@@ -185,21 +187,19 @@ __global__ __launch_bounds__(256) void refine_prepare_kernel(long total, int nve
     out[i] = active ? (TO)(r[i] / refine_scale(norms, v)) : (TO)0;
 }
 
-// x += active_v ? scale_v * c : 0.
+// d = active_v ? scale_v * c : 0; x += d; optionally c_out = d (float64).
 template <class TC>
 __global__ __launch_bounds__(256) void refine_update_kernel(long total, int nvec,
                                                             double* __restrict__ x,
                                                             const TC* __restrict__ c,
                                                             const double* __restrict__ norms,
-                                                            int mask) {
+                                                            int mask, double* __restrict__ c_out) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const int v = (int)(i % nvec);
-    if ((mask >> v) & 1) x[i] += refine_scale(norms, v) * (double)c[i];
-}
-
-__global__ void scale_target_kernel(int nvec, double* state, int v, double factor) {
-    if (threadIdx.x == 0 && v < nvec) state[WPC_KS_TARGET * nvec + v] *= factor;
+    const double d = ((mask >> v) & 1) ? refine_scale(norms, v) * (double)c[i] : 0.0;
+    x[i] += d;
+    if (c_out != nullptr) c_out[i] = d;
 }
 
 // values[diag(i)] += alpha * d[i]; rows without a stored diagonal count in status.
@@ -285,7 +285,7 @@ int wpc_refine_prepare(int n, int nvec, const double* r, const double* norms, in
 }
 
 int wpc_refine_update(int n, int nvec, double* x, const void* c, int c_dtype, const double* norms,
-                      int mask, void* stream) {
+                      int mask, double* c_out, void* stream) {
     const long total = (long)n * nvec;
     if (total <= 0) return WPC_OK;
     if (nvec < 1 || nvec > 8) return WPC_EARG;
@@ -293,19 +293,12 @@ int wpc_refine_update(int n, int nvec, double* x, const void* c, int c_dtype, co
     hipStream_t s = as_stream(stream);
     if (c_dtype == WPC_F32)
         hipLaunchKernelGGL(refine_update_kernel<float>, dim3(grid), dim3(256), 0, s, total, nvec, x,
-                           (const float*)c, norms, mask);
+                           (const float*)c, norms, mask, c_out);
     else if (c_dtype == WPC_F64)
         hipLaunchKernelGGL(refine_update_kernel<double>, dim3(grid), dim3(256), 0, s, total, nvec, x,
-                           (const double*)c, norms, mask);
+                           (const double*)c, norms, mask, c_out);
     else
         return WPC_EDTYPE;
-    return launch_status();
-}
-
-int wpc_krylov_scale_target(int nvec, double* state, int v, double factor, void* stream) {
-    if (v < 0 || v >= nvec) return WPC_EARG;
-    hipLaunchKernelGGL(scale_target_kernel, dim3(1), dim3(64), 0, as_stream(stream), nvec, state, v,
-                       factor);
     return launch_status();
 }
 

@@ -55,14 +55,40 @@ class TrueResidual:
                                     dtype=torch.float64, device=dev)
         self.norms = torch.empty(2 * nvec, dtype=torch.float64, device=dev)
 
-    def run(self, x: torch.Tensor, b: torch.Tensor):
-        """Launch on the current stream; results in self.r and self.norms."""
+    def run(self, x: torch.Tensor, b: torch.Tensor, norms: torch.Tensor | None = None):
+        """Launch on the current stream: self.r = b - A x (b may be self.r itself) and
+        norms (default self.norms) = [||r||^2_{M^-1}, ||x||^2_M] per right-hand side."""
         A = self.matrix
+        out = self.norms if norms is None else norms
         nat.call("wpc_dd_residual", self.n, nat.ptr(A.offsets), nat.ptr(A.columns),
                  nat.ptr(A.values), nat.ptr(self.mass), nat.ptr(self.inverse), float(self.alpha),
                  nat.ptr(x), nat.ptr(b), nat.ptr(self.t_hi), nat.ptr(self.t_lo), nat.ptr(self.r),
-                 self.nvec, nat.ptr(self.partials), nat.ptr(self.norms), nat.stream())
-        return self.norms
+                 self.nvec, nat.ptr(self.partials), nat.ptr(out), nat.stream())
+        return out
+
+
+EPS = 2.0 ** -53
+
+
+def witness_bounds(norms_next, norms_witness, norms_prev, nvec: int, alpha: float) -> np.ndarray:
+    """Certificate for x' = fl(x + c), with c the correction computed from the
+    residual r of x (the witness):
+
+        ||x' - x*||_M <= ||r - A c||_{M^-1} / alpha + eps ||x'||_M
+
+    since x + c - x* = c - A^-1 r and ||v||^2_M <= v^T A v / alpha <= ||A v||^2_{M^-1}
+    / alpha^2 for A >= alpha M.  Unlike field_bounds(x'), this does not see the
+    rounding noise of x' itself, which the lambda_min bound charges at alpha.
+    norms_witness = [||r - A c||^2_{M^-1}, ...] (wpc_dd_residual with b = r, x = c),
+    norms_prev = [||r||^2_{M^-1}, ...] of x (r stored rounded: slack eps ||r||),
+    norms_next = [..., ||x'||^2_M]."""
+    rho = np.sqrt(np.asarray(norms_witness, dtype=np.float64)[:nvec])
+    r = np.sqrt(np.asarray(norms_prev, dtype=np.float64)[:nvec])
+    x = np.sqrt(np.asarray(norms_next, dtype=np.float64)[nvec:2 * nvec])
+    num = (1.0 + 4 * EPS) * rho + EPS * r
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out = np.where(num == 0.0, 0.0, num / (alpha * x)) + 2 * EPS
+    return np.where(np.isnan(out), np.inf, out)
 
 
 def field_bounds(norms, nvec: int, alpha: float) -> np.ndarray:

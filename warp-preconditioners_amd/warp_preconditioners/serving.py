@@ -19,16 +19,21 @@ round-robins chunk launches over all admitted requests, so requests that arrive
 while others are on the GPU join at chunk boundaries and their work overlaps on
 the device; host reads of solver state are asynchronous (pinned buffers + events).
 
-Stopping is certified (refinement.field_bounds): a request finishes when, for
-every right-hand side, ||b - A x||_{M^-1} / (alpha ||x||_M) <= tol with the
-residual recomputed in compensated float64.  Two precision policies:
+Stopping is certified: a request finishes when, for every right-hand side, an
+upper bound on the relative mass-weighted field error is <= tol.  The float64
+iterate is improved by iterative refinement: each outer step solves A c = r/||r||
+with preconditioned CG to a relative residual ``inner_tol`` and adds the scaled
+correction.  The bound is the smaller of refinement.field_bounds(x') =
+||b - A x'||_{M^-1} / (alpha ||x'||_M) and refinement.witness_bounds, which uses
+the correction itself: ||(r - A c)||_{M^-1} / (alpha ||x'||_M) + 2 eps, all
+residuals recomputed with compensated float64.  The witness form matters for the
+factored biharmonic operator, whose lambda_max(M^-1 A)/alpha reaches ~1e10 at
+96^3: there the rounding of x' to float64 alone puts ~1e-7 into the first form,
+while the true error is ~1e-16 of that size.  Two inner precisions:
 
-* ``"f64"``   -- CG in float64 on the float64 operator with a float32 factor; when
-  the recursive residual meets its target the bound is checked and, where it
-  fails, that right-hand side's target is tightened and CG resumes;
-* ``"mixed"`` -- iterative refinement: float32 CG on a float32 copy of the operator
-  with a row-scaled bfloat16 factor (built in float64) solves for corrections
-  of the float64 iterate against compensated float64 residuals.
+* ``"f64"``   -- float64 CG on the float64 operator with a float32 factor;
+* ``"mixed"`` -- float32 CG on a float32 copy of the operator with a row-scaled
+  bfloat16 factor (built in float64).
 
 Results do not depend on what else is in flight: every kernel reduces in a fixed
 order, row mappings are chosen without timing, and a converged right-hand side's
@@ -39,7 +44,6 @@ any mix given the same factor provenance (built / cached / refit).
 from __future__ import annotations
 
 import ctypes
-import math
 import time
 from collections import OrderedDict, deque
 from dataclasses import dataclass, field
@@ -52,7 +56,7 @@ from . import assembly
 from .fsai import FSAI
 from .fsai_reuse import refit_clone
 from .linear import KrylovWork
-from .refinement import TrueResidual, field_bounds
+from .refinement import TrueResidual, field_bounds, witness_bounds
 from .sparse import BsrMatrix
 from .sparse_operator import SparseOperator
 from .squared_laplacian import SquaredLaplacianOperator
@@ -109,7 +113,7 @@ class ServingConfig:
     precision: dict = field(default_factory=lambda: {"laplacian": "mixed", "biharmonic": "f64"})
     factor_dtype: dict = field(default_factory=lambda: {"mixed": torch.bfloat16,
                                                         "f64": torch.float32})
-    inner_tol: float = 1e-4
+    inner_tol: dict = field(default_factory=lambda: {"mixed": 1e-4, "f64": 1e-6})
     chunk: int = 25
     maxiter: int = 200000
     max_outer: int = 16
@@ -241,17 +245,17 @@ class Slot:
             self.X = torch.zeros(size, dtype=torch.float64, device=dev)
             self.B = torch.empty(size, dtype=torch.float64, device=dev)
             self.res = TrueResidual(system.op64, system.mass, nvec)
-            if system.policy == "mixed":
-                self.Rin = torch.empty(size, dtype=torch.float32, device=dev)
-                self.C = torch.empty(size, dtype=torch.float32, device=dev)
-                self.work = KrylovWork("cg", system.inner, system.P, n, nvec, torch.float32, dev)
-                self.x_iter = self.C
-            else:
-                self.work = KrylovWork("cg", system.op64, system.P, n, nvec, torch.float64, dev)
-                self.x_iter = self.X
+            vdt = system.vdtype
+            self.Rin = torch.empty(size, dtype=vdt, device=dev)
+            self.C = torch.empty(size, dtype=vdt, device=dev)
+            self.c64 = torch.empty(size, dtype=torch.float64, device=dev)
+            self.norms_w = torch.empty(2 * nvec, dtype=torch.float64, device=dev)
+            self.work = KrylovWork("cg", system.inner, system.P, n, nvec, vdt, dev)
+            self.x_iter = self.C
         self.bytes = torch.cuda.memory_allocated() - before
         self.state_host = torch.empty(12 * nvec, dtype=torch.float64, pin_memory=True)
         self.norms_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
+        self.norms_w_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
         self.x_host = torch.empty(n * nvec, dtype=torch.float64, pin_memory=True)
         self.graph = None
         self.graph_steps = 0
@@ -383,12 +387,12 @@ class Job:
         s = self.slot
         self.want_check = False
         iters = s.state_host.view(12, -1)[_ITER].numpy()[: self.req.nrhs]
-        if self.system.policy == "mixed":
-            self.iterations += iters.astype(np.int64)
-            nat.call("wpc_refine_update", self.system.n, self.req.nrhs, nat.ptr(s.X), nat.ptr(s.C),
-                     nat.code(torch.float32), nat.ptr(s.res.norms), self.mask, nat.stream())
-        else:
-            self.iterations = iters.astype(np.int64)
+        self.iterations += iters.astype(np.int64)
+        nat.call("wpc_refine_update", self.system.n, self.req.nrhs, nat.ptr(s.X), nat.ptr(s.C),
+                 nat.code(s.C.dtype), nat.ptr(s.res.norms), self.mask, nat.ptr(s.c64), nat.stream())
+        # witness certificate of the updated iterate: r - A c, in place over r
+        s.res.run(s.c64, s.res.r, norms=s.norms_w)
+        s.norms_w_host.copy_(s.norms_w, non_blocking=True)
         self._residual()
         self.phase = "wait_residual"
 
@@ -400,17 +404,21 @@ class Job:
             st = s.state_host.view(12, -1).numpy()[:, :k]
             rr, target = st[_RR], st[_TARGET]
             self.far = bool(np.all(rr > 1e4 * target))
-            total = self.iterations + st[_ITER] if self.system.policy == "mixed" else st[_ITER]
-            if np.all(rr <= target) or np.max(total) >= self.cfg.maxiter:
+            if np.all(rr <= target) or np.max(self.iterations + st[_ITER]) >= self.cfg.maxiter:
                 self.want_check = True
             return
         if what == "residual":
             norms = s.norms_host.numpy()[: 2 * k].copy()
             if self.bnorm is None:
                 self.bnorm = np.sqrt(norms[:k])
+                self.prev_norms = norms
                 self._begin_solve(first=True, norms=norms)
                 return
-            self.bounds = field_bounds(norms, k, self.system.alpha)
+            witness = s.norms_w_host.numpy()[: 2 * k].copy()
+            self.bounds = np.minimum(
+                field_bounds(norms, k, self.system.alpha),
+                witness_bounds(norms, witness, self.prev_norms, k, self.system.alpha))
+            self.prev_norms = norms
             self.rel = np.sqrt(norms[:k]) / np.where(self.bnorm > 0, self.bnorm, 1.0)
             self._begin_solve(first=False, norms=norms)
             return
@@ -451,24 +459,11 @@ class Job:
                 return
         self.outer += 1
         self.far = True
+        self.mask = int(sum(1 << v for v in range(k) if fail[v]))
         with torch.cuda.stream(self.system.stream):
-            if self.system.policy == "mixed":
-                self.mask = int(sum(1 << v for v in range(k) if fail[v]))
-                nat.call("wpc_refine_prepare", self.system.n, k, nat.ptr(s.res.r),
-                         nat.ptr(s.res.norms), self.mask, nat.ptr(s.Rin), nat.code(torch.float32),
-                         nat.stream())
-                self._start_solve(s.Rin, s.C, self.cfg.inner_tol)
-            elif first:
-                # recursive target ~ tol / sqrt(n) relative to ||b||_2 (the bound
-                # scales like sqrt(n) times the relative residual on these meshes);
-                # certification then tightens individual right-hand sides
-                self._start_solve(s.B, s.X, tol / math.sqrt(self.system.n))
-            else:
-                for v in range(k):
-                    if fail[v]:
-                        factor = min(0.25, (0.5 * tol / self.bounds[v]) ** 2)
-                        nat.call("wpc_krylov_scale_target", k, nat.ptr(s.work.state), v,
-                                 float(factor), nat.stream())
+            nat.call("wpc_refine_prepare", self.system.n, k, nat.ptr(s.res.r), nat.ptr(s.res.norms),
+                     self.mask, nat.ptr(s.Rin), nat.code(s.Rin.dtype), nat.stream())
+            self._start_solve(s.Rin, s.C, self.cfg.inner_tol[self.system.policy])
         self.phase = "solve"
 
     def release(self):
@@ -503,7 +498,7 @@ class ServingEngine:
     def estimate(req: Request, policy: str):
         n = req.mesh.side ** 3
         system = (1500 if req.kind == "biharmonic" else 700) * n
-        slot = (80 if policy == "f64" else 56) * n * req.nrhs
+        slot = (96 if policy == "f64" else 56) * n * req.nrhs
         return system, slot
 
     def _make_room(self, need, keep=None):
