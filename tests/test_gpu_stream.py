@@ -40,7 +40,7 @@ def test_stream_matches_oracle_bitwise(oracle, dtype):
     np.testing.assert_allclose(got[17], ref[17], rtol=1e-5 if dtype == torch.float32 else 1e-12)
 
 
-@pytest.mark.parametrize("nvec", [1, 2, 3, 4, 8])
+@pytest.mark.parametrize("nvec", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_stream_batched_dot_and_determinism(nvec):
     rng = np.random.default_rng(nvec)
     a = random_spd(rng, 5000, 0.002)
@@ -99,3 +99,18 @@ def test_batched_equals_single_rhs_solves():
         out = linear.solve_batched("cg", A, b[:, None], x[:, None], M=P, tol=1e-9, check_every=1)
         assert out["iterations"][0] == outb["iterations"][v]
         np.testing.assert_allclose(host(x), host(X[:, v]), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("nvec", [5, 6, 7])
+def test_batched_cg_odd_vector_counts(nvec):
+    from warp_preconditioners import FSAI, linear
+
+    rng = np.random.default_rng(10 + nvec)
+    a = random_spd(rng, 3000, 0.003)
+    A = upload(a)
+    P = FSAI(A, max_row_size=8)
+    B = torch.tensor(rng.normal(size=(3000, nvec)), device=dev())
+    X = torch.zeros_like(B)
+    out = linear.solve_batched("cg", A, B, X, M=P, tol=1e-10, check_every=5)
+    assert out["work"].use_fused
+    np.testing.assert_allclose(host(X), np.linalg.solve(a, host(B)), rtol=1e-7, atol=1e-9)

@@ -294,6 +294,9 @@ using namespace wpc;
         case 2: { constexpr int NV = 2; CALL; break; }      \
         case 3: { constexpr int NV = 3; CALL; break; }      \
         case 4: { constexpr int NV = 4; CALL; break; }      \
+        case 5: { constexpr int NV = 5; CALL; break; }      \
+        case 6: { constexpr int NV = 6; CALL; break; }      \
+        case 7: { constexpr int NV = 7; CALL; break; }      \
         case 8: { constexpr int NV = 8; CALL; break; }      \
         default: return WPC_EARG;                           \
     }

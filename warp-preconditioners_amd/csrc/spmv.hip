@@ -103,6 +103,9 @@ static int spmv_nv(int nvec, int n, const int* off, const int* col, const void* 
         case 2: return spmv_launch<TV, TX, 2>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
         case 3: return spmv_launch<TV, TX, 3>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
         case 4: return spmv_launch<TV, TX, 4>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
+        case 5: return spmv_launch<TV, TX, 5>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
+        case 6: return spmv_launch<TV, TX, 6>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
+        case 7: return spmv_launch<TV, TX, 7>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
         case 8: return spmv_launch<TV, TX, 8>(n, off, col, val, scale, x, y, z, alpha, beta, lanes, s);
     }
     return WPC_EARG;
@@ -304,6 +307,9 @@ static int dot_launch(long rows, int nvec, const void* x, const void* y, double*
         case 2: DOT(2);
         case 3: DOT(3);
         case 4: DOT(4);
+        case 5: DOT(5);
+        case 6: DOT(6);
+        case 7: DOT(7);
         case 8: DOT(8);
         default: return WPC_EARG;
     }

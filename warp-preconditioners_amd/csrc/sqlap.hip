@@ -184,6 +184,9 @@ int wpc_sqlap_apply(int n_full, const int* offsets, const int* columns, const vo
         case 2: SQ(T, 2);            \
         case 3: SQ(T, 3);            \
         case 4: SQ(T, 4);            \
+        case 5: SQ(T, 5);            \
+        case 6: SQ(T, 6);            \
+        case 7: SQ(T, 7);            \
         case 8: SQ(T, 8);            \
         default: return WPC_EARG;    \
     }

@@ -1,0 +1,338 @@
+// Streamed CSR products ("CSR-stream"): the hot sparse kernel of every apply.
+//
+// Rows are pre-partitioned into blocks of at most 256 rows and kStreamNnz stored
+// entries.  A workgroup loads its block's contiguous column/value range into LDS
+// with fully coalesced loads (the HBM stream), then one thread per row sums its
+// entries serially in storage order while gathering x (nvec interleaved vectors,
+// mostly cache hits).  A row longer than kStreamNnz gets a block of its own and
+// is reduced by the whole workgroup.
+//
+// Epilogue:  z_i = alpha * (s_i * sum_i + d_i * u_i) + beta * y_i   (s, d, u optional)
+// covers the FSAI passes (s = per-row scale^2 of a narrow factor), the factored
+// operator's two passes (s = 1/mass; d = w*mass, u = x) and plain products.  An
+// optional fused inner product writes one partial per (block, vector); a
+// single-workgroup launch folds them in block order (deterministic).
+
+
+// Layout of the code: this header holds the kernels and their launch templates;
+// each stream_nv<N>.hip instantiates them for N interleaved vectors (so the eight
+// instantiation sets compile in parallel), and stream.hip holds the C entry points.
+
+#pragma once
+
+#include "reduce.h"
+
+namespace wpc {
+
+constexpr int kStreamNnz = 2048;
+
+struct StreamArgs {
+    int nb;
+    const int* blocks;
+    int n;
+    int lg;  // log2(lanes per row), or -1 for the streamed mapping
+    const int* off;
+    const int* col;
+    const void* val;
+    int val_dtype;
+    const void* s;
+    bool s32;  // s stored as float32 (else the vector type)
+    const void* d;
+    const void* u;
+    const void* x;
+    const void* y;
+    void* z;
+    int vec_dtype;
+    double alpha;
+    double beta;
+    const void* w;
+    double* partials;
+    double* out;
+    hipStream_t st;
+};
+
+int stream_entry_1(const StreamArgs& a);
+int stream_entry_2(const StreamArgs& a);
+int stream_entry_3(const StreamArgs& a);
+int stream_entry_4(const StreamArgs& a);
+int stream_entry_5(const StreamArgs& a);
+int stream_entry_6(const StreamArgs& a);
+int stream_entry_7(const StreamArgs& a);
+int stream_entry_8(const StreamArgs& a);
+
+}  // namespace wpc
+
+#ifdef WPC_STREAM_NV
+
+namespace wpc {
+namespace {
+
+
+template <class TV> struct LdsOf { using type = TV; };
+template <> struct LdsOf<f16s> { using type = uint16_t; };
+template <> struct LdsOf<bf16s> { using type = uint16_t; };
+
+template <class TA, class TV> __device__ __forceinline__ TA lds_val(typename LdsOf<TV>::type v);
+template <> __device__ __forceinline__ double lds_val<double, double>(double v) { return v; }
+template <> __device__ __forceinline__ double lds_val<double, float>(float v) { return v; }
+template <> __device__ __forceinline__ float lds_val<float, float>(float v) { return v; }
+template <> __device__ __forceinline__ double lds_val<double, f16s>(uint16_t v) { return f16_to_float(v); }
+template <> __device__ __forceinline__ float lds_val<float, f16s>(uint16_t v) { return f16_to_float(v); }
+template <> __device__ __forceinline__ double lds_val<double, bf16s>(uint16_t v) { return bf16_to_float(v); }
+template <> __device__ __forceinline__ float lds_val<float, bf16s>(uint16_t v) { return bf16_to_float(v); }
+
+template <class TV> __device__ __forceinline__ typename LdsOf<TV>::type raw_val(const TV* p, long i);
+template <> __device__ __forceinline__ double raw_val<double>(const double* p, long i) { return p[i]; }
+template <> __device__ __forceinline__ float raw_val<float>(const float* p, long i) { return p[i]; }
+template <> __device__ __forceinline__ uint16_t raw_val<f16s>(const f16s* p, long i) { return p[i].b; }
+template <> __device__ __forceinline__ uint16_t raw_val<bf16s>(const bf16s* p, long i) { return p[i].b; }
+
+template <class TX> struct AccOf { using type = TX; };
+template <> struct AccOf<f16s> { using type = float; };
+
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+__global__ __launch_bounds__(kStreamRows) void stream_kernel(
+    const int* __restrict__ blocks, const int* __restrict__ off, const int* __restrict__ col,
+    const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
+    const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
+    typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
+    double* __restrict__ partials, unsigned* counter, double* out) {
+    using TA = typename AccOf<TX>::type;
+    using TL = typename LdsOf<TV>::type;
+    __shared__ int s_col[kStreamNnz];
+    __shared__ TL s_val[kStreamNnz];
+    __shared__ double red[kStreamRows];
+    const int blk = xcd_block();
+    const int r0 = blocks[blk], r1 = blocks[blk + 1];
+    const int e0 = off[r0];
+    const int nnz = off[r1] - e0;
+    TA acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    const bool use_a = alpha != TA(0);
+    int row = r0 + threadIdx.x;
+    bool own = row < r1;
+    if (nnz <= kStreamNnz) {
+        if (use_a) {
+            for (int k = threadIdx.x; k < nnz; k += kStreamRows) {
+                s_col[k] = col[e0 + k];
+                s_val[k] = raw_val<TV>(val, e0 + k);
+            }
+        }
+        __syncthreads();
+        if (own && use_a) {
+            const int a = off[row] - e0, b = off[row + 1] - e0;
+            for (int k = a; k < b; ++k) {
+                const long c = s_col[k];
+                const TA av = lds_val<TA, TV>(s_val[k]);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] += av * St<TX>::template ld<TA>(x, c * NV + v);
+            }
+        }
+    } else {
+        // one long row (r1 == r0 + 1): strided partial sums, fixed-order block reduction
+        row = r0;
+        own = threadIdx.x == 0;
+        double part[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) part[v] = 0.0;
+        if (use_a) {
+            for (int k = threadIdx.x; k < nnz; k += kStreamRows) {
+                const long c = col[e0 + k];
+                const TA av = lds_val<TA, TV>(raw_val<TV>(val, e0 + k));
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    part[v] += (double)(av * St<TX>::template ld<TA>(x, c * NV + v));
+            }
+        }
+        block_sum<NV>(part, red);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = (TA)part[v];
+        __syncthreads();
+    }
+    double dot[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) dot[v] = 0.0;
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long k = (long)row * NV + v;
+            TA r = TA(0);
+            if (use_a) {
+                r = acc[v];
+                if (HS) r = (TA)s[row] * r;
+                if (HD) r += St<TX>::template ld<TA>(d, row) * St<TX>::template ld<TA>(u, k);
+                r *= alpha;
+            }
+            if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
+            St<TX>::template st<TA>(z, k, r);
+            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) * (double)St<TX>::template ld<TA>(z, k);
+        }
+    }
+    if (DOT) {
+        block_sum<NV>(dot, red);
+        write_partials<NV>(dot, partials, blk);
+    }
+}
+
+// Multi-lane mapping of the same product: L = 1 << lg lanes own one row, stride
+// over its entries and reduce with xor shuffles (no LDS staging).  Same epilogue
+// and fused inner product as stream_kernel.
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+__global__ __launch_bounds__(kStreamRows) void lanes_kernel(
+    int n, int lg, const int* __restrict__ off, const int* __restrict__ col,
+    const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
+    const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
+    typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
+    double* __restrict__ partials) {
+    using TA = typename AccOf<TX>::type;
+    __shared__ double red[kStreamRows];
+    const int L = 1 << lg;
+    const int blk = xcd_block();
+    const long row = ((long)blk * kStreamRows + threadIdx.x) >> lg;
+    const int sub = threadIdx.x & (L - 1);
+    const bool live = row < n;
+    const bool use_a = alpha != TA(0);
+    TA acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    if (live && use_a) {
+        const int e1 = off[row + 1];
+        for (int e = off[row] + sub; e < e1; e += L) {
+            const long c = col[e];
+            const TA av = lds_val<TA, TV>(raw_val<TV>(val, e));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += av * St<TX>::template ld<TA>(x, c * NV + v);
+        }
+    }
+    for (int m = L >> 1; m > 0; m >>= 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += shfl_xor(acc[v], m, kWave);
+    }
+    double dot[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) dot[v] = 0.0;
+    if (live && sub == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long k = row * NV + v;
+            TA r = TA(0);
+            if (use_a) {
+                r = acc[v];
+                if (HS) r = (TA)s[row] * r;
+                if (HD) r += St<TX>::template ld<TA>(d, row) * St<TX>::template ld<TA>(u, k);
+                r *= alpha;
+            }
+            if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
+            St<TX>::template st<TA>(z, k, r);
+            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) * (double)St<TX>::template ld<TA>(z, k);
+        }
+    }
+    if (DOT) {
+        block_sum<NV>(dot, red);
+        write_partials<NV>(dot, partials, blk);
+    }
+}
+
+// Fixed-order fold of per-block partials by one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
+                                                  const double* __restrict__ partials,
+                                                  double* __restrict__ out) {
+    __shared__ double red[1024 / kWave][8];
+    double acc[8];
+    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+    for (int b = threadIdx.x; b < P; b += 1024)
+        for (int v = 0; v < nvec; ++v) acc[v] += partials[(long)b * nvec + v];
+    for (int v = 0; v < nvec; ++v) {
+        double x = acc[v];
+        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < nvec) {
+        double t = 0.0;
+        for (int w = 0; w < 1024 / kWave; ++w) t += red[w][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+template <class TV, class TX, class TS, int NV>
+int stream_launch(int nb, const int* blocks, int n, int lg, const int* off,
+                         const int* col, const void* val, const void* s, const void* d,
+                         const void* u, const void* x, const void* y, void* z, double alpha,
+                         double beta, const void* w, double* partials, double* out,
+                         hipStream_t st) {
+    using TA = typename AccOf<TX>::type;
+    const bool hs = s != nullptr, hd = d != nullptr, dt = w != nullptr;
+    const int grid = lg < 0 ? nb : (int)blocks_for((long)n << lg, kStreamRows);
+    if (grid <= 0 || (lg < 0 && nb <= 0) || (lg >= 0 && n <= 0)) return WPC_OK;
+#define SL(HS, HD, DT)                                                                          \
+    do {                                                                                        \
+        if (lg < 0)                                                                             \
+            hipLaunchKernelGGL((stream_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(nb),           \
+                               dim3(kStreamRows), 0, st, blocks, off, col, (const TV*)val,      \
+                               (const TS*)s, (const TX*)d, (const TX*)u, (const TX*)x,          \
+                               (const TX*)y, (TX*)z, (TA)alpha, (TA)beta, (const TX*)w,         \
+                               partials, nullptr, out);                                         \
+        else                                                                                    \
+            hipLaunchKernelGGL((lanes_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(grid),          \
+                               dim3(kStreamRows), 0, st, n, lg, off, col, (const TV*)val,       \
+                               (const TS*)s, (const TX*)d, (const TX*)u, (const TX*)x,          \
+                               (const TX*)y, (TX*)z, (TA)alpha, (TA)beta, (const TX*)w,         \
+                               partials);                                                       \
+    } while (0)
+    if (!hs && !hd && !dt) SL(false, false, false);
+    else if (hs && !hd && !dt) SL(true, false, false);
+    else if (!hs && hd && !dt) SL(false, true, false);
+    else if (hs && hd && !dt) SL(true, true, false);
+    else if (!hs && !hd && dt) SL(false, false, true);
+    else if (hs && !hd && dt) SL(true, false, true);
+    else if (!hs && hd && dt) SL(false, true, true);
+    else SL(true, true, true);
+#undef SL
+    if (dt && out) {
+        const int nvec = NV;
+        hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(1024), 0, st, grid, nvec, partials, out);
+    }
+    return launch_status();
+}
+
+
+template <int NV>
+int stream_for_nv(const StreamArgs& a) {
+#define ST(TV, TX)                                                                              \
+    do {                                                                                        \
+        if (a.s32)                                                                              \
+            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.n, a.lg, a.off, a.col,    \
+                                                     a.val, a.s, a.d, a.u, a.x, a.y, a.z,        \
+                                                     a.alpha, a.beta, a.w, a.partials, a.out,    \
+                                                     a.st);                                      \
+        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.n, a.lg, a.off, a.col, a.val,     \
+                                             a.s, a.d, a.u, a.x, a.y, a.z, a.alpha, a.beta, a.w, \
+                                             a.partials, a.out, a.st);                           \
+    } while (0)
+    if (a.vec_dtype == WPC_F64) {
+        if (a.val_dtype == WPC_F64) ST(double, double);
+        if (a.val_dtype == WPC_F32) ST(float, double);
+        if (a.val_dtype == WPC_F16) ST(f16s, double);
+        if (a.val_dtype == WPC_BF16) ST(bf16s, double);
+    } else if (a.vec_dtype == WPC_F32) {
+        if (a.val_dtype == WPC_F32) ST(float, float);
+        if (a.val_dtype == WPC_F16) ST(f16s, float);
+        if (a.val_dtype == WPC_BF16) ST(bf16s, float);
+    }
+#undef ST
+    return WPC_EDTYPE;
+}
+
+}  // namespace
+
+#define WPC_STREAM_CAT2(a, b) a##b
+#define WPC_STREAM_CAT(a, b) WPC_STREAM_CAT2(a, b)
+int WPC_STREAM_CAT(stream_entry_, WPC_STREAM_NV)(const StreamArgs& a) {
+    return stream_for_nv<WPC_STREAM_NV>(a);
+}
+
+}  // namespace wpc
+
+#endif  // WPC_STREAM_NV

@@ -63,6 +63,8 @@ from .squared_laplacian import SquaredLaplacianOperator
 
 KINDS = ("laplacian", "biharmonic")
 _ITER, _RR, _TARGET = 4, 1, 2  # Krylov state fields (include/wpc.h WPC_KS_*)
+# tightest inner relative residual worth asking of each inner precision
+_INNER_FLOOR = {"f64": 1e-12, "mixed": 1e-6}
 
 
 @dataclass(frozen=True)
@@ -118,6 +120,8 @@ class ServingConfig:
     maxiter: int = 200000
     max_outer: int = 16
     keep_solutions: bool = True
+    # one right-hand side per job, for the one-solve-at-a-time comparison path
+    split_rhs: bool = False
 
 
 @dataclass
@@ -137,6 +141,7 @@ class Result:
     converged: bool
     factor: str  # "built", "refit" or "cached"
     x: np.ndarray | None = None
+    history: list = field(default_factory=list)  # per outer step: (iterations, bounds)
 
     @property
     def latency(self):
@@ -315,6 +320,7 @@ class Job:
         self.far = True
         self.best = np.full(req.nrhs, np.inf)
         self.stalls = 0
+        self.history = []
         self.want_check = False
         self.finished = False
         self.result = None
@@ -415,9 +421,14 @@ class Job:
                 self._begin_solve(first=True, norms=norms)
                 return
             witness = s.norms_w_host.numpy()[: 2 * k].copy()
-            self.bounds = np.minimum(
-                field_bounds(norms, k, self.system.alpha),
-                witness_bounds(norms, witness, self.prev_norms, k, self.system.alpha))
+            plain = field_bounds(norms, k, self.system.alpha)
+            wit = witness_bounds(norms, witness, self.prev_norms, k, self.system.alpha)
+            fresh = np.minimum(plain, wit)
+            # a right-hand side outside this step's mask kept its iterate, so the
+            # certificate it already had still holds
+            moved = np.array([(self.mask >> v) & 1 for v in range(k)], dtype=bool)
+            self.bounds = np.where(moved, fresh, np.minimum(self.bounds, fresh))
+            self.history.append((self.iterations.tolist(), plain.tolist(), wit.tolist()))
             self.prev_norms = norms
             self.rel = np.sqrt(norms[:k]) / np.where(self.bnorm > 0, self.bnorm, 1.0)
             self._begin_solve(first=False, norms=norms)
@@ -432,7 +443,7 @@ class Job:
                 self.req.rid, self.req.kind, self.system.spec.side, self.system.n, k,
                 self.req.arrival, self.t_admit, eng.clock(), [int(v) for v in self.iterations],
                 self.outer, [float(v) for v in self.bounds], [float(v) for v in self.rel],
-                bool(np.all(self.bounds <= self.cfg.tol)), self.provenance, x)
+                bool(np.all(self.bounds <= self.cfg.tol)), self.provenance, x, self.history)
 
     def _begin_solve(self, first, norms):
         """After a residual: finish, or start/continue the Krylov phase."""
@@ -460,10 +471,16 @@ class Job:
         self.outer += 1
         self.far = True
         self.mask = int(sum(1 << v for v in range(k) if fail[v]))
+        policy = self.system.policy
+        inner = self.cfg.inner_tol[policy]
+        if not first:
+            # aim the correction at what is still missing (one more step, usually)
+            need = 0.1 * tol / float(np.max(self.bounds[fail]))
+            inner = min(inner, max(_INNER_FLOOR[policy], need))
         with torch.cuda.stream(self.system.stream):
             nat.call("wpc_refine_prepare", self.system.n, k, nat.ptr(s.res.r), nat.ptr(s.res.norms),
                      self.mask, nat.ptr(s.Rin), nat.code(s.Rin.dtype), nat.stream())
-            self._start_solve(s.Rin, s.C, self.cfg.inner_tol[self.system.policy])
+            self._start_solve(s.Rin, s.C, inner)
         self.phase = "solve"
 
     def release(self):
@@ -565,6 +582,8 @@ class ServingEngine:
         With ``realtime`` each request becomes visible at its arrival time; otherwise
         all are available at once (admission is still limited by slots and budget)."""
         self.t0 = time.perf_counter()
+        if self.cfg.split_rhs:
+            return self._serve_split(requests, realtime, on_result)
         queue = deque(sorted(requests, key=lambda r: r.arrival))
         active, results = [], {}
         while queue or active:
@@ -589,3 +608,40 @@ class ServingEngine:
                 elif queue and realtime:
                     time.sleep(max(0.0, min(queue[0].arrival - self.clock(), 0.01)))
         return [results[r.rid] for r in requests]
+
+    def _serve_split(self, requests, realtime, on_result):
+        """Every right-hand side of every request as a job of its own (nvec = 1);
+        a request finishes with its last right-hand side."""
+        parts, owner = [], {}
+        for req in requests:
+            for v in range(req.nrhs):
+                sub = Request(len(parts), req.arrival, req.mesh, req.kind, 1,
+                              req.rhs_seed * 8 + v, req.alpha)
+                owner[sub.rid] = req
+                parts.append(sub)
+        self.cfg.split_rhs = False
+        try:
+            sub_results = self.serve(parts, realtime, None)
+        finally:
+            self.cfg.split_rhs = True
+        merged = {}
+        for sub, res in zip(parts, sub_results):
+            req = owner[sub.rid]
+            m = merged.get(req.rid)
+            if m is None:
+                merged[req.rid] = m = Result(req.rid, req.kind, res.side, res.n, 0, req.arrival,
+                                             res.admitted, res.finished, [], 0, [], [], True,
+                                             res.factor, None)
+            m.nrhs += 1
+            m.admitted = min(m.admitted, res.admitted)
+            m.finished = max(m.finished, res.finished)
+            m.iterations += res.iterations
+            m.outer = max(m.outer, res.outer)
+            m.field_bound += res.field_bound
+            m.relative_residual += res.relative_residual
+            m.converged = m.converged and res.converged
+        out = [merged[r.rid] for r in requests]
+        if on_result is not None:
+            for r in out:
+                on_result(r)
+        return out
