@@ -423,8 +423,8 @@ def main():
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mapping", default=None,
-                    help="force the row mapping of the G, GT and K products, e.g. 4,0,8 "
-                         "(0 = streamed, L = L lanes per row); default autotunes")
+                    help="force the row mapping of the G, GT and K products, e.g. --mapping=4,0,-1 "
+                         "(0 = streamed, L = L lanes per row, -1 = sliced); default autotunes")
     ap.add_argument("--workload", default="biharmonic64", choices=["biharmonic64", "serving"],
                     help="biharmonic64: the factored-biharmonic 64^3 solve (default); serving: "
                          "the request stream through the continuous-batching engine")
@@ -502,7 +502,8 @@ def main():
                      "avg_us": times[dominant] * 1e6},
         "kernels_us": {k: v * 1e6 for k, v in times.items()},
         "kernels_GBs": {k: bytes_[k] / times[k] / 1e9 for k, v in times.items()},
-        "row_mapping": {k: ("streamed" if m == 0 else f"{m} lanes/row") for k, m in tuned.items()},
+        "row_mapping": {k: ("streamed" if m == 0 else "sliced" if m == -1 else f"{m} lanes/row")
+                        for k, m in tuned.items()},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "iterations": s.iterations,

@@ -114,3 +114,85 @@ def test_batched_cg_odd_vector_counts(nvec):
     out = linear.solve_batched("cg", A, B, X, M=P, tol=1e-10, check_every=5)
     assert out["work"].use_fused
     np.testing.assert_allclose(host(X), np.linalg.solve(a, host(B)), rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_sliced_matches_oracle_bitwise(oracle, dtype):
+    # The sliced mapping sums every row serially in storage order, long rows
+    # included: bit-identical to the oracle's serial product.
+    from warp_preconditioners.sparse import StreamPlan
+
+    rng = np.random.default_rng(5)
+    a = ss.random(3001, 3001, density=0.004, random_state=rng, format="lil")
+    a[0, :] = 0
+    a[17, :] = rng.normal(size=3001)
+    a[2999, :] = 0
+    a = a.tocsr()
+    npd = np.float32 if dtype == torch.float32 else np.float64
+    A = upload(a, dtype)
+    plan = StreamPlan(A, lanes=StreamPlan.SLICED)
+    x = rng.normal(size=3001).astype(npd)
+    y = rng.normal(size=3001).astype(npd)
+    z = torch.empty(3001, dtype=dtype, device=dev())
+    plan.run(torch.tensor(x, device=dev()), torch.tensor(y, device=dev()), z, 0.7, -0.3)
+    np.testing.assert_array_equal(host(z), oracle.spmv(oracle_csr(oracle, a, npd), x, y, 0.7, -0.3))
+
+
+@pytest.mark.parametrize("nvec", [1, 3, 5, 8])
+@pytest.mark.parametrize("vals", [torch.float64, torch.float32, torch.bfloat16])
+def test_sliced_equals_streamed_with_epilogue(nvec, vals):
+    from warp_preconditioners.sparse import BsrMatrix, StreamPlan
+
+    rng = np.random.default_rng(nvec)
+    n = 2 * 64 * 7 + 13  # ragged last slice
+    a = ss.random(n, n, density=0.01, random_state=rng, format="lil")
+    for r in (3, 64, 65, 700):
+        a[r, :] = 0  # empty rows inside slices
+    a[100, : 150] = rng.normal(size=150)  # one long row pads its slice
+    a = a.tocsr()
+    A64 = upload(a)
+    A = BsrMatrix(n, n, (1, 1), A64.offsets, A64.columns, A64.values.to(vals))
+    xd = torch.float32 if vals != torch.float64 else torch.float64
+    X = torch.tensor(rng.normal(size=(n, nvec)), dtype=xd, device=dev())
+    Y = torch.tensor(rng.normal(size=(n, nvec)), dtype=xd, device=dev())
+    U = torch.tensor(rng.normal(size=(n, nvec)), dtype=xd, device=dev())
+    S = torch.tensor(rng.uniform(0.5, 2, size=n), dtype=torch.float32, device=dev())
+    D = torch.tensor(rng.uniform(0.5, 2, size=n), dtype=xd, device=dev())
+    outs = []
+    for mode in (0, StreamPlan.SLICED):
+        plan = StreamPlan(A, lanes=mode)
+        Z = torch.empty_like(X)
+        dots = torch.zeros(nvec, dtype=torch.float64, device=dev())
+        plan.run(X.reshape(-1), Y.reshape(-1), Z.reshape(-1), 0.9, -0.4, nvec, s=S, d=D,
+                 u=U.reshape(-1), w=X.reshape(-1), out=dots)
+        outs.append((host(Z), host(dots)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-10)
+
+
+def test_sliced_copy_follows_refit():
+    # FSAI.update rewrites G/GT in place: the sliced copies must follow.
+    from warp_preconditioners import FSAI
+    from warp_preconditioners.sparse import StreamPlan
+
+    rng = np.random.default_rng(12)
+    a1 = random_spd(rng, 2000, 0.004)
+    a2 = a1.copy()
+    nz = np.nonzero(a2)
+    a2[nz] *= 1.0 + 0.1 * rng.uniform(size=len(nz[0]))
+    a2 = 0.5 * (a2 + a2.T) + 2.0 * np.eye(2000)
+    A1, A2 = upload(a1), upload(a2)
+    P = FSAI(A1, max_row_size=8, reuse_pattern=True)
+    for plan in P.plans():
+        plan.fixed = StreamPlan.SLICED
+    x = torch.tensor(rng.normal(size=2000), device=dev())
+    z = torch.empty_like(x)
+    P.apply_batch(x, x, z, 1.0, 0.0, 1)  # builds the sliced copies
+    P.update(A2)
+    P.apply_batch(x, x, z, 1.0, 0.0, 1)
+    ref = FSAI(A1, max_row_size=8, reuse_pattern=True).update(A2)
+    for plan in ref.plans():
+        plan.fixed = 0  # streamed: the same summation order as the sliced mapping
+    zr = torch.empty_like(x)
+    ref.apply_batch(x, x, zr, 1.0, 0.0, 1)
+    np.testing.assert_array_equal(host(z), host(zr))

@@ -28,9 +28,10 @@ constexpr int kStreamNnz = 2048;
 
 struct StreamArgs {
     int nb;
-    const int* blocks;
+    const int* blocks;  // streamed mapping: row partition; sliced mapping: slice_ptr
+    const int* slice_len;  // sliced mapping only
     int n;
-    int lg;  // log2(lanes per row), or -1 for the streamed mapping
+    int lg;  // log2(lanes per row), -1 for the streamed mapping, -2 for the sliced one
     const int* off;
     const int* col;
     const void* val;
@@ -235,6 +236,83 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 }
 
 // Fixed-order fold of per-block partials by one workgroup of 1024 threads.
+// Sliced-ELL mapping of the same product: rows in slices of 64 (one wavefront);
+// a slice's entries are stored column-major -- entry k of the slice's row r sits at
+// (slice_ptr[s] + k) * 64 + r -- and every row is padded to the slice's longest
+// row with zero values on a column the row already holds.  One thread per row
+// reads fully coalesced index/value streams, issues its gathers four entries at a
+// time, and sums in storage order, so results equal the streamed mapping's bit for
+// bit (padding adds exact zeros).  Same epilogue and fused inner product.
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+__global__ __launch_bounds__(kStreamRows) void sell_kernel(
+    int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
+    const int* __restrict__ col, const TV* __restrict__ val, const TS* __restrict__ s,
+    const TX* __restrict__ d, const TX* __restrict__ u, const TX* __restrict__ x, const TX* y,
+    TX* z, typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta,
+    const TX* __restrict__ w, double* __restrict__ partials) {
+    using TA = typename AccOf<TX>::type;
+    __shared__ double red[kStreamRows];
+    const int blk = xcd_block();
+    const int row = blk * kStreamRows + threadIdx.x;
+    const bool live = row < n;
+    const bool use_a = alpha != TA(0);
+    TA acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    if (live && use_a) {
+        const int slice = row >> 6;
+        const long base = (long)slice_ptr[slice] * 64 + (row & 63);
+        const int len = slice_len[slice];
+        int k = 0;
+        for (; k + 4 <= len; k += 4) {
+            long c[4];
+            TA a[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long p = base + (long)(k + j) * 64;
+                c[j] = col[p];
+                a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * NV + v);
+            }
+        }
+        for (; k < len; ++k) {
+            const long p = base + (long)k * 64;
+            const long c = col[p];
+            const TA a = lds_val<TA, TV>(raw_val<TV>(val, p));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += a * St<TX>::template ld<TA>(x, c * NV + v);
+        }
+    }
+    double dot[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) dot[v] = 0.0;
+    if (live) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long k = (long)row * NV + v;
+            TA r = TA(0);
+            if (use_a) {
+                r = acc[v];
+                if (HS) r = (TA)s[row] * r;
+                if (HD) r += St<TX>::template ld<TA>(d, row) * St<TX>::template ld<TA>(u, k);
+                r *= alpha;
+            }
+            if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
+            St<TX>::template st<TA>(z, k, r);
+            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) * (double)St<TX>::template ld<TA>(z, k);
+        }
+    }
+    if (DOT) {
+        block_sum<NV>(dot, red);
+        write_partials<NV>(dot, partials, blk);
+    }
+}
+
 __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
                                                   const double* __restrict__ partials,
                                                   double* __restrict__ out) {
@@ -257,18 +335,23 @@ __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
 }
 
 template <class TV, class TX, class TS, int NV>
-int stream_launch(int nb, const int* blocks, int n, int lg, const int* off,
-                         const int* col, const void* val, const void* s, const void* d,
-                         const void* u, const void* x, const void* y, void* z, double alpha,
-                         double beta, const void* w, double* partials, double* out,
-                         hipStream_t st) {
+int stream_launch(int nb, const int* blocks, const int* slice_len, int n, int lg, const int* off,
+                  const int* col, const void* val, const void* s, const void* d, const void* u,
+                  const void* x, const void* y, void* z, double alpha, double beta, const void* w,
+                  double* partials, double* out, hipStream_t st) {
     using TA = typename AccOf<TX>::type;
     const bool hs = s != nullptr, hd = d != nullptr, dt = w != nullptr;
-    const int grid = lg < 0 ? nb : (int)blocks_for((long)n << lg, kStreamRows);
-    if (grid <= 0 || (lg < 0 && nb <= 0) || (lg >= 0 && n <= 0)) return WPC_OK;
+    const int grid = lg == -1 ? nb : (int)blocks_for((long)n << (lg < 0 ? 0 : lg), kStreamRows);
+    if (grid <= 0 || (lg == -1 && nb <= 0) || (lg != -1 && n <= 0)) return WPC_OK;
 #define SL(HS, HD, DT)                                                                          \
     do {                                                                                        \
-        if (lg < 0)                                                                             \
+        if (lg == -2)                                                                           \
+            hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(grid),           \
+                               dim3(kStreamRows), 0, st, n, blocks, slice_len, col,             \
+                               (const TV*)val, (const TS*)s, (const TX*)d, (const TX*)u,        \
+                               (const TX*)x, (const TX*)y, (TX*)z, (TA)alpha, (TA)beta,         \
+                               (const TX*)w, partials);                                         \
+        else if (lg == -1)                                                                      \
             hipLaunchKernelGGL((stream_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(nb),           \
                                dim3(kStreamRows), 0, st, blocks, off, col, (const TV*)val,      \
                                (const TS*)s, (const TX*)d, (const TX*)u, (const TX*)x,          \
@@ -303,11 +386,13 @@ int stream_for_nv(const StreamArgs& a) {
 #define ST(TV, TX)                                                                              \
     do {                                                                                        \
         if (a.s32)                                                                              \
-            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.n, a.lg, a.off, a.col,    \
+            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.n, a.lg,     \
+                                                     a.off, a.col,                               \
                                                      a.val, a.s, a.d, a.u, a.x, a.y, a.z,        \
                                                      a.alpha, a.beta, a.w, a.partials, a.out,    \
                                                      a.st);                                      \
-        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.n, a.lg, a.off, a.col, a.val,     \
+        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.n, a.lg, a.off,      \
+                                             a.col, a.val,                                       \
                                              a.s, a.d, a.u, a.x, a.y, a.z, a.alpha, a.beta, a.w, \
                                              a.partials, a.out, a.st);                           \
     } while (0)

@@ -88,6 +88,9 @@ class RefitPlan:
             owner.row_scale2.copy_(self.candidate_scale2)
         nat.call("wpc_scatter", self.transpose.numel(), G.values.element_size(),
                  nat.ptr(G.values), nat.ptr(self.transpose), nat.ptr(GT.values), s)
+        if owner._plans is not None:
+            for plan in owner._plans:
+                plan.refresh_values()
 
 
 def quality(owner, A, probes=4, seed=17):

@@ -86,16 +86,20 @@ class BsrMatrix:
 class StreamPlan:
     """Launch plan for ``wpc_csr_stream`` on one scalar CSR matrix.
 
-    Two row mappings exist: streamed (mode 0: row blocks of <= 256 rows / 2048
-    entries staged through LDS, computed once on the host from the offsets) and
-    multi-lane (mode L: L lanes per row).  ``lanes=None`` autotunes: on the first
+    Three row mappings exist: streamed (mode 0: row blocks of <= 256 rows / 2048
+    entries staged through LDS, computed once on the host from the offsets),
+    multi-lane (mode L: L lanes per row) and sliced (mode -1: a sliced-ELL copy of
+    the matrix, one thread per row; bit-identical to mode 0).  The sliced copy is
+    built on first use and refreshed by ``refresh_values`` after the CSR values
+    change in place.  ``lanes=None`` autotunes: on the first
     product for a given (vector count, vector dtype) outside graph capture, every
     mode is timed with HIP events on the current stream and the fastest is kept;
     under capture a row-length heuristic decides.  The fused inner-product
     partials belong to the plan, so one plan must not run on two streams at once.
     """
 
-    CANDIDATES = (0, 2, 4, 8, 16)
+    CANDIDATES = (0, 2, 4, 8, 16, -1)
+    SLICED = -1
     # Choices are shared by every plan over the same index buffers in this process,
     # so re-planning one matrix never changes its summation order.
     _chosen = {}
@@ -115,6 +119,8 @@ class StreamPlan:
         self.modes = {}
         self.timings = {}
         self._partials = None
+        self._sell = None  # (slice_ptr, slice_len, columns, map)
+        self._sell_values = None
         self.A = A
         self.mean_row = float(off[-1]) / max(A.nrow, 1)
 
@@ -128,15 +134,57 @@ class StreamPlan:
         plan.nblocks, plan.blocks, plan.fixed = other.nblocks, other.blocks, other.fixed
         plan.modes, plan.timings = dict(other.modes), {}
         plan._partials = None
+        plan._sell = other._sell
+        plan._sell_values = None
         plan.A = A
         plan.mean_row = other.mean_row
         return plan
 
+    def _sliced(self):
+        """(slice_ptr, slice_len, columns, values) of the sliced copy, built on
+        first use (one host read of the padded size)."""
+        A = self.A
+        if self._sell is None:
+            ns = nat.lib().wpc_sell_slices(A.nrow)
+            dev = A.device
+            slice_len = torch.empty(max(ns, 1), dtype=torch.int32, device=dev)
+            slice_ptr = torch.zeros(ns + 1, dtype=torch.int32, device=dev)
+            s = nat.stream()
+            nat.call("wpc_sell_lengths", A.nrow, nat.ptr(A.offsets), nat.ptr(slice_len), s)
+            if ns:
+                nat.call("wpc_scan_inclusive", ns, nat.ptr(slice_len), nat.ptr(slice_ptr[1:]), s)
+            total = int(slice_ptr[-1].item()) * 64
+            cols = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+            smap = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+            nat.call("wpc_sell_fill", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns),
+                     nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols), nat.ptr(smap), s)
+            self._sell = (slice_ptr, slice_len, cols, smap, total)
+        if self._sell_values is None:
+            self._sell_values = torch.empty(max(self._sell[4], 1), dtype=A.values.dtype,
+                                            device=A.device)
+            self.refresh_values()
+        return self._sell[0], self._sell[1], self._sell[2], self._sell_values
+
+    def refresh_values(self):
+        """Re-gather the sliced copy's values after A.values changed in place."""
+        if self._sell is not None and self._sell_values is not None:
+            A = self.A
+            nat.call("wpc_sell_values", self._sell[4], nat.ptr(self._sell[3]), nat.ptr(A.values),
+                     A.values.element_size(), nat.ptr(self._sell_values), nat.stream())
+
     def heuristic(self):
-        """Row mapping chosen without timing: streamed for short rows, else 8 lanes."""
-        return 0 if self.mean_row <= 10 else 8
+        """Row mapping chosen without timing: the sliced mapping, or 8 lanes per row
+        for rows too long to pad (its copy is built on first use, so under capture
+        only an already-built copy qualifies)."""
+        if self.mean_row > 64:
+            return 8
+        if torch.cuda.is_current_stream_capturing() and self._sell_values is None:
+            return 0  # same sums as the sliced mapping, no copy to build
+        return self.SLICED
 
     def workgroups(self, mode):
+        if mode == self.SLICED:
+            return (self.A.nrow + 255) // 256
         return self.nblocks if mode == 0 else (self.A.nrow * mode + 255) // 256
 
     def partials(self, nvec, mode=None):
@@ -187,6 +235,16 @@ class StreamPlan:
     def _launch(self, mode, x, y, z, alpha, beta, nvec, s=None, d=None, u=None, w=None, out=None,
                 values=None, part=None):
         A = self.A
+        if mode == self.SLICED and values is None:
+            sp, sl, sc, sv = self._sliced()
+            nat.call("wpc_sell_stream", A.nrow, nat.ptr(sp), nat.ptr(sl), nat.ptr(sc), nat.ptr(sv),
+                     nat.code(sv.dtype), nat.ptr(s), nat.code(s.dtype) if s is not None else 0,
+                     nat.ptr(d), nat.ptr(u), nat.ptr(x), nat.ptr(y), nat.ptr(z), nat.code(x.dtype),
+                     nvec, float(alpha), float(beta), nat.ptr(w), nat.ptr(part), nat.ptr(out),
+                     nat.stream())
+            return
+        if mode == self.SLICED:
+            mode = 0  # explicit values: the streamed mapping sums in the same order
         vals = A.values if values is None else values
         nat.call("wpc_csr_stream", self.nblocks, nat.ptr(self.blocks), A.nrow, mode,
                  nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(vals), nat.code(vals.dtype),
