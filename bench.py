@@ -220,24 +220,44 @@ def kernel_times(s, reps=30):
     npart = nat.lib().wpc_krylov_partials(n)
     parts = torch.zeros(npart * k, dtype=torch.float64, device="cuda:0")
     code = nat.code(dt)
+    from warp_preconditioners.linear import FOLD_IN_PLACE
+
     stream = torch.cuda.current_stream()
+    tuned = {"G": pg.mode(k, dt), "GT": pgt.mode(k, dt), "K": pl.mode(k, dt)}
+    pk = pl.workgroups(tuned["K"])  # p.Ap partials (last operator pass)
+    pz = pgt.workgroups(tuned["GT"])  # r.z partials (last preconditioner pass)
     kernels = {
         "fsai_G_pass": lambda: pg.run(X, None, T, 1.0, 0.0, k, s=P.row_scale2),
         "fsai_GT_pass_dot": lambda: pgt.run(T, None, Y, 1.0, 0.0, k, w=X),
         "sqlap_forward": lambda: pl.run(X, None, T, 1.0, 0.0, k, s=op.inverse_mass),
         "sqlap_backward_dot": lambda: pl.run(T, None, Y, 1.0, 0.0, k, d=op._wm, u=X, w=X),
-        "cg_alpha": lambda: nat.call("wpc_cg_alpha", k, pl.nblocks, nat.ptr(pl.partials(k)),
-                                     nat.ptr(state), stream.cuda_stream),
-        "cg_update": lambda: nat.call("wpc_cg_update", n, k, nat.ptr(X), nat.ptr(Y), nat.ptr(T),
-                                      nat.ptr(T), code, nat.ptr(state), nat.ptr(parts),
-                                      stream.cuda_stream),
-        "cg_beta": lambda: nat.call("wpc_cg_beta", k, pgt.nblocks, nat.ptr(pgt.partials(k)),
-                                    npart, nat.ptr(parts), nat.ptr(state), stream.cuda_stream),
-        "cg_direction": lambda: nat.call("wpc_cg_direction", n, k, nat.ptr(T), nat.ptr(Y), code,
-                                         nat.ptr(state), stream.cuda_stream),
     }
+    # the vector steps exactly as the solver issues them (folded in place when the
+    # partial counts allow, linear.KrylovWork.step)
+    fold_a = pk * k <= FOLD_IN_PLACE
+    fold_b = fold_a and max(pz, npart) * k <= FOLD_IN_PLACE
+    if fold_a:
+        kernels["cg_update_fold"] = lambda: nat.call(
+            "wpc_cg_update_fold", n, k, nat.ptr(X), nat.ptr(Y), nat.ptr(T), nat.ptr(T), code, pk,
+            nat.ptr(pl.partials(k)), nat.ptr(state), nat.ptr(parts), stream.cuda_stream)
+    else:
+        kernels["cg_alpha"] = lambda: nat.call("wpc_cg_alpha", k, pk, nat.ptr(pl.partials(k)),
+                                               nat.ptr(state), stream.cuda_stream)
+        kernels["cg_update"] = lambda: nat.call(
+            "wpc_cg_update", n, k, nat.ptr(X), nat.ptr(Y), nat.ptr(T), nat.ptr(T), code,
+            nat.ptr(state), nat.ptr(parts), stream.cuda_stream)
+    if fold_b:
+        kernels["cg_direction_fold"] = lambda: nat.call(
+            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), code, pz,
+            nat.ptr(pgt.partials(k)), npart, nat.ptr(parts), nat.ptr(state), stream.cuda_stream)
+    else:
+        kernels["cg_beta"] = lambda: nat.call("wpc_cg_beta", k, pz, nat.ptr(pgt.partials(k)),
+                                              npart, nat.ptr(parts), nat.ptr(state),
+                                              stream.cuda_stream)
+        kernels["cg_direction"] = lambda: nat.call(
+            "wpc_cg_direction", n, k, nat.ptr(T), nat.ptr(Y), code, nat.ptr(state),
+            stream.cuda_stream)
     out = {}
-    tuned = {"G": pg.mode(k, dt), "GT": pgt.mode(k, dt), "K": pl.mode(k, dt)}
     for name, fn in kernels.items():
         for _ in range(3):
             fn()
@@ -261,12 +281,14 @@ def kernel_times(s, reps=30):
         "fsai_GT_pass_dot": nnzG * (vb_g + 4) + idx + 3 * n * k * xb,
         "sqlap_forward": nnzK * (vb_k + 4) + idx + 2 * n * k * xb + n * vb_k,
         "sqlap_backward_dot": nnzK * (vb_k + 4) + idx + 3 * n * k * xb + n * vb_k,
-        "cg_alpha": 8 * pl.nblocks * k,
+        "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
-        "cg_beta": 8 * (pgt.nblocks + npart) * k,
+        "cg_update_fold": 6 * n * k * xb + 8 * pk * k,
+        "cg_beta": 8 * (pz + npart) * k,
         "cg_direction": 3 * n * k * xb,
+        "cg_direction_fold": 3 * n * k * xb + 8 * (pz + npart) * k,
     }
-    return out, bytes_, tuned
+    return out, {name: bytes_[name] for name in out}, tuned
 
 
 COPY_PROBE_BYTES = 256 << 20

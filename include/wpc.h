@@ -415,6 +415,19 @@ int wpc_cg_beta(int nvec, int nparts_rz, const double* part_rz, int nparts_rr,
 int wpc_cg_direction(int n, int nvec, const void* z, void* p, int dtype, const double* state,
                      void* stream);
 
+/* The same iteration in four launches instead of six: every workgroup of the
+ * update folds the p.Ap partials (and of the direction step the r.z and r.r
+ * partials) in one fixed order, so all workgroups agree on ALPHA/BETA bit for bit
+ * and workgroup 0 records the scalars.  Meant for small partial counts
+ * (nparts * nvec <= 4096: the folds are repeated per workgroup).  The pair is used
+ * together: the update saves RZ in RZ_OLD, which the direction step reads. */
+int wpc_cg_update_fold(int n, int nvec, void* x, void* r, const void* p, const void* ap, int dtype,
+                       int nparts, const double* part_denom, double* state, double* partials,
+                       void* stream);
+int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, int nparts_rz,
+                          const double* part_rz, int nparts_rr, const double* part_rr,
+                          double* state, void* stream);
+
 /* ------------------------------------------------------------------ certified residuals
  * No single reference routine: these supply the float64 true residual b - A x
  * that the reference's solves check through Warp's cg/cr (`bsr_mv` at

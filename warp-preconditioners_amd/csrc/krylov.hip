@@ -251,6 +251,93 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
     }
 }
 
+// The same two steps with the scalar folds folded in, for partial counts small
+// enough that every workgroup can afford to fold them: each workgroup folds in the
+// same fixed order, so all agree bit for bit, and workgroup 0 records the scalars.
+// The fields each kernel reads and the fields workgroup 0 writes are disjoint:
+//   update:    reads RZ, RR, TARGET      writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA
+//   direction: reads RZ_OLD, ACTIVE      writes RZ_NEW, RR_NEW, BETA, RZ, RR, ITER
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
+    int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+    const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
+    double* __restrict__ partials) {
+    __shared__ double red[kKrylovThreads];
+    __shared__ double sums[8];
+    fold_all<kKrylovThreads>(part_denom, P, NV, sums);
+    T alpha[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double denom = sums[v];
+        const bool active = state[WPC_KS_RR * NV + v] > state[WPC_KS_TARGET * NV + v];
+        const double a = active ? state[WPC_KS_RZ * NV + v] / denom : 0.0;
+        alpha[v] = (T)a;
+        if (blockIdx.x == 0 && threadIdx.x == v) {
+            state[WPC_KS_RZ_OLD * NV + v] = state[WPC_KS_RZ * NV + v];
+            state[WPC_KS_DENOM * NV + v] = denom;
+            state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
+            state[WPC_KS_ALPHA * NV + v] = a;
+        }
+    }
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long k = i * NV + v;
+            T rk = r[k];
+            if (alpha[v] != T(0)) {
+                x[k] += alpha[v] * p[k];
+                rk -= alpha[v] * ap[k];
+                r[k] = rk;
+            }
+            acc[v] += (double)rk * (double)rk;
+        }
+    }
+    block_sum<NV>(acc, red);
+    write_partials<NV>(acc, partials);
+}
+
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
+    int n, const T* __restrict__ z, T* __restrict__ p, int P_rz, const double* __restrict__ part_rz,
+    int P_rr, const double* __restrict__ part_rr, double* state) {
+    __shared__ double s_rz[8], s_rr[8];
+    fold_all<kKrylovThreads>(part_rz, P_rz, NV, s_rz);
+    fold_all<kKrylovThreads>(part_rr, P_rr, NV, s_rr);
+    T beta[NV];
+    bool act[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        act[v] = state[WPC_KS_ACTIVE * NV + v] != 0.0;
+        const double b = act[v] ? s_rz[v] / state[WPC_KS_RZ_OLD * NV + v] : 0.0;
+        beta[v] = (T)b;
+        if (blockIdx.x == 0 && threadIdx.x == v) {
+            state[WPC_KS_RZ_NEW * NV + v] = s_rz[v];
+            state[WPC_KS_RR_NEW * NV + v] = s_rr[v];
+            state[WPC_KS_BETA * NV + v] = b;
+            if (act[v]) {
+                state[WPC_KS_RZ * NV + v] = s_rz[v];
+                state[WPC_KS_RR * NV + v] = s_rr[v];
+                state[WPC_KS_ITER * NV + v] += 1.0;
+            }
+        }
+    }
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (!act[v]) continue;
+            const long k = i * NV + v;
+            p[k] = z[k] + beta[v] * p[k];
+        }
+    }
+}
+
 // p = z + BETA p for slots that stepped.
 template <class T, int NV>
 __global__ __launch_bounds__(kKrylovThreads) void cg_direction_kernel(
@@ -374,6 +461,24 @@ int wpc_cg_direction(int n, int nvec, const void* z, void* p, int dtype, const d
     const int P = krylov_blocks(n);
     hipStream_t s = as_stream(stream);
     VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, state)))
+    return launch_status();
+}
+
+int wpc_cg_update_fold(int n, int nvec, void* x, void* r, const void* p, const void* ap, int dtype,
+                       int nparts, const double* part_denom, double* state, double* partials,
+                       void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_update_fold_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, nparts, part_denom, state, partials)))
+    return launch_status();
+}
+
+int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, int nparts_rz,
+                          const double* part_rz, int nparts_rr, const double* part_rr,
+                          double* state, void* stream) {
+    const int P = krylov_blocks(n);
+    hipStream_t s = as_stream(stream);
+    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_fold_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, nparts_rz, part_rz, nparts_rr, part_rr, state)))
     return launch_status();
 }
 

@@ -108,6 +108,11 @@ def preconditioner(A: BsrMatrix, ptype: str = "diag") -> LinearOperator:
     return op
 
 
+# Largest nparts * nvec whose scalar fold is repeated by every workgroup of the
+# following vector kernel instead of running as a kernel of its own.
+FOLD_IN_PLACE = 4096
+
+
 class KrylovWork:
     """Device buffers and launch sequence of one CG/CR solve on nvec interleaved RHS."""
 
@@ -180,14 +185,25 @@ class KrylovWork:
             # | beta | direction
             part, nparts = self.A.apply_batch(self.p, self.ap, self.ap, 1.0, 0.0, self.nvec,
                                               dot_w=self.p)
-            nat.call("wpc_cg_alpha", self.nvec, nparts, nat.ptr(part), st, s)
-            nat.call("wpc_cg_update", self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
-                     nat.ptr(self.p), nat.ptr(self.ap), self.code, st, pr, s)
+            if nparts * self.nvec <= FOLD_IN_PLACE:
+                nat.call("wpc_cg_update_fold", self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
+                         nat.ptr(self.p), nat.ptr(self.ap), self.code, nparts, nat.ptr(part), st,
+                         pr, s)
+            else:
+                nat.call("wpc_cg_alpha", self.nvec, nparts, nat.ptr(part), st, s)
+                nat.call("wpc_cg_update", self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
+                         nat.ptr(self.p), nat.ptr(self.ap), self.code, st, pr, s)
+            fold_a = nparts * self.nvec <= FOLD_IN_PLACE
             part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                               dot_w=self.r)
-            nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), self.P, pr, st, s)
-            nat.call("wpc_cg_direction", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
-                     self.code, st, s)
+            if fold_a and max(nparts, self.P) * self.nvec <= FOLD_IN_PLACE:
+                nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
+                         nat.ptr(self.p), self.code, nparts, nat.ptr(part), self.P, pr, st, s)
+            else:
+                # (after a fused update too: it leaves RZ as the scalar fold expects)
+                nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), self.P, pr, st, s)
+                nat.call("wpc_cg_direction", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
+                         self.code, st, s)
             return
         if self.method == 0:
             self._op(self.A, self.p, self.ap, self.ap, 1.0, 0.0)
