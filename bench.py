@@ -77,6 +77,7 @@ class Solver:
                                                 mass_weight=alpha)
         Q64 = from_scipy(wl["Q"], dev, torch.float64)
         factor = {"plain-f64": torch.float32, "batched-f64": torch.float32,
+                  "batched-f64-bf16": torch.bfloat16, "batched-f64-f16": torch.float16,
                   "plain-f32": torch.float32, "batched-f32": torch.float32,
                   "batched-mixed": torch.float32, "batched-mixed-bf16": torch.bfloat16}[mode]
         torch.cuda.synchronize()
@@ -122,7 +123,7 @@ class Solver:
                 its += out["iterations"]
             self.iterations = its
             return
-        if self.mode in ("batched-f64", "batched-f32"):
+        if self.mode in ("batched-f64", "batched-f64-bf16", "batched-f64-f16", "batched-f32"):
             f32 = self.mode == "batched-f32"
             if f32:
                 if self.work is None:
@@ -217,6 +218,7 @@ def kernel_times(s, reps=30):
     pg, pgt = P.plans()
     pl = op._stream()
     state = torch.ones(12 * k, dtype=torch.float64, device="cuda:0")
+    state[k:2 * k] = 2.0  # RR > TARGET: every right-hand side steps (as in a live solve)
     npart = nat.lib().wpc_krylov_partials(n)
     parts = torch.zeros(npart * k, dtype=torch.float64, device="cuda:0")
     code = nat.code(dt)
@@ -348,6 +350,118 @@ def cpu_baseline(wl, s, iters_needed, sample_iters=40):
                        f"scaled by {iters_needed} iterations per solve; same FSAI factor")
 
 
+def block_bench(name):
+    """Block-system comparisons, one RHS, relative residual 1e-8 (true residual
+    recomputed in compensated float64), error against a float64 reference solve:
+
+    kron48     -- K (x) C: 48^3 7-point Dirichlet Laplacian, C = R diag(1,1,1e6) R^T
+                  (seeded Haar rotation), 3x3 BSR, n = 331,776; block Jacobi and FSAI
+                  width 16, CG and CR, float64 and mixed (float32 Krylov, narrow
+                  row-scaled factor, float64 refinement); b ~ N(0,1), seed 0.
+    elasticity -- P1 linear elasticity on a 40x40x160-vertex jittered Kuhn beam (E=1,
+                  nu=0.45, clamped at x=0, body force (0,0,-1)), 3x3 BSR, n = 763,200;
+                  block ILU(0) with 3 factor sweeps against FSAI width 8, CG, float32
+                  and mixed precision.
+    Prints one JSON line with a record per method."""
+    import torch
+
+    import warp_preconditioners as wp
+    from warp_preconditioners import linear, workloads
+    from warp_preconditioners.sparse import BsrMatrix
+
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    if name == "kron48":
+        off, col, val, _, _ = workloads.kron_coupling(48, 1.0e6, seed=0)
+        nb = len(off) - 1
+        rhs = np.random.default_rng(0).standard_normal(nb * 3)
+    else:
+        v, t, _ = workloads.kuhn_grid(None, 0.15, seed=0, shape=(160, 40, 40))
+        K, f, _, _ = workloads.p1_elasticity(v, t, v[:, 0] == 0.0, young=1.0, poisson=0.45)
+        off, col, val = K.indptr.astype(np.int32), K.indices.astype(np.int32), np.ascontiguousarray(K.data)
+        nb = len(off) - 1
+        rhs = np.ascontiguousarray(f).reshape(-1)
+    log(f"{name}: {nb} block rows, {len(col)} blocks ({time.perf_counter() - t0:.1f}s host setup)")
+
+    def bsr(dtype):
+        return BsrMatrix(nb, nb, (3, 3), torch.tensor(off, device=dev), torch.tensor(col, device=dev),
+                         torch.tensor(val, dtype=dtype, device=dev))
+
+    A64, A32 = bsr(torch.float64), bsr(torch.float32)
+    op64, op32 = wp.SparseOperator(A64, row_lanes=1), wp.SparseOperator(A32, row_lanes=1)
+    b = torch.tensor(rhs, device=dev)
+    n = b.numel()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t
+
+    Pref, _ = timed(lambda: wp.FSAI(A64, max_row_size=16 if name == "kron48" else 8))
+    xref = torch.zeros(n, dtype=torch.float64, device=dev)
+    ref = linear.solve_refined("cg", op64, b, xref, M=Pref, tol=1e-13, inner_tol=1e-10)
+    log(f"float64 reference: {ref}")
+
+    def error(x):
+        return float(torch.linalg.vector_norm(x - xref) / torch.linalg.vector_norm(xref))
+
+    methods = []
+    if name == "kron48":
+        for prec in ("jacobi", "fsai16"):
+            for method in ("cg", "cr"):
+                methods.append((f"{prec}-{method}-f64", prec, method, "f64", None))
+        for prec, fdt in (("fsai16", torch.bfloat16), ("fsai16", torch.float16), ("jacobi", None)):
+            methods.append((f"{prec}-cg-mixed" + (f"-{str(fdt)[6:]}" if fdt else ""), prec, "cg",
+                            "mixed", fdt))
+    else:
+        for prec in ("ilu0", "fsai8"):
+            methods.append((f"{prec}-cg-f64", prec, "cg", "f64", None))
+            methods.append((f"{prec}-cg-f32", prec, "cg", "f32", None))
+        methods.append(("fsai8-cg-mixed-bfloat16", "fsai8", "cg", "mixed", torch.bfloat16))
+        methods.append(("ilu0-cg-mixed", "ilu0", "cg", "mixed", None))
+    records = []
+    for label, prec, method, mode, fdt in methods:
+        A = A64 if mode == "f64" else A32
+        if prec == "jacobi":
+            # the tolerance the reference passes for K (x) C (block_challenge.py:101)
+            P, setup = timed(lambda: wp.BlockJacobi(A, pivot_tolerance=1e-15))
+        elif prec == "ilu0":
+            P, setup = timed(lambda: wp.BlockILU0(A, factor_sweeps=3))
+        else:
+            width = int(prec[4:])
+            fd = fdt or (torch.float32 if mode != "f32" else torch.float32)
+            src = A64 if mode != "f32" else A32
+            P, setup = timed(lambda: wp.FSAI(src, max_row_size=width, factor_dtype=fd))
+        x = torch.zeros(n, dtype=torch.float64, device=dev)
+        if mode == "f64":
+            out, solve = timed(lambda: linear.solve_refined(method, op64, b, x, M=P, tol=1e-8,
+                                                            inner_tol=1e-8, max_outer=4))
+        elif mode == "mixed":
+            out, solve = timed(lambda: linear.solve_refined(method, op64, b, x, M=P, inner=op32,
+                                                            tol=1e-8, inner_tol=1e-4))
+        else:  # plain float32: recursive residual target 1e-8, reported as reached
+            b32 = b.float()
+            x32 = torch.zeros_like(b32)
+            o, solve = timed(lambda: linear.solve_batched(method, op32, b32[:, None], x32[:, None],
+                                                          M=P, tol=1e-8, maxiter=20000))
+            x = x32.double()
+            from warp_preconditioners.refinement import TrueResidual
+            res = TrueResidual(op64, torch.ones(n, dtype=torch.float64, device=dev), 1)
+            nr = res.run(x, b).cpu().numpy()
+            out = dict(iterations=o["iterations"][0], outer=1,
+                       relative_residual=float(nr[0]) ** 0.5 / float(torch.linalg.vector_norm(b)),
+                       converged=o["residuals"][0] <= o["targets"][0])
+        rec = dict(method=label, setup_s=setup, solve_s=solve, iterations=out["iterations"],
+                   outer=out["outer"], relative_residual=out["relative_residual"],
+                   error_vs_f64=error(x), converged=bool(out["converged"]))
+        log(json.dumps(rec))
+        records.append(rec)
+    print(json.dumps({"workload": name, "n": n, "blocks": int(len(col)), "reference": ref,
+                      "methods": records}), flush=True)
+
+
 def serving_bench(args):
     """Serving stream: Poisson arrivals (200/s, seed 7), meshes of side U{16..max},
     Laplacian+mass or factored biharmonic, 1-8 N(0,1) right-hand sides, meshes
@@ -437,8 +551,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched-f64",
-                    choices=["plain-f64", "plain-f32", "batched-f64", "batched-f32",
-                             "batched-mixed", "batched-mixed-bf16"])
+                    choices=["plain-f64", "plain-f32", "batched-f64", "batched-f64-bf16",
+                             "batched-f64-f16", "batched-f32", "batched-mixed",
+                             "batched-mixed-bf16"])
     ap.add_argument("--rtol", type=float, default=1e-5,
                     help="recursive relative-residual stopping tolerance")
     ap.add_argument("--compare", action="store_true", help="also time the plain one-RHS path")
@@ -447,9 +562,11 @@ def main():
     ap.add_argument("--mapping", default=None,
                     help="force the row mapping of the G, GT and K products, e.g. --mapping=4,0,-1 "
                          "(0 = streamed, L = L lanes per row, -1 = sliced); default autotunes")
-    ap.add_argument("--workload", default="biharmonic64", choices=["biharmonic64", "serving"],
+    ap.add_argument("--workload", default="biharmonic64",
+                    choices=["biharmonic64", "serving", "kron48", "elasticity"],
                     help="biharmonic64: the factored-biharmonic 64^3 solve (default); serving: "
-                         "the request stream through the continuous-batching engine")
+                         "the request stream through the continuous-batching engine; kron48 / "
+                         "elasticity: the block-coupled and elasticity solver comparisons")
     ap.add_argument("--requests", type=int, default=512, help="serving: stream length")
     ap.add_argument("--max-side", type=int, default=96, help="serving: largest mesh side")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -464,6 +581,9 @@ def main():
 
     if args.workload == "serving":
         serving_bench(args)
+        return
+    if args.workload in ("kron48", "elasticity"):
+        block_bench(args.workload)
         return
     wl = build_workload(args.side)
     s = Solver(wl, args.mode)

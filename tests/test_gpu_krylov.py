@@ -66,3 +66,38 @@ def test_exact_count_device_loop():
     assert int(nit.cpu()[0]) == it1
     np.testing.assert_array_equal(host(x), host(x2))
     assert float(rsq.cpu()[0]) ** 0.5 <= atol
+
+
+@pytest.mark.parametrize("prec", ["jacobi", "fsai-bf16"])
+def test_refined_solve_reaches_true_residual(prec):
+    # K (x) C with eigenvalue contrast 1e6 (eps * cond ~ 1e-7): float32 inner CG with
+    # float64 refinement reaches a compensated true relative residual <= 1e-8 and the
+    # direct solution.
+    import scipy.sparse as ss
+    import scipy.sparse.linalg as sla
+
+    import warp_preconditioners as wp
+    from warp_preconditioners import linear, workloads
+    from warp_preconditioners.sparse import BsrMatrix
+
+    off, col, val, _, k = workloads.kron_coupling(12, 1.0e6, seed=3)
+    nb = len(off) - 1
+
+    def bsr(dt):
+        return BsrMatrix(nb, nb, (3, 3), torch.tensor(off, device=dev()),
+                         torch.tensor(col, device=dev()), torch.tensor(val, dtype=dt, device=dev()))
+
+    A64, A32 = bsr(torch.float64), bsr(torch.float32)
+    op64, op32 = wp.SparseOperator(A64, row_lanes=1), wp.SparseOperator(A32, row_lanes=1)
+    # pivot tolerance as the reference passes for this system (block_challenge.py:101)
+    P = wp.BlockJacobi(A32, pivot_tolerance=1e-15) if prec == "jacobi" else wp.FSAI(A64, max_row_size=16,
+                                                              factor_dtype=torch.bfloat16)
+    rng = np.random.default_rng(9)
+    bh = rng.standard_normal(nb * 3)
+    b = torch.tensor(bh, device=dev())
+    x = torch.zeros_like(b)
+    out = linear.solve_refined("cg", op64, b, x, M=P, inner=op32, tol=1e-8, inner_tol=1e-4)
+    assert out["converged"] and out["relative_residual"] <= 1e-8 and out["outer"] >= 2
+    dense = ss.bsr_matrix((val, col, off), shape=(nb * 3, nb * 3)).tocsc()
+    xs = sla.spsolve(dense, bh)
+    assert np.linalg.norm(host(x) - xs) / np.linalg.norm(xs) < 1e-5

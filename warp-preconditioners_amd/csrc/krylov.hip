@@ -257,13 +257,35 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
 // The fields each kernel reads and the fields workgroup 0 writes are disjoint:
 //   update:    reads RZ, RR, TARGET      writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA
 //   direction: reads RZ_OLD, ACTIVE      writes RZ_NEW, RR_NEW, BETA, RZ, RR, ITER
+// Rows each thread loads before the fold, so that the HBM reads overlap the fold's
+// latency (bounded so the prefetched values stay in registers).
+template <int NV> constexpr int kPrefetch = NV <= 2 ? 4 : (NV <= 4 ? 2 : 1);
+
 template <class T, int NV>
 __global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
     int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
     double* __restrict__ partials) {
+    constexpr int PF = kPrefetch<NV>;
     __shared__ double red[kKrylovThreads];
     __shared__ double sums[8];
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    T xv[PF][NV], rv[PF][NV], pv[PF][NV], av[PF][NV];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
+        if (i < end) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const long k = i * NV + v;
+                xv[j][v] = x[k];
+                rv[j][v] = r[k];
+                pv[j][v] = p[k];
+                av[j][v] = ap[k];
+            }
+        }
+    }
     fold_all<kKrylovThreads>(part_denom, P, NV, sums);
     T alpha[NV];
 #pragma unroll
@@ -282,9 +304,24 @@ __global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
     double acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-    long begin, end;
-    chunk_of(n, &begin, &end);
-    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
+        if (i < end) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const long k = i * NV + v;
+                T rk = rv[j][v];
+                if (alpha[v] != T(0)) {
+                    x[k] = xv[j][v] + alpha[v] * pv[j][v];
+                    rk -= alpha[v] * av[j][v];
+                    r[k] = rk;
+                }
+                acc[v] += (double)rk * (double)rk;
+            }
+        }
+    }
+    for (long i = begin + threadIdx.x + (long)PF * kKrylovThreads; i < end; i += kKrylovThreads) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const long k = i * NV + v;
@@ -305,7 +342,22 @@ template <class T, int NV>
 __global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
     int n, const T* __restrict__ z, T* __restrict__ p, int P_rz, const double* __restrict__ part_rz,
     int P_rr, const double* __restrict__ part_rr, double* state) {
+    constexpr int PF = kPrefetch<NV>;
     __shared__ double s_rz[8], s_rr[8];
+    long begin, end;
+    chunk_of(n, &begin, &end);
+    T zv[PF][NV], pv[PF][NV];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
+        if (i < end) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                zv[j][v] = z[i * NV + v];
+                pv[j][v] = p[i * NV + v];
+            }
+        }
+    }
     fold_all<kKrylovThreads>(part_rz, P_rz, NV, s_rz);
     fold_all<kKrylovThreads>(part_rr, P_rr, NV, s_rr);
     T beta[NV];
@@ -326,9 +378,16 @@ __global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
             }
         }
     }
-    long begin, end;
-    chunk_of(n, &begin, &end);
-    for (long i = begin + threadIdx.x; i < end; i += kKrylovThreads) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
+        if (i < end) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                if (act[v]) p[i * NV + v] = zv[j][v] + beta[v] * pv[j][v];
+        }
+    }
+    for (long i = begin + threadIdx.x + (long)PF * kKrylovThreads; i < end; i += kKrylovThreads) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (!act[v]) continue;

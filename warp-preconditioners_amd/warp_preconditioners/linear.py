@@ -427,3 +427,68 @@ def gmres(A, b, x, tol=None, atol=None, restart=31, maxiter=0, M=None, callback=
         if callback is not None:
             callback(iters, beta, target)
     return iters, beta, target
+
+
+def solve_refined(method, A, B, X, M=None, inner=None, tol=1.0e-8, inner_tol=1.0e-4,
+                  max_outer=20, maxiter=100000, check_every=25):
+    """Iterative refinement to a true relative residual ``tol`` on k right-hand sides.
+
+    ``A`` is the float64 system as a ``SparseOperator``; its residual b - A x is
+    recomputed with compensated accumulation (wpc_dd_residual), so the reported
+    residual is not limited by float64 cancellation (relevant when eps * cond(A)
+    approaches ``tol``).  Each outer step solves ``inner`` (default A; e.g. a
+    float32 copy with a matching preconditioner M) for the correction of the
+    still-failing right-hand sides, scaled to unit size, with CG/CR to
+    ``inner_tol`` and adds it in float64.  B and X are float64 [n, k] (or [n])
+    device arrays; X is the initial guess and receives the solution.  Returns
+    per-RHS ``iterations`` (inner steps summed), ``relative_residual`` and
+    ``outer`` (steps taken).
+    """
+    from .refinement import TrueResidual
+
+    squeeze = B.dim() == 1
+    B2 = B.reshape(B.shape[0], -1)
+    X2 = X.reshape(X.shape[0], -1)
+    n, k = B2.shape
+    if B2.dtype != torch.float64 or X2.dtype != torch.float64:
+        raise TypeError("solve_refined takes float64 B and X")
+    if not (X2.is_contiguous() and B2.is_contiguous()):
+        raise ValueError("B and X must be contiguous")
+    inner = A if inner is None else inner
+    vdt = getattr(inner, "scalar_type", torch.float64)
+    dev = B2.device
+    ones = torch.ones(n, dtype=torch.float64, device=dev)
+    res = TrueResidual(A, ones, k)
+    zero = torch.zeros_like(X2)
+    bn = torch.sqrt(res.run(zero.reshape(-1), B2.reshape(-1))[:k]).cpu().numpy()
+    norms = res.run(X2.reshape(-1), B2.reshape(-1)).cpu().numpy()
+    Rin = torch.empty(n * k, dtype=vdt, device=dev)
+    C = torch.empty(n * k, dtype=vdt, device=dev)
+    iters = [0] * k
+    work = None
+    outer = 0
+    scale = lambda v: v if v > 0 else 1.0  # noqa: E731
+    while outer < max_outer:
+        rel = [float(norms[v]) ** 0.5 / scale(bn[v]) for v in range(k)]
+        fail = [rel[v] > tol for v in range(k)]
+        if not any(fail):
+            break
+        mask = sum(1 << v for v in range(k) if fail[v])
+        s = nat.stream()
+        nat.call("wpc_refine_prepare", n, k, nat.ptr(res.r), nat.ptr(res.norms), mask, nat.ptr(Rin),
+                 nat.code(vdt), s)
+        C.zero_()
+        out = solve_batched(method, inner, Rin.view(n, k), C.view(n, k), M=M, tol=inner_tol,
+                            maxiter=maxiter, check_every=check_every, work=work)
+        work = out["work"]
+        iters = [a + b for a, b in zip(iters, out["iterations"])]
+        nat.call("wpc_refine_update", n, k, nat.ptr(X2), nat.ptr(C), nat.code(vdt),
+                 nat.ptr(res.norms), mask, None, nat.stream())
+        norms = res.run(X2.reshape(-1), B2.reshape(-1)).cpu().numpy()
+        outer += 1
+    rel = [float(norms[v]) ** 0.5 / scale(bn[v]) for v in range(k)]
+    out = dict(iterations=iters, relative_residual=rel, outer=outer,
+               converged=all(r <= tol for r in rel))
+    if squeeze:
+        out["iterations"], out["relative_residual"] = out["iterations"][0], out["relative_residual"][0]
+    return out

@@ -210,7 +210,11 @@ class StreamPlan:
             self.modes[key] = m
         return m
 
-    def _tune(self, nvec, dtype, reps=5):
+    # A mapping must beat the sliced one (storage-order sums, bit-identical to the
+    # streamed mapping and the serial oracle) by this margin to be chosen instead.
+    SLICED_MARGIN = 0.03
+
+    def _tune(self, nvec, dtype, reps=10):
         n = self.A.nrow
         x = torch.ones(n * nvec, dtype=dtype, device=self.A.device)
         z = torch.empty_like(x)
@@ -230,6 +234,9 @@ class StreamPlan:
             self.timings[(nvec, dtype, m)] = t
             if t < best_t:
                 best, best_t = m, t
+        sliced = self.timings.get((nvec, dtype, self.SLICED))
+        if sliced is not None and self.mean_row <= 64 and sliced <= best_t * (1 + self.SLICED_MARGIN):
+            return self.SLICED
         return best
 
     def _launch(self, mode, x, y, z, alpha, beta, nvec, s=None, d=None, u=None, w=None, out=None,
