@@ -271,6 +271,15 @@ int wpc_sqlap_apply(int n_full, const int* offsets, const int* columns, const vo
                     const void* x, const void* y, void* z, int nvec, double alpha, double beta,
                     int boundary, int lanes, void* stream);
 
+/* Block-adaptive FSAI on b x b BSR matrices (b = 2..4, b * max_blocks <= 32).
+ * Replaces benchmarks/block_fsai_candidate.py:43-171 (diagonal + build); with
+ * refit != 0 the block pattern is read back from `cols` (:76-82, the rows as
+ * sorted by wpc_fsai_sort_rows).  Writes scale[n], padded cols/vals[n * b *
+ * max_blocks] in pattern order; status[0] counts nonpositive pivots. */
+int wpc_block_fsai_build(int nb, int b, int max_blocks, const int* offsets, const int* columns,
+                         const void* values, int dtype, void* scale, int* cols, void* vals,
+                         int* status, int refit, void* stream);
+
 /* ------------------------------------------------------------ assembly */
 
 /* P1 tetrahedra: local[16*t + 4a + b] = vol_t * grad(phi_a) . grad(phi_b) and

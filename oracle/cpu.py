@@ -206,6 +206,39 @@ def fsai(A: CSR, max_row_size=8, kap_tolerance=1.0e-3, pivot_floor=None, factor_
                 pivot_floor=pivot_floor, width=max_row_size)
 
 
+def block_fsai(A: CSR, max_blocks=2, factor_dtype=None, refit_from=None):
+    """benchmarks/block_fsai_candidate.py:174-215 (and the refit of :217-254 when
+    ``refit_from`` is a previous result): dict(G, GT, cols, vals, scale)."""
+    b = A.block
+    if b < 2:
+        raise ValueError("block FSAI needs blocks of size 2 or more")
+    nb = A.nrow
+    n = nb * b
+    width = b * max_blocks
+    dt = code(A.dtype)
+    scale = np.zeros(n, A.dtype)
+    status = np.zeros(1, np.int32)
+    if refit_from is not None:
+        cols = refit_from["cols"].copy()
+        vals = np.zeros(n * width, A.dtype)
+    else:
+        cols = np.zeros(n * width, np.int32)
+        vals = np.zeros(n * width, A.dtype)
+    val = np.ascontiguousarray(A.val.reshape(-1))
+    lib().orc_block_fsai_build(dt, nb, b, max_blocks, _p(A.off), _p(A.col), _p(val), _p(scale),
+                               _p(cols), _p(vals), _p(status), int(refit_from is not None))
+    if status[0]:
+        raise ValueError("Nonpositive block FSAI pivot")
+    G = pack_factor(n, width, cols, vals)
+    # fsai_pack.py:15-39 sorts the padded rows in place; a later refit reads its
+    # block pattern from those sorted rows (block_fsai_candidate.py:76-82)
+    counts = np.zeros(n + 1, np.int32)
+    lib().orc_fsai_sort_rows(code(vals.dtype), n, width, _p(cols), _p(vals), _p(counts))
+    if factor_dtype is not None and np.dtype(factor_dtype) != np.dtype(A.dtype):
+        G = CSR(G.off, G.col, convert(G.val, factor_dtype), n, n, 1)
+    return dict(G=G, GT=transpose(G), cols=cols, vals=vals, scale=scale, width=width)
+
+
 def spmv(A: CSR, x, y=None, alpha=1.0, beta=0.0, vec_dtype=None):
     """sparse_operator.py:64-87 semantics (scalar CSR, accumulation in vector type)."""
     vec_dtype = np.dtype(vec_dtype or x.dtype)

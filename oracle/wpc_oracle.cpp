@@ -741,6 +741,159 @@ void orc_scatter(int elem, long count, const void* vals, const int* map, void* o
         std::memcpy((char*)out + (long)map[e] * elem, (const char*)vals + e * elem, elem);
 }
 
+}  // extern "C" (templates follow)
+
+// ------------------------------------------------------------ block FSAI (candidate)
+// benchmarks/block_fsai_candidate.py:23-41 -- one scalar entry of a BSR matrix
+// (binary search of block column j/b in block row i/b, then the (i%b, j%b) entry
+// of the row-major block).
+template <class T>
+static T bsr_entry(const int* off, const int* col, const T* val, int b, int i, int j) {
+    int lo = off[i / b], hi = off[i / b + 1];
+    const int end = hi;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (col[mid] < j / b) lo = mid + 1;
+        else hi = mid;
+    }
+    T v = T(0);
+    if (lo < end && col[lo] == j / b) v = val[(long)lo * b * b + (i % b) * b + (j % b)];
+    return v;
+}
+
+// benchmarks/block_fsai_candidate.py:43-57 -- scale = 1/sqrt(A_ii), status[0]++ if not.
+template <class T>
+static void block_fsai_diagonal(int n, int b, const int* off, const int* col, const T* val,
+                                T* scale, int* status) {
+    for (int i = 0; i < n; ++i) {
+        const T d = bsr_entry(off, col, val, b, i, i);
+        if (d > T(0) && std::isfinite(d)) {
+            scale[i] = T(1) / std::sqrt(d);
+        } else {
+            status[0] += 1;
+            scale[i] = T(0);
+        }
+    }
+}
+
+// benchmarks/block_fsai_candidate.py:59-171 -- one block row: greedy block-pattern
+// growth (Cholesky of the equilibrated principal submatrix recomputed per step,
+// b identity right-hand sides, block scores), then G_i = chol(Z_ii)^-1 Z^T D^-1/2
+// written as b padded scalar rows of `width` = b * B entries in pattern order.
+template <class T>
+static void block_fsai_row(int i, int b, int B, const int* off, const int* col, const T* val,
+                           const T* scale, int* cols, T* vals, int* status, bool refit) {
+    const int width = b * B;
+    std::vector<int> pattern(B, -1);
+    pattern[0] = i;
+    int size = 1;
+    if (refit) {
+        for (int p = 0; p < B; ++p) {
+            const int c = cols[(long)i * b * width + p * b];
+            if (c >= 0 && c / b != i) pattern[size++] = c / b;
+        }
+    }
+    std::vector<T> z((size_t)width * b, T(0));
+    std::vector<T> chol((size_t)width * width);
+    for (int step = 0; step < B; ++step) {
+        std::fill(chol.begin(), chol.end(), T(0));
+        const int count = size * b;
+        for (int p = 0; p < count; ++p) {
+            const int r = pattern[p / b] * b + p % b;
+            for (int q = 0; q <= p; ++q) {
+                const int c = pattern[q / b] * b + q % b;
+                T v = bsr_entry(off, col, val, b, r, c) * scale[r] * scale[c];
+                for (int k = 0; k < q; ++k) v -= chol[p * width + k] * chol[q * width + k];
+                if (p == q) {
+                    if (v <= T(0) || !std::isfinite(v)) {
+                        status[0] += 1;
+                        v = T(1);
+                    }
+                    chol[p * width + q] = std::sqrt(v);
+                } else {
+                    chol[p * width + q] = v / chol[q * width + q];
+                }
+            }
+        }
+        for (int c = 0; c < b; ++c) {
+            for (int p = 0; p < count; ++p) {
+                T v = p == c ? T(1) : T(0);
+                for (int q = 0; q < p; ++q) v -= chol[p * width + q] * z[q * b + c];
+                z[p * b + c] = v / chol[p * width + p];
+            }
+            for (int rev = 0; rev < count; ++rev) {
+                const int p = count - rev - 1;
+                T v = z[p * b + c];
+                for (int q = p + 1; q < count; ++q) v -= chol[q * width + p] * z[q * b + c];
+                z[p * b + c] = v / chol[p * width + p];
+            }
+        }
+        if (refit || step == B - 1) break;
+        int best = -1;
+        T best_score = T(0);
+        for (int p = 0; p < size; ++p) {
+            const int row = pattern[p];
+            for (int e = off[row]; e < off[row + 1]; ++e) {
+                const int candidate = col[e];
+                bool selected = candidate >= i;
+                for (int q = 0; q < size; ++q)
+                    if (pattern[q] == candidate) selected = true;
+                if (selected) continue;
+                T score = T(0);
+                for (int r = 0; r < b; ++r) {
+                    const int cr = candidate * b + r;
+                    for (int c = 0; c < b; ++c) {
+                        T residual = T(0);
+                        for (int q = 0; q < count; ++q) {
+                            const int j = pattern[q / b] * b + q % b;
+                            residual += bsr_entry(off, col, val, b, cr, j) * scale[cr] * scale[j] *
+                                        z[q * b + c];
+                        }
+                        score += residual * residual;
+                    }
+                }
+                if (score > best_score || (score == best_score && candidate < best)) {
+                    best = candidate;
+                    best_score = score;
+                }
+            }
+        }
+        if (best < 0 || best_score == T(0)) break;
+        pattern[size++] = best;
+    }
+    std::vector<T> norm((size_t)b * b, T(0));
+    for (int r = 0; r < b; ++r) {
+        for (int c = 0; c <= r; ++c) {
+            T v = z[r * b + c];
+            for (int k = 0; k < c; ++k) v -= norm[r * b + k] * norm[c * b + k];
+            if (r == c) norm[r * b + c] = std::sqrt(v);
+            else norm[r * b + c] = v / norm[c * b + c];
+        }
+    }
+    for (int q = 0; q < size * b; ++q) {
+        for (int r = 0; r < b; ++r) {
+            T v = z[q * b + r];
+            for (int k = 0; k < r; ++k) v -= norm[r * b + k] * z[q * b + k];
+            z[q * b + r] = v / norm[r * b + r];
+        }
+    }
+    for (int r = 0; r < b; ++r) {
+        for (int q = 0; q < width; ++q) {
+            const long e = ((long)i * b + r) * width + q;
+            cols[e] = -1;
+            vals[e] = T(0);
+            if (q < size * b) {
+                const int j = pattern[q / b] * b + q % b;
+                cols[e] = j;
+                vals[e] = z[q * b + r] * scale[j];
+            }
+        }
+    }
+}
+
+extern "C" {
+
+
 int orc_fsai_refit(int dt, int storage, int n, const int* off, const int* col, const void* val,
                    const int* goff, const int* gcol, const void* scale, double floor, void* out,
                    int* bad) {
@@ -854,6 +1007,29 @@ int orc_block_diagonal_only(int dt, int nb, int b, const int* off, const int* co
         } else {
             bad[0] += 1;
         }
+    }
+    return 0;
+}
+
+
+// benchmarks/block_fsai_candidate.py:174-215 -- diagonal + build over all block rows
+// (refit != 0 reuses the pattern stored in cols, :217-254).
+int orc_block_fsai_build(int dt, int nb, int b, int max_blocks, const int* off, const int* col,
+                         const void* val, void* scale, int* cols, void* vals, int* status,
+                         int refit) {
+    const int n = nb * b;
+    if (dt == 4) {
+        block_fsai_diagonal(n, b, off, col, (const double*)val, (double*)scale, status);
+        for (int i = 0; i < nb; ++i)
+            block_fsai_row(i, b, max_blocks, off, col, (const double*)val, (const double*)scale,
+                           cols, (double*)vals, status, refit != 0);
+    } else if (dt == 3) {
+        block_fsai_diagonal(n, b, off, col, (const float*)val, (float*)scale, status);
+        for (int i = 0; i < nb; ++i)
+            block_fsai_row(i, b, max_blocks, off, col, (const float*)val, (const float*)scale, cols,
+                           (float*)vals, status, refit != 0);
+    } else {
+        return -1;
     }
     return 0;
 }

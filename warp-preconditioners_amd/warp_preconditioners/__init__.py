@@ -8,6 +8,7 @@ and ``warp_preconditioners.sparse.BsrMatrix`` in place of ``warp.sparse.BsrMatri
 """
 
 from .biharmonic import BiharmonicSystem, RepeatedFSAI
+from .block_fsai import BlockFSAI, BsrBlockFSAI
 from .block_ilu import BlockILU0, BlockJacobi
 from .fsai import FSAI
 from .sparse_operator import SparseOperator
@@ -21,4 +22,6 @@ __all__ = [
     "SquaredLaplacianOperator",
     "BlockJacobi",
     "BlockILU0",
+    "BlockFSAI",
+    "BsrBlockFSAI",
 ]
