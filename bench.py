@@ -417,7 +417,9 @@ def block_bench(name):
         methods += [("fsai16-cg-f64-bfloat16", "fsai16", "cg", "f64", torch.bfloat16),
                     ("fsai16-cr-f64-bfloat16", "fsai16", "cr", "f64", torch.bfloat16),
                     ("fsai16-cg-f64-float16", "fsai16", "cg", "f64", torch.float16),
-                    ("fsai16-cg-mixed-bfloat16", "fsai16", "cg", "mixed", torch.bfloat16)]
+                    ("fsai16-cg-mixed-bfloat16", "fsai16", "cg", "mixed", torch.bfloat16),
+                    ("blockfsai5-cg-f64", "blockfsai5", "cg", "f64", None),
+                    ("blockfsai5-cr-f64", "blockfsai5", "cr", "f64", None)]
     else:
         # block ILU(0) with Jacobi-sweep triangular solves is a nonsymmetric
         # preconditioner: CG may break down on it (recorded), GMRES is the fit
@@ -437,6 +439,10 @@ def block_bench(name):
             P, setup = timed(lambda: wp.BlockJacobi(A, pivot_tolerance=1e-15))
         elif prec == "ilu0":
             P, setup = timed(lambda: wp.BlockILU0(A, factor_sweeps=3))
+        elif prec.startswith("blockfsai"):
+            # block FSAI: 5 blocks of 3 = 15 scalars per row (block_fsai_candidate.py)
+            P, setup = timed(lambda: wp.BlockFSAI(A, max_blocks=int(prec[9:]),
+                                                  factor_dtype=torch.float32))
         else:
             width = int(prec[4:])
             P, setup = timed(lambda: wp.FSAI(A, max_row_size=width,
