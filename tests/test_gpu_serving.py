@@ -305,7 +305,7 @@ def test_serving_budget_evicts_and_still_serves():
     # every switch of mesh must evict the idle previous system first
     reqs = [Request(i, 0.0, MeshSpec(11, 100 + i % 3), "laplacian", 2, 50 + i) for i in range(6)]
     one = sum(ServingEngine.estimate(reqs[0], "mixed"))
-    cfg = ServingConfig(budget_bytes=1.01 * one, streams=2, max_active=4, chunk=10)
+    cfg = ServingConfig(budget_bytes=1.05 * one, streams=2, max_active=4, chunk=10)
     eng = ServingEngine(cfg)
     out = eng.serve(reqs, realtime=False)
     assert all(r.converged for r in out)

@@ -11,13 +11,15 @@ The engine keeps *systems* (mesh, operator, FSAI factor) resident in HBM under a
 byte budget.  A repeated mesh reuses its system as is; a new mesh whose topology
 is already cached refreshes a cached factor numerically on the stored supports
 (``refit_clone``, the reuse path of fsai_reuse.py) instead of rerunning the
-adaptive search; an unseen topology builds a factor.  Systems live on a pool of
-HIP streams (default priority); every admitted request owns a *slot* (its Krylov
-workspace, captured as a HIP graph of ``chunk`` CG steps) on its system's stream.
-The loop admits arriving requests whenever a slot and budget are free, then
-round-robins chunk launches over all admitted requests, so requests that arrive
-while others are on the GPU join at chunk boundaries and their work overlaps on
-the device; host reads of solver state are asynchronous (pinned buffers + events).
+adaptive search; an unseen topology builds a factor.  Systems are built on a
+worker thread (setup overlaps running solves) and live on a pool of HIP streams
+(default priority); every admitted request owns a *slot* (its Krylov workspace,
+captured as a HIP graph of ``chunk`` CG steps) on its system's stream.  The loop
+admits arrived requests smallest estimated work first whenever a slot and budget
+are free, and launches chunks for the jobs with the least remaining work first up
+to a GPU-fill cap, so requests that arrive while others are on the GPU join at
+chunk boundaries and their work overlaps on the device; host reads of solver state
+are asynchronous (pinned buffers + events).
 
 Stopping is certified: a request finishes when, for every right-hand side, an
 upper bound on the relative mass-weighted field error is <= tol.  The float64
@@ -218,6 +220,8 @@ class System:
             if p.fixed is None:
                 p.fixed = p.heuristic()
             p.partials(8)
+            if p.fixed == p.SLICED:
+                p._sliced()  # build the sliced copy here, not on the scheduling thread
         self.P._scratch(self.vdtype, n * 8)
         if isinstance(self.inner, SquaredLaplacianOperator):
             self.inner._scratch(self.vdtype, n * 8)
@@ -244,7 +248,6 @@ class Slot:
         self.system, self.nvec = system, nvec
         n = system.n
         dev = system.mass.device
-        before = torch.cuda.memory_allocated()
         with torch.cuda.stream(system.stream):
             size = n * nvec
             self.X = torch.zeros(size, dtype=torch.float64, device=dev)
@@ -257,7 +260,10 @@ class Slot:
             self.norms_w = torch.empty(2 * nvec, dtype=torch.float64, device=dev)
             self.work = KrylovWork("cg", system.inner, system.P, n, nvec, vdt, dev)
             self.x_iter = self.C
-        self.bytes = torch.cuda.memory_allocated() - before
+        r, w = self.res, self.work
+        self.bytes = tensor_bytes([self.X, self.B, self.Rin, self.C, self.c64, self.norms_w, r.r,
+                                   r.t_hi, r.t_lo, r.partials, r.norms, w.r, w.p, w.ap, w.z,
+                                   w.map, w.az, w.parts, w.state, w.dot_parts])
         self.state_host = torch.empty(12 * nvec, dtype=torch.float64, pin_memory=True)
         self.norms_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
         self.norms_w_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
@@ -321,6 +327,8 @@ class Job:
         self.best = np.full(req.nrhs, np.inf)
         self.stalls = 0
         self.history = []
+        self.chunks_launched = 0
+        self.fill = gpu_fill(system.n, req.nrhs)
         self.want_check = False
         self.finished = False
         self.result = None
@@ -350,15 +358,28 @@ class Job:
         self.phase = "result"
 
     # ------------------------------------------------------------------ driving
-    def advance(self) -> bool:
+    def poll(self) -> bool:
+        """Consume completed host reads in order."""
         did = False
         while self.pending and self.pending[0][0].query():
             _, what = self.pending.popleft()
             self._on(what)
             did = True
-        if not self.finished and self._enqueue():
-            did = True
         return did
+
+    def enqueue(self) -> bool:
+        return not self.finished and self._enqueue()
+
+    def advance(self) -> bool:
+        did = self.poll()
+        return self.enqueue() or did
+
+    def remaining(self) -> float:
+        """Estimated seconds of GPU work left (scheduling priority)."""
+        est = estimated_iterations(self.req.kind, self.system.spec.side)
+        done = self.chunks_launched * self.cfg.chunk
+        left = max(est - done, self.cfg.chunk)
+        return 1.0e-10 * left * self.system.n * (4 + self.req.nrhs)
 
     def _enqueue(self) -> bool:
         s = self.slot
@@ -385,6 +406,7 @@ class Job:
                     s.state_host.copy_(s.work.state, non_blocking=True)
                     self._record("state")
                     self.chunks_out += 1
+                    self.chunks_launched += 1
                     return True
         return False
 
@@ -488,10 +510,62 @@ class Job:
         self.system.active -= 1
 
 
+def tensor_bytes(*objs) -> int:
+    """Bytes of the distinct device storages reachable from the given objects'
+    attributes (used for budget accounting: exact under concurrent allocation)."""
+    seen_obj, seen_store = set(), set()
+    total = 0
+    stack = list(objs)
+    while stack:
+        o = stack.pop()
+        if o is None or id(o) in seen_obj:
+            continue
+        seen_obj.add(id(o))
+        if isinstance(o, torch.Tensor):
+            if o.is_cuda:
+                st = o.untyped_storage()
+                if st.data_ptr() not in seen_store:
+                    seen_store.add(st.data_ptr())
+                    total += st.nbytes()
+            continue
+        if isinstance(o, (list, tuple)):
+            stack.extend(o)
+        elif isinstance(o, dict):
+            stack.extend(o.values())
+        elif hasattr(o, "__dict__") and type(o).__module__.startswith(__package__):
+            stack.extend(v for k, v in vars(o).items() if k not in ("system", "eng", "template"))
+    return total
+
+
+def estimated_iterations(kind: str, side: int) -> float:
+    """Inner CG steps per right-hand side to a certified 1e-8 on the jittered grid of
+    `side` (fitted to measurements: sides 16..96, refinement included)."""
+    return 6.3 * side if kind == "laplacian" else 1.6 * side * side
+
+
+def estimated_seconds(req: Request, cached: bool) -> float:
+    n = req.mesh.side ** 3
+    work = 1.0e-10 * estimated_iterations(req.kind, req.mesh.side) * n * (4 + req.nrhs)
+    return work + (0.0 if cached else 0.2 * n / 96 ** 3)
+
+
+def gpu_fill(n: int, nrhs: int) -> float:
+    """Fraction of the device one system's products keep busy (~1 from about two
+    million unknown-RHS pairs up)."""
+    return min(1.0, n * nrhs / 2.0e6)
+
+
 class ServingEngine:
-    """Admission, factor cache and the chunk scheduler (one GPU, one host thread)."""
+    """Admission, factor cache and the chunk scheduler (one GPU; scheduling on the
+    calling thread, system setup on one worker thread)."""
+
+    FILL_CAP = 1.5  # launches go to the highest-priority jobs up to this much GPU fill
+    AGING = 0.5  # admission priority: estimated seconds - AGING * seconds waited
 
     def __init__(self, config: ServingConfig | None = None, device=None):
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+
         self.cfg = config or ServingConfig()
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         torch.cuda.set_device(self.device)
@@ -500,6 +574,10 @@ class ServingEngine:
         self._next = 0
         self.systems = OrderedDict()  # (kind, mesh, alpha) -> System, LRU order
         self.templates = {}  # (kind, side, alpha, policy) -> System whose factor is refit
+        self._templates_lock = threading.Lock()
+        self._pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="wpc-setup")
+        self.building = {}  # key -> (future, estimated bytes)
+        self.waiting = {}  # key -> [(request, admitted at)]
         self.t0 = None
         self.stats = dict(built=0, refit=0, cached=0, evicted=0, over_budget=0, setup_s=0.0,
                           peak_bytes=0)
@@ -508,13 +586,17 @@ class ServingEngine:
         return time.perf_counter() - self.t0
 
     def used_bytes(self):
-        return sum(s.bytes + sum(x.bytes for v in s.slots.values() for x in v)
-                   for s in self.systems.values())
+        return (sum(s.bytes + sum(x.bytes for v in s.slots.values() for x in v)
+                    for s in self.systems.values())
+                + sum(b for _, b in self.building.values()))
 
     @staticmethod
     def estimate(req: Request, policy: str):
+        """Upper bounds of a system's bytes (measured 900 B per unknown for Laplacian+mass,
+        1,210 for the biharmonic kind, whose build also holds the assembled Q, ~780) and
+        of a slot's (exact: 56 or 96 B per unknown and right-hand side)."""
         n = req.mesh.side ** 3
-        system = (1500 if req.kind == "biharmonic" else 700) * n
+        system = (2000 if req.kind == "biharmonic" else 1000) * n
         slot = (96 if policy == "f64" else 56) * n * req.nrhs
         return system, slot
 
@@ -525,88 +607,130 @@ class ServingEngine:
             if victim is None:
                 return False
             s = self.systems.pop(victim)
-            for key, t in list(self.templates.items()):
-                if t is s:
-                    del self.templates[key]
+            with self._templates_lock:
+                for key, t in list(self.templates.items()):
+                    if t is s:
+                        del self.templates[key]
             s.release()
             self.stats["evicted"] += 1
         return True
 
-    def _system(self, req: Request):
-        key = (req.kind, req.mesh, req.alpha)
-        s = self.systems.get(key)
-        if s is not None:
-            self.systems.move_to_end(key)
-            self.stats["cached"] += 1
-            return s, "cached"
+    # ------------------------------------------------------------ setup worker
+    def _build(self, req: Request, stream):
+        """Runs on the setup thread: assemble the mesh operator, build or refit the
+        factor on the setup stream; returns (system, ready event, seconds)."""
+        torch.cuda.set_device(self.device)
+        t = time.perf_counter()
         policy = self.cfg.precision[req.kind]
         tkey = (req.kind, req.mesh.side, req.alpha, policy)
-        template = self.templates.get(tkey)
-        stream = self.streams[self._next % len(self.streams)]
-        self._next += 1
-        t = time.perf_counter()
-        before = torch.cuda.memory_allocated()
+        with self._templates_lock:
+            template = self.templates.get(tkey)
         with torch.cuda.stream(self.setup_stream):
             s = System(self.cfg, req.kind, req.mesh, req.alpha, stream, template)
             ready = torch.cuda.Event()
             ready.record(self.setup_stream)
-        stream.wait_event(ready)
-        s.bytes = torch.cuda.memory_allocated() - before
-        self.stats["setup_s"] += time.perf_counter() - t
-        self.systems[key] = s
+        s.bytes = tensor_bytes(s)
         if template is None:
-            self.templates[tkey] = s
-        self.stats[s.provenance] += 1
-        return s, s.provenance
+            with self._templates_lock:
+                self.templates.setdefault(tkey, s)
+        return s, ready, time.perf_counter() - t
 
-    def _admit(self, req: Request, active) -> bool:
-        if len(active) >= self.cfg.max_active:
-            return False
+    def _collect_builds(self, active):
+        for key, (fut, _) in list(self.building.items()):
+            if not fut.done():
+                continue
+            del self.building[key]
+            s, ready, seconds = fut.result()
+            s.stream.wait_event(ready)
+            self.stats["setup_s"] += seconds
+            self.stats[s.provenance] += 1
+            self.systems[key] = s
+            for k, (req, t_admit) in enumerate(self.waiting.pop(key, [])):
+                if k:
+                    self.stats["cached"] += 1
+                active.append(Job(self, req, s, s.slot(req.nrhs),
+                                  s.provenance if k == 0 else "cached", t_admit))
+            self.stats["peak_bytes"] = max(self.stats["peak_bytes"], self.used_bytes())
+
+    # ------------------------------------------------------------ admission
+    def _admit(self, req: Request, active, force=False) -> bool:
         key = (req.kind, req.mesh, req.alpha)
         s = self.systems.get(key)
         sys_b, slot_b = self.estimate(req, self.cfg.precision[req.kind])
-        need = (0 if s is not None else sys_b) + slot_b
+        need = (0 if (s is not None or key in self.building) else sys_b) + slot_b
         if not self._make_room(need, keep=s):
-            if active:
+            if not force:
                 return False
-            # nothing in flight and still over budget: admit alone rather than stall
             self.stats["over_budget"] += 1
-        system, provenance = self._system(req)
-        slot = system.slot(req.nrhs)
-        active.append(Job(self, req, system, slot, provenance, self.clock()))
+        if s is not None:
+            self.systems.move_to_end(key)
+            self.stats["cached"] += 1
+            active.append(Job(self, req, s, s.slot(req.nrhs), "cached", self.clock()))
+        elif key in self.building:
+            self.waiting[key].append((req, self.clock()))
+        else:
+            stream = self.streams[self._next % len(self.streams)]
+            self._next += 1
+            self.building[key] = (self._pool.submit(self._build, req, stream), sys_b)
+            self.waiting[key] = [(req, self.clock())]
         self.stats["peak_bytes"] = max(self.stats["peak_bytes"], self.used_bytes())
         return True
+
+    def _priority(self, req: Request, now: float) -> float:
+        cached = (req.kind, req.mesh, req.alpha) in self.systems
+        return estimated_seconds(req, cached) - self.AGING * max(0.0, now - req.arrival)
 
     def serve(self, requests, realtime=True, on_result=None):
         """Run the request list to completion; returns Results in request order.
         With ``realtime`` each request becomes visible at its arrival time; otherwise
-        all are available at once (admission is still limited by slots and budget)."""
+        all are available at once.  Arrived requests are admitted smallest estimated
+        work first (aged by waiting time) while slots and budget allow; chunk launches
+        go to the jobs with the least estimated remaining work first, up to FILL_CAP
+        of GPU fill, so small requests run side by side and a large one alone."""
         self.t0 = time.perf_counter()
         if self.cfg.split_rhs:
             return self._serve_split(requests, realtime, on_result)
         queue = deque(sorted(requests, key=lambda r: r.arrival))
-        active, results = [], {}
-        while queue or active:
-            now = self.clock()
-            while queue and (not realtime or queue[0].arrival <= now):
-                if not self._admit(queue[0], active):
-                    break
-                queue.popleft()
-            progressed = False
-            for job in active:
-                progressed |= job.advance()
-            done = [j for j in active if j.finished]
-            for j in done:
-                active.remove(j)
-                j.release()
-                results[j.req.rid] = j.result
-                if on_result is not None:
-                    on_result(j.result)
-            if not progressed and not done:
-                if active:
-                    time.sleep(50e-6)
-                elif queue and realtime:
-                    time.sleep(max(0.0, min(queue[0].arrival - self.clock(), 0.01)))
+        ready, active, results = [], [], {}
+        try:
+            while queue or ready or active or self.building:
+                now = self.clock()
+                while queue and (not realtime or queue[0].arrival <= now):
+                    ready.append(queue.popleft())
+                self._collect_builds(active)
+                while ready:
+                    pending = len(active) + sum(len(v) for v in self.waiting.values())
+                    if pending >= self.cfg.max_active:
+                        break
+                    req = min(ready, key=lambda r: self._priority(r, now))
+                    if not self._admit(req, active, force=pending == 0):
+                        break
+                    ready.remove(req)
+                progressed = False
+                for job in active:
+                    progressed |= job.poll()
+                fill = 0.0
+                for job in sorted(active, key=Job.remaining):
+                    if fill >= self.FILL_CAP:
+                        break
+                    if not job.finished:
+                        progressed |= job.enqueue()
+                        fill += job.fill
+                done = [j for j in active if j.finished]
+                for j in done:
+                    active.remove(j)
+                    j.release()
+                    results[j.req.rid] = j.result
+                    if on_result is not None:
+                        on_result(j.result)
+                if not progressed and not done:
+                    if active or self.building:
+                        time.sleep(50e-6)
+                    elif queue and realtime:
+                        time.sleep(max(0.0, min(queue[0].arrival - self.clock(), 0.01)))
+        finally:
+            for fut, _ in self.building.values():
+                fut.cancel()
         return [results[r.rid] for r in requests]
 
     def _serve_split(self, requests, realtime, on_result):
