@@ -14,7 +14,7 @@ namespace wpc {
 constexpr int kKrylovThreads = kStreamRows;
 
 int krylov_blocks(int n) {
-    long b = ((long)n + 255) / 256;
+    long b = ((long)n + 511) / 512;
     if (b > 1024) b = 1024;
     return (int)(b < 1 ? 1 : b);
 }
