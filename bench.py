@@ -485,6 +485,50 @@ def block_bench(name):
                       "methods": records}), flush=True)
 
 
+def apply_bench(sides=(64, 96), nvecs=(1, 3, 8), reps=50):
+    """FSAI-apply HBM throughput: z = G^T (G x) with the width-8 factor of the
+    assembled biharmonic Q (device-assembled on the jittered grid, float32 factor,
+    float64 vectors), both passes timed together with HIP events on the launch
+    stream; algorithmic bytes 2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (+ the
+    r.z dot read when fused).  Prints one JSON line."""
+    import torch
+
+    import warp_preconditioners as wp
+    from warp_preconditioners import assembly
+
+    dev = torch.device("cuda", 0)
+    rows = []
+    for side in sides:
+        v, t = assembly.jittered_grid(side, 0, dev)
+        K, m = assembly.p1_laplacian(v, t)
+        Q = assembly.squared_operator(K, m, 1.0)
+        del v, t
+        P = wp.FSAI(Q, max_row_size=8, factor_dtype=torch.float32)
+        n, nnz = P.G.nrow, P.G.nnz
+        for k in nvecs:
+            x = torch.randn(n * k, dtype=torch.float64, device=dev)
+            z = torch.empty_like(x)
+            for _ in range(3):
+                P.apply_batch(x, x, z, 1.0, 0.0, k)
+            stream = torch.cuda.current_stream()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                P.apply_batch(x, x, z, 1.0, 0.0, k)
+            e1.record(stream)
+            e1.synchronize()
+            us = e0.elapsed_time(e1) / reps * 1e3
+            nbytes = 2 * nnz * (4 + 4) + 8 * (n + 1) + 4 * n * k * 8
+            pg, pgt = P.plans()
+            rows.append(dict(side=side, n=n, nnzG=nnz, nvec=k, us=us, GBs=nbytes / us / 1e3,
+                             frac=nbytes / us / 1e3 / HBM_PEAK_GBS,
+                             mapping=[pg.mode(k, torch.float64), pgt.mode(k, torch.float64)]))
+            log(json.dumps(rows[-1]))
+        del P, Q, K, m
+    print(json.dumps({"metric": "FSAI-apply HBM GB/s", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                      "rows": rows}), flush=True)
+
+
 def serving_bench(args):
     """Serving stream: Poisson arrivals (200/s, seed 7), meshes of side U{16..max},
     Laplacian+mass or factored biharmonic, 1-8 N(0,1) right-hand sides, meshes
@@ -587,7 +631,7 @@ def main():
                     help="force the row mapping of the G, GT and K products, e.g. --mapping=4,0,-1 "
                          "(0 = streamed, L = L lanes per row, -1 = sliced); default autotunes")
     ap.add_argument("--workload", default="biharmonic64",
-                    choices=["biharmonic64", "serving", "kron48", "elasticity"],
+                    choices=["biharmonic64", "serving", "kron48", "elasticity", "apply"],
                     help="biharmonic64: the factored-biharmonic 64^3 solve (default); serving: "
                          "the request stream through the continuous-batching engine; kron48 / "
                          "elasticity: the block-coupled and elasticity solver comparisons")
@@ -608,6 +652,9 @@ def main():
         return
     if args.workload in ("kron48", "elasticity"):
         block_bench(args.workload)
+        return
+    if args.workload == "apply":
+        apply_bench()
         return
     wl = build_workload(args.side)
     s = Solver(wl, args.mode)
