@@ -2,11 +2,12 @@
 
     python tests/golden/make_golden.py [--threads 8]
 
-* config1_fsai.npz -- full factor of the CPU-runnable configuration: 64x64 interior
-  five-point Laplacian (diag 4, off -1), float32, FSAI width 8, step 1,
+* laplace2d64_fsai_w8.npz -- full factor of the 64x64-interior 2D five-point
+  Laplacian (diag 4, off -1), float32, FSAI width 8, step 1,
   kap_tolerance 1e-3 (offsets, columns, values), the oracle's truncated-row
   count, and the float64 PCG iteration count to 1e-8 for b ~ N(0,1) seed 0.
-* patterns.json + rows_*.npz -- for the full-size configurations: SHA-256 of the
+* patterns.json + rows_*.npz -- for the full-size biharmonic 64^3 Q, K (x) C 48^3
+  and elasticity-beam matrices: SHA-256 of the
   factor pattern (offsets then columns as little-endian int32) and of the values,
   plus 1000 sampled rows (seed 1) with their columns and values.
 """
@@ -58,16 +59,16 @@ def config_matrices():
     v, t, _ = workloads.kuhn_grid(64, 0.15, seed=0)
     K, m = workloads.p1_laplacian(v, t)
     Q = workloads.squared_operator(K, m, 1.0)
-    out.append(("config2_Q_w8", cpu.CSR.from_scipy(Q, np.float64),
+    out.append(("biharmonic64_Q_w8", cpu.CSR.from_scipy(Q, np.float64),
                 dict(max_row_size=8, factor_dtype=np.float32)))
     off, col, val, _, _ = workloads.kron_coupling(48, 1.0e6, seed=0)
     A3 = cpu.CSR(off, col, np.ascontiguousarray(val), len(off) - 1, len(off) - 1, 3)
-    out.append(("config3_KxC_w16", A3, dict(max_row_size=16)))
+    out.append(("kron48_KxC_w16", A3, dict(max_row_size=16)))
     vb, tb, _ = workloads.kuhn_grid(None, 0.15, seed=0, shape=(160, 40, 40))
     Kb, _, _, _ = workloads.p1_elasticity(vb, tb, vb[:, 0] == 0.0, young=1.0, poisson=0.45)
     A4 = cpu.CSR(Kb.indptr.astype(np.int32), Kb.indices.astype(np.int32),
                  np.ascontiguousarray(Kb.data), Kb.shape[0] // 3, Kb.shape[1] // 3, 3)
-    out.append(("config4_elasticity_w8", A4, dict(max_row_size=8, factor_dtype=np.float32)))
+    out.append(("beam40x40x160_elasticity_w8", A4, dict(max_row_size=8, factor_dtype=np.float32)))
     return out
 
 
@@ -84,10 +85,10 @@ def main():
     P64 = dict(G=cpu.CSR(P["G"].off, P["G"].col, P["G"].val.astype(np.float64), 4096, 4096),
                GT=cpu.CSR(P["GT"].off, P["GT"].col, P["GT"].val.astype(np.float64), 4096, 4096))
     _, iters, _ = cpu.krylov("cg", b, A=cpu.CSR.from_scipy(lap), P=P64, tol=1e-8, maxiter=4096)
-    np.savez_compressed(HERE / "config1_fsai.npz", offsets=P["G"].off, columns=P["G"].col,
+    np.savez_compressed(HERE / "laplace2d64_fsai_w8.npz", offsets=P["G"].off, columns=P["G"].col,
                         values=P["G"].val, truncated=np.int32(P["truncated_rows"]),
                         cg_iterations=np.int32(iters))
-    print("config1: nnz", P["G"].nnz, "truncated", P["truncated_rows"], "CG iterations", iters)
+    print("laplace2d64: nnz", P["G"].nnz, "truncated", P["truncated_rows"], "CG iterations", iters)
     if args.skip_large:
         return
     meta = {}

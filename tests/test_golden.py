@@ -8,10 +8,10 @@ import numpy as np
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
-def test_config1_factor_and_iterations(oracle):
+def test_laplace2d64_factor_and_iterations(oracle):
     from warp_preconditioners import workloads
 
-    g = np.load(GOLDEN / "config1_fsai.npz")
+    g = np.load(GOLDEN / "laplace2d64_fsai_w8.npz")
     lap = workloads.laplacian_2d(64)
     P = oracle.fsai(oracle.CSR.from_scipy(lap, np.float32), max_row_size=8)
     np.testing.assert_array_equal(P["G"].off, g["offsets"])

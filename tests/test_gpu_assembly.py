@@ -54,7 +54,7 @@ def test_squared_operator_bitwise_equals_host_spgemm(alpha):
 
 
 def test_device_pipeline_factor_matches_oracle_hash():
-    # Full-size configuration 2 assembled from the host K on the device: the FSAI
+    # The full 64^3 biharmonic Q assembled from the host K on the device: the FSAI
     # factor equals the oracle's (golden SHA-256).
     import hashlib
     import json
@@ -70,6 +70,6 @@ def test_device_pipeline_factor_matches_oracle_hash():
     h = hashlib.sha256()
     h.update(host(P.G.offsets).tobytes())
     h.update(host(P.G.columns).tobytes())
-    assert h.hexdigest() == meta["config2_Q_w8"]["pattern_sha256"]
+    assert h.hexdigest() == meta["biharmonic64_Q_w8"]["pattern_sha256"]
     assert hashlib.sha256(host(P.G.values).tobytes()).hexdigest() == \
-        meta["config2_Q_w8"]["values_sha256"]
+        meta["biharmonic64_Q_w8"]["values_sha256"]

@@ -173,7 +173,7 @@ def test_build_matches_oracle_bitwise(wp, oracle, dtype, width, step, tol):
 
 @pytest.mark.parametrize("block", [1, 3])
 def test_build_matches_oracle_on_blocks_and_grid(wp, oracle, block):
-    # 2D five-point Laplacian (the CPU-runnable configuration's operator) and its
+    # 2D five-point Laplacian (24x24 interior grid) and its
     # 3x3-blocked copy: the factor pattern and values must match bit for bit.
     a = laplacian_2d(24).toarray() if block == 1 else np.kron(laplacian_2d(8).toarray(),
                                                                np.array([[4, 1, 0], [1, 4, 1], [0, 1, 4.0]]))

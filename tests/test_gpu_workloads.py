@@ -23,10 +23,10 @@ def digest(*arrays):
     return h.hexdigest()
 
 
-def test_config1_factor_bitwise_and_cg(oracle):
+def test_laplace2d64_factor_bitwise_and_cg(oracle):
     from warp_preconditioners import FSAI, linear, workloads
 
-    g = np.load(GOLDEN / "config1_fsai.npz")
+    g = np.load(GOLDEN / "laplace2d64_fsai_w8.npz")
     lap = workloads.laplacian_2d(64)
     A = upload(lap, torch.float32)
     P = FSAI(A, max_row_size=8)
@@ -46,15 +46,15 @@ def test_config1_factor_bitwise_and_cg(oracle):
     assert abs(it32 - int(g["cg_iterations"])) <= max(1, 0.02 * int(g["cg_iterations"]))
 
 
-def _config(name):
+def _workload(name):
     from warp_preconditioners import workloads
     from warp_preconditioners.sparse import BsrMatrix
 
-    if name == "config2_Q_w8":
+    if name == "biharmonic64_Q_w8":
         v, t, _ = workloads.kuhn_grid(64, 0.15, seed=0)
         K, m = workloads.p1_laplacian(v, t)
         return upload(workloads.squared_operator(K, m, 1.0))
-    if name == "config3_KxC_w16":
+    if name == "kron48_KxC_w16":
         off, col, val, _, _ = workloads.kron_coupling(48, 1.0e6, seed=0)
         return BsrMatrix(len(off) - 1, len(off) - 1, (3, 3), torch.tensor(off, device=dev()),
                          torch.tensor(col, device=dev()), torch.tensor(val, device=dev()))
@@ -66,12 +66,12 @@ def _config(name):
                      torch.tensor(np.ascontiguousarray(Kb.data), device=dev()))
 
 
-@pytest.mark.parametrize("name", ["config2_Q_w8", "config3_KxC_w16", "config4_elasticity_w8"])
+@pytest.mark.parametrize("name", ["biharmonic64_Q_w8", "kron48_KxC_w16", "beam40x40x160_elasticity_w8"])
 def test_full_size_factor_matches_oracle_hash(name):
     from warp_preconditioners import FSAI
 
     meta = json.loads((GOLDEN / "patterns.json").read_text())[name]
-    A = _config(name)
+    A = _workload(name)
     fd = {"float32": torch.float32, "float64": torch.float64}[meta["fsai"]["factor_dtype"]]
     P = FSAI(A, max_row_size=meta["fsai"]["max_row_size"], factor_dtype=fd)
     off, col, val = host(P.G.offsets), host(P.G.columns), host(P.G.values)
