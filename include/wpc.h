@@ -470,6 +470,63 @@ int wpc_refine_update(int n, int nvec, double* x, const void* c, int c_dtype, co
 int wpc_add_diagonal(int n, const int* offsets, const int* columns, double* values,
                      const double* d, double alpha, int* status, void* stream);
 
+/* ------------------------------------------------------------------ GMRES basis
+ * The Arnoldi products of restarted GMRES (warp.optim.linear.gmres as called at
+ * tests/test_indefinite.py:130 and tests/test_biharmonic.py:312) on a row-major
+ * basis V (row j at V + j*ld, ld >= n), 1 <= k <= 128 rows:
+ *   wpc_gram:          h[j] = V[j] . w (double, deterministic two-stage sum;
+ *                      partials: double[wpc_gram_partials(n) * k])
+ *   wpc_basis_combine: w += alpha * sum_j h[j] V[j] (accumulated in double). */
+int wpc_gram_partials(long n);
+int wpc_gram(long n, int k, const void* V, long ld, const void* w, int dtype, double* partials,
+             double* h, void* stream);
+int wpc_basis_combine(long n, int k, const void* V, long ld, const double* h, double alpha,
+                      void* w, int dtype, void* stream);
+
+/* ------------------------------------------------------------------ mixed systems
+ * Interleaved 2/3-field indefinite harmonic systems and their two block
+ * preconditioners.  Vertex i of a system vector holds x[i*order + c]; fields
+ * are n-vectors.  Arithmetic follows the reference expressions without
+ * contraction.  dtype is WPC_F32 or WPC_F64 for every array of one call. */
+
+/* mixed.py:19-57 (MixedHarmonicSystem's build kernel).  Writes n + nnz(K)
+ * triplets: rows/cols int32, blocks [count][order][order], hvalues [count]
+ * (the diagonal triplets carry weight*M / -M and shift*M, the others -K and
+ * K); the caller sums duplicates.  bad[0] counts non-positive or non-finite
+ * mass entries. */
+int wpc_mixed_build(int n, int order, const int* offsets, const int* columns, const void* values,
+                    const void* mass, double weight, double shift, int* rows, int* cols,
+                    void* blocks, void* hvalues, int* bad, int dtype, void* stream);
+/* mixed.py:66-79: out = x[:, c], or x[:, c] / mass when mass is not NULL. */
+int wpc_mixed_component(int n, int order, const void* x, int c, const void* mass, void* out,
+                        int dtype, void* stream);
+/* mixed.py:81-86, 118-121: out = mass * x (divide = 0) or x / mass (divide = 1);
+ * out may alias x. */
+int wpc_mass_multiply(int n, const void* x, const void* mass, int divide, void* out, int dtype,
+                      void* stream);
+/* mixed.py:88-102: out = x[:, 0] - k1 (+ k2 when order = 3). */
+int wpc_mixed_rhs(int n, int order, const void* x, const void* k1, const void* k2, void* out,
+                  int dtype, void* stream);
+/* mixed.py:104-116: out = -(ku + x[:, c]) / mass. */
+int wpc_mixed_auxiliary(int n, int order, const void* x, int c, const void* ku, const void* mass,
+                        void* out, int dtype, void* stream);
+/* mixed.py:123-143: z = alpha * (u, a[, lam]) + beta * y; alpha = 0 reads no
+ * field and beta = 0 never reads y; y and z may alias. */
+int wpc_mixed_finish(int n, int order, const void* u, const void* a, const void* lam,
+                     const void* y, void* z, double alpha, double beta, int dtype, void* stream);
+/* mixed.py:330-345 (ShiftedBlock unpack); c is unused when order = 2. */
+int wpc_shifted_unpack(int n, int order, const void* x, void* a, void* b, void* c, double scale,
+                       int dtype, void* stream);
+/* mixed.py:347-372 (ShiftedBlock finish): order 2 gives (a / scale, b * scale),
+ * order 3 (a, b, c), then z = alpha * value + beta * y as above. */
+int wpc_shifted_finish(int n, int order, const void* a, const void* b, const void* c,
+                       const void* y, void* z, double scale, double alpha, double beta, int dtype,
+                       void* stream);
+/* mixed.py:147-160: out[k] = s_{k mod order} * x[k] for count = n * order
+ * interleaved entries (order 1 scales a plain vector by s0); out may alias x. */
+int wpc_vector_scale(long count, int order, const void* x, void* out, double s0, double s1,
+                     double s2, int dtype, void* stream);
+
 /* ------------------------------------------------------------------ graphs
  * No reference counterpart (Warp captures with its own graph API around
  * cg/cr): capture the library launches issued on `stream` between begin and end

@@ -429,6 +429,106 @@ def bsr_spmv(A: CSR, x, y=None, alpha=1.0, beta=0.0):
     return z
 
 
+# ---------------------------------------------------------------- mixed systems
+def mixed_system(K: CSR, mass, order=2, data_weight=1.0, equilibrate=False):
+    """mixed.py:180-243 restated: dict(matrix, H, mass, scale, data_weight).
+
+    matrix is the interleaved order x order BSR sum of the vertex blocks
+    (weight*m, -m couplings) and the K blocks (-k couplings); H = K + shift*M.
+    Duplicates (K's diagonal) sum two terms, so the summation order is moot.
+    """
+    dt = np.dtype(K.dtype)
+    n = K.nrow
+    mass = np.array(mass, dt)
+    scale = np.ones(order, dt)
+    if equilibrate:
+        root = data_weight ** (1 / order)
+        scale = np.array([data_weight ** -0.25, data_weight ** 0.25] if order == 2
+                         else [1 / root, 1.0, root], dt)
+        mass = (dt.type(root) * mass).astype(dt)
+        data_weight = 1.0
+    if np.any(~np.isfinite(mass)) or np.any(mass <= 0):
+        raise ValueError("mass must be finite positive")
+    weight, shift = dt.type(data_weight), dt.type(data_weight ** (1 / order))
+    rows = np.concatenate([np.arange(n), np.repeat(np.arange(n), np.diff(K.off))])
+    cols = np.concatenate([np.arange(n), K.col])
+    blocks = np.zeros((n + K.nnz, order, order), dt)
+    blocks[:n, 0, 0] = weight * mass
+    k = -K.val
+    if order == 2:
+        blocks[:n, 1, 1] = -mass
+        blocks[n:, 0, 1] = blocks[n:, 1, 0] = k
+    else:
+        blocks[:n, 1, 2] = blocks[:n, 2, 1] = -mass
+        blocks[n:, 0, 2] = blocks[n:, 2, 0] = blocks[n:, 1, 1] = k
+    hvals = np.concatenate([shift * mass, K.val]).astype(dt)
+    key = rows.astype(np.int64) * n + cols
+    perm = np.argsort(key, kind="stable")
+    uniq, first = np.unique(key[perm], return_index=True)
+    vals = np.add.reduceat(blocks[perm], first, axis=0).astype(dt)
+    hsum = np.add.reduceat(hvals[perm], first).astype(dt)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(np.bincount(uniq // n, minlength=n))
+    col = uniq % n
+    matrix = CSR(_i32(off), _i32(col), vals, n, n, order)
+    H = CSR(_i32(off), _i32(col), hsum, n, n, 1)
+    return dict(matrix=matrix, H=H, K=K, mass=mass, scale=scale, order=order,
+                data_weight=data_weight)
+
+
+def matching_schur_apply(system, inverse, x, y=None, alpha=1.0, beta=0.0):
+    """mixed.py:305-322: block elimination with repeated H inverses.
+
+    ``inverse(v)`` applies the approximate H^-1 (e.g. ``lambda v: fsai_apply(P, v)``).
+    """
+    K, m, order = system["K"], system["mass"], system["order"]
+    dt = m.dtype
+    x = np.asarray(x, dt).reshape(-1, order)
+    v = spmv(K, x[:, 1] / m)
+    w = np.zeros_like(m)
+    if order == 3:
+        w = spmv(K, x[:, 2] / m)
+        w = spmv(K, w / m)
+    g = x[:, 0] - v
+    if order == 3:
+        g = g + w
+    u = inverse(g)
+    for step in range(order - 1):
+        u = inverse(m * u)
+        if order == 3 and step == 1:
+            u = -u
+    v = spmv(K, u)
+    a = -(v + x[:, 1 if order == 2 else 2]) / m
+    lam = np.zeros_like(m)
+    if order == 3:
+        v = spmv(K, a)
+        lam = -(v + x[:, 1]) / m
+    fields = [u, a, lam][:order]
+    value = np.zeros_like(x) if alpha == 0 else np.stack(fields, 1) * dt.type(alpha)
+    if beta != 0:
+        value = value + dt.type(beta) * np.asarray(y, dt).reshape(-1, order)
+    return value
+
+
+def shifted_block_apply(system, inverse, x, y=None, alpha=1.0, beta=0.0):
+    """mixed.py:330-372 and 415-427: one H inverse per field."""
+    m, order = system["mass"], system["order"]
+    dt = m.dtype
+    x = np.asarray(x, dt).reshape(-1, order)
+    scale = dt.type(system["data_weight"] ** 0.25)
+    half = dt.type(0.5)
+    if order == 2:
+        a = (x[:, 0] / scale - scale * x[:, 1]) * half
+        b = (-x[:, 0] / scale - scale * x[:, 1]) * half
+        value = np.stack([inverse(a) / scale, inverse(b) * scale], 1)
+    else:
+        value = np.stack([inverse(-x[:, 2]), inverse(-x[:, 1]), inverse(-x[:, 0])], 1)
+    value = np.zeros_like(x) if alpha == 0 else value * dt.type(alpha)
+    if beta != 0:
+        value = value + dt.type(beta) * np.asarray(y, dt).reshape(-1, order)
+    return value
+
+
 # ---------------------------------------------------------------- Krylov
 def krylov(method, b, *, A: CSR | None = None, K: CSR | None = None, mass=None, weight=0.0,
            P=None, diag=None, x0=None, tol=1e-8, atol=0.0, maxiter=10000, check_every=1,

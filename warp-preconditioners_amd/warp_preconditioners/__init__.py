@@ -11,6 +11,7 @@ from .biharmonic import BiharmonicSystem, RepeatedFSAI
 from .block_fsai import BlockFSAI, BsrBlockFSAI
 from .block_ilu import BlockILU0, BlockJacobi
 from .fsai import FSAI
+from .mixed import MatchingSchur, MixedHarmonicSystem, ShiftedBlock
 from .sparse_operator import SparseOperator
 from .squared_laplacian import SquaredLaplacianOperator
 
@@ -22,6 +23,9 @@ __all__ = [
     "SquaredLaplacianOperator",
     "BlockJacobi",
     "BlockILU0",
+    "MixedHarmonicSystem",
+    "ShiftedBlock",
+    "MatchingSchur",
     "BlockFSAI",
     "BsrBlockFSAI",
 ]

@@ -343,16 +343,21 @@ def solve_batched(method, A, B, X, M=None, tol=1.0e-8, atol=0.0, maxiter=10000, 
                 targets=[float(v) ** 0.5 for v in st[2]], chunks=chunks, work=work)
 
 
+GMRES_MAX_RESTART = 127  # basis rows the Gram kernel takes (128 with the new vector)
+
+
 def gmres(A, b, x, tol=None, atol=None, restart=31, maxiter=0, M=None, callback=None,
           check_every=31, use_cuda_graph=True, is_left_preconditioner=False):
     """Restarted right-preconditioned GMRES; returns (iterations, residual, atol).
 
     The signature follows warp.optim.linear.gmres as the reference's tests call it
     (tests/test_biharmonic.py:312, tests/test_indefinite.py:130).  Operator and
-    preconditioner applies run on the HIP kernels; the Krylov basis is kept on the
-    device and orthogonalized by classical Gram-Schmidt with one
-    reorthogonalization; the small Hessenberg least-squares problem is solved on
-    the host with Givens rotations.  Left preconditioning is not supported.
+    preconditioner applies, the Gram products and basis updates (classical
+    Gram-Schmidt with one reorthogonalization) and the norms run on the HIP
+    kernels with the Krylov basis resident on the device; only the small
+    Hessenberg least-squares problem is solved on the host, with Givens rotations
+    (one device-to-host copy of the new Hessenberg column per iteration).  Left
+    preconditioning is not supported.
     """
     if is_left_preconditioner:
         raise NotImplementedError("gmres supports right preconditioning only")
@@ -362,30 +367,47 @@ def gmres(A, b, x, tol=None, atol=None, restart=31, maxiter=0, M=None, callback=
     nat.require_device(bf, "b")
     n = bf.numel()
     dt = bf.dtype
+    code = nat.code(dt)
     if maxiter == 0:
         maxiter = A.shape[0]
     if tol is None and atol is None:
         tol = _default_tol(dt)
     tol = 0.0 if tol is None else float(tol)
     atol = 0.0 if atol is None else float(atol)
-    m = max(1, int(restart))
-    V = torch.empty((m + 1, n), dtype=dt, device=bf.device)
-    Z = torch.empty((m, n), dtype=dt, device=bf.device) if M is not None else V
+    m = max(1, min(int(restart), GMRES_MAX_RESTART))
+    dev = bf.device
+    V = torch.empty((m + 1, n), dtype=dt, device=dev)
+    Z = torch.empty((m, n), dtype=dt, device=dev) if M is not None else V
     r = torch.empty_like(bf)
     w = torch.empty_like(bf)
-    bnorm = float(torch.linalg.vector_norm(bf.double()))
-    target = max(atol, tol * bnorm)
+    s = nat.stream()
+    parts = torch.empty(nat.lib().wpc_gram_partials(n) * (m + 1), dtype=torch.float64, device=dev)
+    dparts = torch.empty(nat.lib().wpc_dot_partials(n), dtype=torch.float64, device=dev)
+    # per iteration: h (m+1), h2 (m+1), ||w||^2 (1); the final coefficients y (m)
+    col = torch.zeros(2 * (m + 1) + 1, dtype=torch.float64, device=dev)
+    ycoef = torch.empty(m, dtype=torch.float64, device=dev)
+
+    def norm(v):
+        nat.call("wpc_dot", n, 1, nat.ptr(v), nat.ptr(v), code, nat.ptr(dparts), nat.ptr(col[-1:]),
+                 s)
+        return math.sqrt(max(float(col[-1].item()), 0.0))
+
+    def scaled(src, dst, factor):
+        nat.call("wpc_vector_scale", n, 1, nat.ptr(src), nat.ptr(dst), float(factor), 0.0, 0.0,
+                 code, s)
 
     def residual():
         A.matvec(xf, bf, r, -1.0, 1.0)
-        return float(torch.linalg.vector_norm(r.double()))
+        return norm(r)
 
+    bnorm = norm(bf)
+    target = max(atol, tol * bnorm)
     beta = residual()
     iters = 0
     if callback is not None:
         callback(0, beta, target)
     while iters < maxiter and beta > target:
-        V[0].copy_(r / beta)
+        scaled(r, V[0], 1.0 / beta)
         H = np.zeros((m + 1, m))
         cs, sn = np.zeros(m), np.zeros(m)
         g = np.zeros(m + 1)
@@ -395,16 +417,21 @@ def gmres(A, b, x, tol=None, atol=None, restart=31, maxiter=0, M=None, callback=
             if M is not None:
                 M.matvec(V[j], Z[j], Z[j], 1.0, 0.0)
             A.matvec(Z[j], w, w, 1.0, 0.0)
-            h = torch.mv(V[: j + 1], w)
-            w.sub_(torch.mv(V[: j + 1].t(), h))
-            h2 = torch.mv(V[: j + 1], w)
-            w.sub_(torch.mv(V[: j + 1].t(), h2))
-            hcol = (h + h2).double().cpu().numpy()
-            hn = float(torch.linalg.vector_norm(w.double()))
+            h, h2 = col[: j + 1], col[m + 1: m + 2 + j]
+            for coef in (h, h2):
+                nat.call("wpc_gram", n, j + 1, nat.ptr(V), n, nat.ptr(w), code, nat.ptr(parts),
+                         nat.ptr(coef), s)
+                nat.call("wpc_basis_combine", n, j + 1, nat.ptr(V), n, nat.ptr(coef), -1.0,
+                         nat.ptr(w), code, s)
+            nat.call("wpc_dot", n, 1, nat.ptr(w), nat.ptr(w), code, nat.ptr(dparts),
+                     nat.ptr(col[-1:]), s)
+            host = col.cpu().numpy()
+            hcol = host[: j + 1] + host[m + 1: m + 2 + j]
+            hn = math.sqrt(max(float(host[-1]), 0.0))
             H[: j + 1, j] = hcol
             H[j + 1, j] = hn
             if hn > 0:
-                V[j + 1].copy_(w / hn)
+                scaled(w, V[j + 1], 1.0 / hn)
             for i in range(j):
                 t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
                 H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
@@ -422,7 +449,9 @@ def gmres(A, b, x, tol=None, atol=None, restart=31, maxiter=0, M=None, callback=
         y = np.zeros(k)
         for i in range(k - 1, -1, -1):
             y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
-        xf.add_(torch.mv(Z[:k].t(), torch.tensor(y, dtype=dt, device=bf.device)))
+        ycoef[:k].copy_(torch.from_numpy(y))
+        nat.call("wpc_basis_combine", n, k, nat.ptr(Z), n, nat.ptr(ycoef), 1.0, nat.ptr(xf), code,
+                 s)
         beta = residual()
         if callback is not None:
             callback(iters, beta, target)

@@ -370,10 +370,6 @@ class Job:
     def enqueue(self) -> bool:
         return not self.finished and self._enqueue()
 
-    def advance(self) -> bool:
-        did = self.poll()
-        return self.enqueue() or did
-
     def remaining(self) -> float:
         """Estimated seconds of GPU work left (scheduling priority)."""
         est = estimated_iterations(self.req.kind, self.system.spec.side)
