@@ -104,27 +104,31 @@ int wpc_csr_stream(int nblocks, const int* blocks, int n, int lanes, const int* 
                    double* partials, double* out, void* stream);
 
 /* Sliced-ELL copy of a scalar CSR matrix for the sliced product mapping: rows in
- * slices of 64, a slice's entries column-major (entry k of row r of slice s at
- * (slice_ptr[s] + k) * 64 + r), rows zero-padded to the slice's longest row on a
- * column they already hold.  Build: wpc_sell_lengths -> slice_ptr[0] = 0,
- * slice_ptr[1..] = inclusive scan of slice_len (wpc_scan_inclusive) ->
- * allocate 64 * slice_ptr[nslices] entries -> wpc_sell_fill -> wpc_sell_values
- * (rerun after the CSR values change).  No reference counterpart: a layout of
- * this library for the products of sparse_operator.py:14-104 / fsai.py:374-381. */
+ * slices of 64, a slice's entries column-major (entry k of the row in position r
+ * of slice s at (slice_ptr[s] + k) * 64 + r), rows zero-padded to the slice's
+ * longest row on a column they already hold.  Position i holds row rows[i] when
+ * `rows` (a permutation of 0..n-1) is given, row i when it is NULL.  Build:
+ * wpc_sell_lengths -> slice_ptr[0] = 0, slice_ptr[1..] = inclusive scan of
+ * slice_len (wpc_scan_inclusive) -> allocate 64 * slice_ptr[nslices] entries ->
+ * wpc_sell_fill -> wpc_sell_values (rerun after the CSR values change).  No
+ * reference counterpart: a layout of this library for the products of
+ * sparse_operator.py:14-104 / fsai.py:374-381. */
 int wpc_sell_slices(int n);
-int wpc_sell_lengths(int n, const int* offsets, int* slice_len, void* stream);
-int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* slice_ptr,
-                  const int* slice_len, int* sell_columns, int* sell_map, void* stream);
+int wpc_sell_lengths(int n, const int* offsets, const int* rows, int* slice_len, void* stream);
+int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* rows,
+                  const int* slice_ptr, const int* slice_len, int* sell_columns, int* sell_map,
+                  void* stream);
 int wpc_sell_values(long count, const int* sell_map, const void* values, int elem_bytes,
                     void* sell_values, void* stream);
 
 /* The product of wpc_csr_stream on the sliced copy: one thread per row, storage
  * order serial sums (bit-identical to the streamed mapping), same epilogue and
- * fused inner product; partials: ceil(n / 256) workgroups. */
-int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* sell_columns,
-                    const void* sell_values, int val_dtype, const void* s, int s_dtype,
-                    const void* d, const void* u, const void* x, const void* y, void* z,
-                    int vec_dtype, int nvec, double alpha, double beta, const void* w,
+ * fused inner product; rows as given to wpc_sell_fill; partials: ceil(n / 256)
+ * workgroups (workgroup b covers positions 256b..256b+255). */
+int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* rows,
+                    const int* sell_columns, const void* sell_values, int val_dtype, const void* s,
+                    int s_dtype, const void* d, const void* u, const void* x, const void* y,
+                    void* z, int vec_dtype, int nvec, double alpha, double beta, const void* w,
                     double* partials, double* out, void* stream);
 
 /* ------------------------------------------------------------ FSAI setup */

@@ -117,11 +117,15 @@ def test_batched_cg_odd_vector_counts(nvec):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_sliced_matches_oracle_bitwise(oracle, dtype):
+@pytest.mark.parametrize("order", ["natural", "by-length"])
+def test_sliced_matches_oracle_bitwise(oracle, monkeypatch, dtype, order):
     # The sliced mapping sums every row serially in storage order, long rows
-    # included: bit-identical to the oracle's serial product.
+    # included: bit-identical to the oracle's serial product, also with the rows
+    # ordered by length inside windows (ragged last window, empty rows).
     from warp_preconditioners.sparse import StreamPlan
 
+    monkeypatch.setattr(StreamPlan, "SORT_GAIN", -1.0 if order == "by-length" else 1e9)
+    monkeypatch.setattr(StreamPlan, "SORT_WINDOW", 200)
     rng = np.random.default_rng(5)
     a = ss.random(3001, 3001, density=0.004, random_state=rng, format="lil")
     a[0, :] = 0
@@ -135,13 +139,24 @@ def test_sliced_matches_oracle_bitwise(oracle, dtype):
     y = rng.normal(size=3001).astype(npd)
     z = torch.empty(3001, dtype=dtype, device=dev())
     plan.run(torch.tensor(x, device=dev()), torch.tensor(y, device=dev()), z, 0.7, -0.3)
+    assert (plan._sell[5] is not None) == (order == "by-length")
     np.testing.assert_array_equal(host(z), oracle.spmv(oracle_csr(oracle, a, npd), x, y, 0.7, -0.3))
+    if order == "by-length":
+        rows = host(plan._sell[5])
+        np.testing.assert_array_equal(np.sort(rows), np.arange(3001))
+        lengths = np.diff(a.indptr)[rows]
+        for w in range(0, 3001, 200):
+            assert np.all(np.diff(lengths[w:w + 200]) <= 0)  # longest first in each window
 
 
 @pytest.mark.parametrize("nvec", [1, 3, 5, 8])
 @pytest.mark.parametrize("vals", [torch.float64, torch.float32, torch.bfloat16])
-def test_sliced_equals_streamed_with_epilogue(nvec, vals):
+@pytest.mark.parametrize("sort", [False, True])
+def test_sliced_equals_streamed_with_epilogue(monkeypatch, nvec, vals, sort):
     from warp_preconditioners.sparse import BsrMatrix, StreamPlan
+
+    monkeypatch.setattr(StreamPlan, "SORT_GAIN", -1.0 if sort else 1e9)
+    monkeypatch.setattr(StreamPlan, "SORT_WINDOW", 128)
 
     rng = np.random.default_rng(nvec)
     n = 2 * 64 * 7 + 13  # ragged last slice

@@ -3,6 +3,9 @@
 // column-major, rows padded to the slice's longest row.  `sell_map` records, for
 // every stored position, the CSR entry it copies (-1 for padding), so a matrix
 // whose values change in place (a refit) refreshes its copy with one gather.
+// An optional row order `rows` (a permutation of 0..n-1) puts row rows[i] in
+// position i of the slices: ordering rows by length inside windows of a few
+// hundred rows keeps the padding small when row lengths vary.
 
 #include "wpc_common.h"
 
@@ -14,11 +17,13 @@ static_assert(kSlice == kWave, "a slice is one wavefront");
 // slice_len[s] = longest row of slice s (one wavefront per slice).
 __global__ __launch_bounds__(256) void sell_lengths_kernel(int n, int nslices,
                                                            const int* __restrict__ off,
+                                                           const int* __restrict__ rows,
                                                            int* __restrict__ slice_len) {
-    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int len = row < n ? off[row + 1] - off[row] : 0;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long row = idx < n ? (rows != nullptr ? rows[idx] : idx) : -1;
+    int len = row >= 0 ? off[row + 1] - off[row] : 0;
     for (int m = kWave / 2; m > 0; m >>= 1) len = max(len, __shfl_xor(len, m, kWave));
-    const long s = row / kSlice;  // a wavefront is exactly one slice (kSlice == kWave)
+    const long s = idx / kSlice;  // a wavefront is exactly one slice (kSlice == kWave)
     if ((threadIdx.x & (kWave - 1)) == 0 && s < nslices) slice_len[s] = len;
 }
 
@@ -26,17 +31,19 @@ __global__ __launch_bounds__(256) void sell_lengths_kernel(int n, int nslices,
 // for an empty row) with map -1.
 __global__ __launch_bounds__(256) void sell_fill_kernel(int n, int nslices, const int* __restrict__ off,
                                                         const int* __restrict__ col,
+                                                        const int* __restrict__ rows,
                                                         const int* __restrict__ slice_ptr,
                                                         const int* __restrict__ slice_len,
                                                         int* __restrict__ scol,
                                                         int* __restrict__ smap) {
-    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= (long)nslices * kSlice) return;
-    const int s = (int)(row / kSlice);
-    const long base = (long)slice_ptr[s] * kSlice + (row % kSlice);
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nslices * kSlice) return;
+    const int s = (int)(idx / kSlice);
+    const long base = (long)slice_ptr[s] * kSlice + (idx % kSlice);
     const int len = slice_len[s];
-    const int a = row < n ? off[row] : 0;
-    const int b = row < n ? off[row + 1] : 0;
+    const long row = idx < n ? (rows != nullptr ? rows[idx] : idx) : -1;
+    const int a = row >= 0 ? off[row] : 0;
+    const int b = row >= 0 ? off[row + 1] : 0;
     const int pad = b > a ? col[b - 1] : 0;
     for (int k = 0; k < len; ++k) {
         const long p = base + (long)k * kSlice;
@@ -65,20 +72,21 @@ extern "C" {
 
 int wpc_sell_slices(int n) { return n > 0 ? (n + kSlice - 1) / kSlice : 0; }
 
-int wpc_sell_lengths(int n, const int* offsets, int* slice_len, void* stream) {
+int wpc_sell_lengths(int n, const int* offsets, const int* rows, int* slice_len, void* stream) {
     const int ns = wpc_sell_slices(n);
     if (ns == 0) return WPC_OK;
     hipLaunchKernelGGL(sell_lengths_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
-                       as_stream(stream), n, ns, offsets, slice_len);
+                       as_stream(stream), n, ns, offsets, rows, slice_len);
     return launch_status();
 }
 
-int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* slice_ptr,
-                  const int* slice_len, int* sell_columns, int* sell_map, void* stream) {
+int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* rows,
+                  const int* slice_ptr, const int* slice_len, int* sell_columns, int* sell_map,
+                  void* stream) {
     const int ns = wpc_sell_slices(n);
     if (ns == 0) return WPC_OK;
     hipLaunchKernelGGL(sell_fill_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
-                       as_stream(stream), n, ns, offsets, columns, slice_ptr, slice_len,
+                       as_stream(stream), n, ns, offsets, columns, rows, slice_ptr, slice_len,
                        sell_columns, sell_map);
     return launch_status();
 }

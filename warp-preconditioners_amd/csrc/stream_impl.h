@@ -30,6 +30,7 @@ struct StreamArgs {
     int nb;
     const int* blocks;  // streamed mapping: row partition; sliced mapping: slice_ptr
     const int* slice_len;  // sliced mapping only
+    const int* rows;       // sliced mapping: row order of the slices (NULL: natural)
     int n;
     int lg;  // log2(lanes per row), -1 for the streamed mapping, -2 for the sliced one
     const int* off;
@@ -235,33 +236,34 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
     }
 }
 
-// Fixed-order fold of per-block partials by one workgroup of 1024 threads.
 // Sliced-ELL mapping of the same product: rows in slices of 64 (one wavefront);
-// a slice's entries are stored column-major -- entry k of the slice's row r sits at
-// (slice_ptr[s] + k) * 64 + r -- and every row is padded to the slice's longest
-// row with zero values on a column the row already holds.  One thread per row
-// reads fully coalesced index/value streams, issues its gathers four entries at a
-// time, and sums in storage order, so results equal the streamed mapping's bit for
-// bit (padding adds exact zeros).  Same epilogue and fused inner product.
+// a slice's entries are stored column-major -- entry k of the row in position r of
+// the slice sits at (slice_ptr[s] + k) * 64 + r -- and every row is padded to the
+// slice's longest row with zero values on a column the row already holds.
+// Position i holds row rows[i] (rows == NULL: row i).  One thread per row reads
+// fully coalesced index/value streams, issues its gathers four entries at a time,
+// and sums in storage order, so results equal the streamed mapping's bit for bit
+// (padding adds exact zeros).  Same epilogue and fused inner product.
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
-    const int* __restrict__ col, const TV* __restrict__ val, const TS* __restrict__ s,
-    const TX* __restrict__ d, const TX* __restrict__ u, const TX* __restrict__ x, const TX* y,
-    TX* z, typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta,
-    const TX* __restrict__ w, double* __restrict__ partials) {
+    const int* __restrict__ rows, const int* __restrict__ col, const TV* __restrict__ val,
+    const TS* __restrict__ s, const TX* __restrict__ d, const TX* __restrict__ u,
+    const TX* __restrict__ x, const TX* y, TX* z, typename AccOf<TX>::type alpha,
+    typename AccOf<TX>::type beta, const TX* __restrict__ w, double* __restrict__ partials) {
     using TA = typename AccOf<TX>::type;
     __shared__ double red[kStreamRows];
     const int blk = xcd_block();
-    const int row = blk * kStreamRows + threadIdx.x;
-    const bool live = row < n;
+    const int idx = blk * kStreamRows + threadIdx.x;
+    const bool live = idx < n;
+    const int row = live && rows != nullptr ? rows[idx] : idx;
     const bool use_a = alpha != TA(0);
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
     if (live && use_a) {
-        const int slice = row >> 6;
-        const long base = (long)slice_ptr[slice] * 64 + (row & 63);
+        const int slice = idx >> 6;
+        const long base = (long)slice_ptr[slice] * 64 + (idx & 63);
         const int len = slice_len[slice];
         int k = 0;
         for (; k + 4 <= len; k += 4) {
@@ -313,6 +315,7 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     }
 }
 
+// Fixed-order fold of per-block partials by one workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
                                                   const double* __restrict__ partials,
                                                   double* __restrict__ out) {
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
 }
 
 template <class TV, class TX, class TS, int NV>
-int stream_launch(int nb, const int* blocks, const int* slice_len, int n, int lg, const int* off,
+int stream_launch(int nb, const int* blocks, const int* slice_len, const int* rows, int n, int lg,
+                  const int* off,
                   const int* col, const void* val, const void* s, const void* d, const void* u,
                   const void* x, const void* y, void* z, double alpha, double beta, const void* w,
                   double* partials, double* out, hipStream_t st) {
@@ -347,7 +351,7 @@ int stream_launch(int nb, const int* blocks, const int* slice_len, int n, int lg
     do {                                                                                        \
         if (lg == -2)                                                                           \
             hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(grid),           \
-                               dim3(kStreamRows), 0, st, n, blocks, slice_len, col,             \
+                               dim3(kStreamRows), 0, st, n, blocks, slice_len, rows, col,       \
                                (const TV*)val, (const TS*)s, (const TX*)d, (const TX*)u,        \
                                (const TX*)x, (const TX*)y, (TX*)z, (TA)alpha, (TA)beta,         \
                                (const TX*)w, partials);                                         \
@@ -386,12 +390,14 @@ int stream_for_nv(const StreamArgs& a) {
 #define ST(TV, TX)                                                                              \
     do {                                                                                        \
         if (a.s32)                                                                              \
-            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.n, a.lg,     \
+            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.rows, a.n,  \
+                                                     a.lg,                                       \
                                                      a.off, a.col,                               \
                                                      a.val, a.s, a.d, a.u, a.x, a.y, a.z,        \
                                                      a.alpha, a.beta, a.w, a.partials, a.out,    \
                                                      a.st);                                      \
-        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.n, a.lg, a.off,      \
+        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.rows, a.n, a.lg,     \
+                                             a.off,                                              \
                                              a.col, a.val,                                       \
                                              a.s, a.d, a.u, a.x, a.y, a.z, a.alpha, a.beta, a.w, \
                                              a.partials, a.out, a.st);                           \
