@@ -561,7 +561,7 @@ def serving_bench(args):
         makespan = max(r.finished for r in out)
         return eng, out, makespan
 
-    eng, out, makespan = run(ServingConfig(keep_solutions=False), "batched")
+    eng, out, makespan = run(ServingConfig(keep_solutions=False, group=args.group), "batched")
     lat = np.array([r.latency for r in out])
     rhs = sum(r.nrhs for r in out)
     bounds = [b for r in out for b in r.field_bound]
@@ -599,6 +599,9 @@ def serving_bench(args):
         "factors": {k: eng.stats[k] for k in ("built", "refit", "cached", "evicted")},
         "setup_s": eng.stats["setup_s"],
         "peak_hbm_bytes": eng.stats["peak_bytes"],
+        "grouped": {"enabled": args.group, "chunks": eng.stats["grouped_chunks"],
+                    "mean_batch": eng.stats["grouped_jobs"] / max(eng.stats["grouped_chunks"], 1),
+                    "capture_failures": sum(g.capture_failures for g, _ in eng.groups.values())},
     }
     if args.compare:
         cfg = ServingConfig(keep_solutions=False, split_rhs=True, max_active=1, streams=1)
@@ -637,6 +640,8 @@ def main():
                          "elasticity: the block-coupled and elasticity solver comparisons")
     ap.add_argument("--requests", type=int, default=512, help="serving: stream length")
     ap.add_argument("--max-side", type=int, default=96, help="serving: largest mesh side")
+    ap.add_argument("--group", action="store_true",
+                    help="serving: advance same-shape small solves with one launch per pass")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="time only the per-iteration kernels (and a 256 MiB copy that "
                          "calibrates HBM byte counters); for counter runs under rocprofv3")

@@ -274,13 +274,18 @@ def _field_error(x, xs, m):
     return np.sqrt(np.sum(m[:, None] * (x - xs) ** 2, 0) / np.sum(m[:, None] * xs ** 2, 0))
 
 
-def test_serving_results_certified_and_mix_invariant():
+@pytest.mark.parametrize("group", [False, True])
+def test_serving_results_certified_and_mix_invariant(group):
+    # With group=True same-shape solves in flight advance in batched launches
+    # (grouped.GroupedCG); results stay bit-identical to solving alone.
     from warp_preconditioners.serving import ServingConfig, ServingEngine, request_stream
 
     reqs = request_stream(count=14, rate=2000.0, seed=7, sides=(6, 13), repeat=0.5, max_rhs=4)
     assert {r.kind for r in reqs} == {"laplacian", "biharmonic"}
-    cfg = ServingConfig(streams=3, max_active=8, chunk=10)
-    mix = ServingEngine(cfg).serve(reqs, realtime=True)
+    cfg = ServingConfig(streams=3, max_active=8, chunk=10, group=group)
+    eng = ServingEngine(cfg)
+    mix = eng.serve(reqs, realtime=True)
+    assert (eng.stats["grouped_chunks"] > 0) == group
     provenance = {r.factor for r in mix}
     assert "cached" in provenance and "built" in provenance
     for req, res in zip(reqs, mix):

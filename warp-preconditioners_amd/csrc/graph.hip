@@ -19,7 +19,10 @@ int wpc_graph_begin(void* stream) {
 int wpc_graph_end(void* stream, void** graph_exec) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(as_stream(stream), &g);
-    if (e != hipSuccess) return (int)e;
+    if (e != hipSuccess) {
+        if (g != nullptr) hipGraphDestroy(g);
+        return (int)e;
+    }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);

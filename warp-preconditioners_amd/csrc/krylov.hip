@@ -49,10 +49,14 @@ __device__ void block_partial(const double* acc, double* red, double* partials) 
     }
 }
 
-__device__ __forceinline__ void chunk_of(int n, long* begin, long* end) {
-    const long chunk = ((long)n + gridDim.x - 1) / gridDim.x;
-    *begin = (long)blockIdx.x * chunk;
+__device__ __forceinline__ void chunk_of(int n, int blk, int nblocks, long* begin, long* end) {
+    const long chunk = ((long)n + nblocks - 1) / nblocks;
+    *begin = (long)blk * chunk;
     *end = *begin + chunk < n ? *begin + chunk : n;
+}
+
+__device__ __forceinline__ void chunk_of(int n, long* begin, long* end) {
+    chunk_of(n, blockIdx.x, gridDim.x, begin, end);
 }
 
 template <class T, int NV>
@@ -262,15 +266,13 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
 template <int NV> constexpr int kPrefetch = NV <= 2 ? 4 : (NV <= 4 ? 2 : 1);
 
 template <class T, int NV>
-__global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
-    int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+__device__ __forceinline__ void cg_update_fold_body(
+    int blk, int nblocks, int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, double* red, double* sums) {
     constexpr int PF = kPrefetch<NV>;
-    __shared__ double red[kKrylovThreads];
-    __shared__ double sums[8];
     long begin, end;
-    chunk_of(n, &begin, &end);
+    chunk_of(n, blk, nblocks, &begin, &end);
     T xv[PF][NV], rv[PF][NV], pv[PF][NV], av[PF][NV];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
         const bool active = state[WPC_KS_RR * NV + v] > state[WPC_KS_TARGET * NV + v];
         const double a = active ? state[WPC_KS_RZ * NV + v] / denom : 0.0;
         alpha[v] = (T)a;
-        if (blockIdx.x == 0 && threadIdx.x == v) {
+        if (blk == 0 && threadIdx.x == v) {
             state[WPC_KS_RZ_OLD * NV + v] = state[WPC_KS_RZ * NV + v];
             state[WPC_KS_DENOM * NV + v] = denom;
             state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
@@ -335,17 +337,28 @@ __global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
         }
     }
     block_sum<NV>(acc, red);
-    write_partials<NV>(acc, partials);
+    write_partials<NV>(acc, partials, blk);
 }
 
 template <class T, int NV>
-__global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
-    int n, const T* __restrict__ z, T* __restrict__ p, int P_rz, const double* __restrict__ part_rz,
-    int P_rr, const double* __restrict__ part_rr, double* state) {
+__global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
+    int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+    const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
+    double* __restrict__ partials) {
+    __shared__ double red[kKrylovThreads];
+    __shared__ double sums[8];
+    cg_update_fold_body<T, NV>(blockIdx.x, gridDim.x, n, x, r, p, ap, P, part_denom, state,
+                               partials, red, sums);
+}
+
+template <class T, int NV>
+__device__ __forceinline__ void cg_direction_fold_body(
+    int blk, int nblocks, int n, const T* __restrict__ z, T* __restrict__ p, int P_rz,
+    const double* __restrict__ part_rz, int P_rr, const double* __restrict__ part_rr,
+    double* state, double* s_rz, double* s_rr) {
     constexpr int PF = kPrefetch<NV>;
-    __shared__ double s_rz[8], s_rr[8];
     long begin, end;
-    chunk_of(n, &begin, &end);
+    chunk_of(n, blk, nblocks, &begin, &end);
     T zv[PF][NV], pv[PF][NV];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -367,7 +380,7 @@ __global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
         act[v] = state[WPC_KS_ACTIVE * NV + v] != 0.0;
         const double b = act[v] ? s_rz[v] / state[WPC_KS_RZ_OLD * NV + v] : 0.0;
         beta[v] = (T)b;
-        if (blockIdx.x == 0 && threadIdx.x == v) {
+        if (blk == 0 && threadIdx.x == v) {
             state[WPC_KS_RZ_NEW * NV + v] = s_rz[v];
             state[WPC_KS_RR_NEW * NV + v] = s_rr[v];
             state[WPC_KS_BETA * NV + v] = b;
@@ -395,6 +408,82 @@ __global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
             p[k] = z[k] + beta[v] * p[k];
         }
     }
+}
+
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
+    int n, const T* __restrict__ z, T* __restrict__ p, int P_rz, const double* __restrict__ part_rz,
+    int P_rr, const double* __restrict__ part_rr, double* state) {
+    __shared__ double s_rz[8], s_rr[8];
+    cg_direction_fold_body<T, NV>(blockIdx.x, gridDim.x, n, z, p, P_rz, part_rz, P_rr, part_rr,
+                                  state, s_rz, s_rr);
+}
+
+// One CG vector step of a grouped launch (all fields 64-bit, in this order):
+// vectors, the partials the step folds (a: p.Ap or r.z; b: r.r), the state, the
+// r.r partials it leaves (update), the system's size, its workgroup count and
+// first global workgroup, and the folded partial counts.
+struct CgTask {
+    void* x;
+    void* r;
+    void* p;
+    const void* ap;
+    const void* z;
+    const double* part_a;
+    const double* part_b;
+    double* state;
+    double* partials;
+    long n;
+    long nblocks;
+    long block0;
+    long pa;
+    long pb;
+};
+static_assert(sizeof(CgTask) == 14 * 8, "CgTask is 14 64-bit words");
+
+__device__ __forceinline__ int task_of(const CgTask* __restrict__ tasks, int ntasks, int g,
+                                       int* s_task) {
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (tasks[mid].block0 <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        *s_task = lo;
+    }
+    __syncthreads();
+    return *s_task;
+}
+
+// The fused update / direction steps of several systems in one launch: each
+// workgroup runs its task's local block, folding that task's partials, exactly as
+// the system's own launch would.
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void grouped_update_fold_kernel(
+    const CgTask* __restrict__ tasks, int ntasks) {
+    __shared__ double red[kKrylovThreads];
+    __shared__ double sums[8];
+    __shared__ int s_task;
+    const int g = blockIdx.x;
+    const CgTask& t = tasks[task_of(tasks, ntasks, g, &s_task)];
+    if (t.n == 0) return;  // capacity padding
+    cg_update_fold_body<T, NV>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (T*)t.x, (T*)t.r,
+                               (const T*)t.p, (const T*)t.ap, (int)t.pa, t.part_a, t.state,
+                               t.partials, red, sums);
+}
+
+template <class T, int NV>
+__global__ __launch_bounds__(kKrylovThreads) void grouped_direction_fold_kernel(
+    const CgTask* __restrict__ tasks, int ntasks) {
+    __shared__ double s_rz[8], s_rr[8];
+    __shared__ int s_task;
+    const int g = blockIdx.x;
+    const CgTask& t = tasks[task_of(tasks, ntasks, g, &s_task)];
+    if (t.n == 0) return;  // capacity padding
+    cg_direction_fold_body<T, NV>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (const T*)t.z,
+                                  (T*)t.p, (int)t.pa, t.part_a, (int)t.pb, t.part_b, t.state,
+                                  s_rz, s_rr);
 }
 
 // p = z + BETA p for slots that stepped.
@@ -538,6 +627,20 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, in
     const int P = krylov_blocks(n);
     hipStream_t s = as_stream(stream);
     VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_fold_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, nparts_rz, part_rz, nparts_rr, part_rr, state)))
+    return launch_status();
+}
+
+int wpc_cg_fold_grouped(int step, int ntasks, const void* tasks, int total_blocks, int dtype,
+                        int nvec, void* stream) {
+    if (ntasks <= 0) return WPC_OK;
+    if (tasks == nullptr || total_blocks <= 0 || (step != 1 && step != 2)) return WPC_EARG;
+    hipStream_t s = as_stream(stream);
+    const CgTask* t = (const CgTask*)tasks;
+    if (step == 1) {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((grouped_update_fold_kernel<T, NV>), dim3(total_blocks), dim3(kKrylovThreads), 0, s, t, ntasks)))
+    } else {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((grouped_direction_fold_kernel<T, NV>), dim3(total_blocks), dim3(kKrylovThreads), 0, s, t, ntasks)))
+    }
     return launch_status();
 }
 

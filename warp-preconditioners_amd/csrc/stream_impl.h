@@ -62,6 +62,50 @@ int stream_entry_6(const StreamArgs& a);
 int stream_entry_7(const StreamArgs& a);
 int stream_entry_8(const StreamArgs& a);
 
+// One product of a grouped launch (all fields 64-bit, in this order; the host
+// builds the table as int64 words): the sliced copy, the epilogue operands, the
+// vectors, the partials and the task's row count and first global workgroup.
+struct SellTask {
+    const int* slice_ptr;
+    const int* slice_len;
+    const int* rows;
+    const int* col;
+    const void* val;
+    const void* s;
+    const void* d;
+    const void* u;
+    const void* x;
+    void* z;
+    const void* w;
+    double* partials;
+    long n;
+    long block0;
+};
+static_assert(sizeof(SellTask) == 14 * 8, "SellTask is 14 64-bit words");
+
+struct GroupedArgs {
+    int ntasks;
+    const SellTask* tasks;
+    int total_blocks;
+    int val_dtype;
+    bool hs;
+    bool s32;
+    bool hd;
+    bool dot;
+    int vec_dtype;
+    double alpha;
+    hipStream_t st;
+};
+
+int stream_grouped_entry_1(const GroupedArgs& a);
+int stream_grouped_entry_2(const GroupedArgs& a);
+int stream_grouped_entry_3(const GroupedArgs& a);
+int stream_grouped_entry_4(const GroupedArgs& a);
+int stream_grouped_entry_5(const GroupedArgs& a);
+int stream_grouped_entry_6(const GroupedArgs& a);
+int stream_grouped_entry_7(const GroupedArgs& a);
+int stream_grouped_entry_8(const GroupedArgs& a);
+
 }  // namespace wpc
 
 #ifdef WPC_STREAM_NV
@@ -245,15 +289,14 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 // and sums in storage order, so results equal the streamed mapping's bit for bit
 // (padding adds exact zeros).  Same epilogue and fused inner product.
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
-__global__ __launch_bounds__(kStreamRows) void sell_kernel(
-    int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
+__device__ __forceinline__ void sell_body(
+    int blk, int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ rows, const int* __restrict__ col, const TV* __restrict__ val,
     const TS* __restrict__ s, const TX* __restrict__ d, const TX* __restrict__ u,
     const TX* __restrict__ x, const TX* y, TX* z, typename AccOf<TX>::type alpha,
-    typename AccOf<TX>::type beta, const TX* __restrict__ w, double* __restrict__ partials) {
+    typename AccOf<TX>::type beta, const TX* __restrict__ w, double* __restrict__ partials,
+    double* red) {
     using TA = typename AccOf<TX>::type;
-    __shared__ double red[kStreamRows];
-    const int blk = xcd_block();
     const int idx = blk * kStreamRows + threadIdx.x;
     const bool live = idx < n;
     const int row = live && rows != nullptr ? rows[idx] : idx;
@@ -313,6 +356,89 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
         block_sum<NV>(dot, red);
         write_partials<NV>(dot, partials, blk);
     }
+}
+
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+__global__ __launch_bounds__(kStreamRows) void sell_kernel(
+    int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
+    const int* __restrict__ rows, const int* __restrict__ col, const TV* __restrict__ val,
+    const TS* __restrict__ s, const TX* __restrict__ d, const TX* __restrict__ u,
+    const TX* __restrict__ x, const TX* y, TX* z, typename AccOf<TX>::type alpha,
+    typename AccOf<TX>::type beta, const TX* __restrict__ w, double* __restrict__ partials) {
+    __shared__ double red[kStreamRows];
+    sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, rows, col, val, s,
+                                           d, u, x, y, z, alpha, beta, w, partials, red);
+}
+
+// The same product on several systems in one launch: workgroup g (after the XCD
+// renumbering) belongs to the task with the largest block0 <= g and runs that
+// task's local block g - block0, so every system computes exactly what its own
+// launch would (same rows per workgroup, same partials), bit for bit.  beta = 0.
+// Tasks with n == 0 pad a captured launch to its capacity and do nothing.
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+__global__ __launch_bounds__(kStreamRows) void grouped_sell_kernel(
+    const SellTask* __restrict__ tasks, int ntasks, typename AccOf<TX>::type alpha) {
+    using TA = typename AccOf<TX>::type;
+    __shared__ double red[kStreamRows];
+    __shared__ int s_task;
+    const int g = xcd_block();
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (tasks[mid].block0 <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        s_task = lo;
+    }
+    __syncthreads();
+    const SellTask& t = tasks[s_task];
+    if (t.n == 0) return;  // capacity padding of a captured launch
+    const int blk = g - (int)t.block0;
+    sell_body<TV, TX, TS, NV, HS, HD, DOT>(
+        blk, (int)t.n, t.slice_ptr, t.slice_len, t.rows, t.col, (const TV*)t.val, (const TS*)t.s,
+        (const TX*)t.d, (const TX*)t.u, (const TX*)t.x, nullptr, (TX*)t.z, alpha, TA(0),
+        (const TX*)t.w, t.partials, red);
+}
+
+template <class TV, class TX, class TS, int NV>
+int grouped_launch(const GroupedArgs& a) {
+    using TA = typename AccOf<TX>::type;
+    if (a.ntasks <= 0 || a.total_blocks <= 0) return WPC_OK;
+#define GL(HS, HD, DT)                                                                          \
+    hipLaunchKernelGGL((grouped_sell_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(a.total_blocks), \
+                       dim3(kStreamRows), 0, a.st, a.tasks, a.ntasks, (TA)a.alpha)
+    if (!a.hs && !a.hd && !a.dot) GL(false, false, false);
+    else if (a.hs && !a.hd && !a.dot) GL(true, false, false);
+    else if (!a.hs && a.hd && !a.dot) GL(false, true, false);
+    else if (a.hs && a.hd && !a.dot) GL(true, true, false);
+    else if (!a.hs && !a.hd && a.dot) GL(false, false, true);
+    else if (a.hs && !a.hd && a.dot) GL(true, false, true);
+    else if (!a.hs && a.hd && a.dot) GL(false, true, true);
+    else GL(true, true, true);
+#undef GL
+    return launch_status();
+}
+
+template <int NV>
+int grouped_for_nv(const GroupedArgs& a) {
+#define GT(TV, TX)                                                                              \
+    do {                                                                                        \
+        if (a.hs && a.s32) return grouped_launch<TV, TX, float, NV>(a);                         \
+        return grouped_launch<TV, TX, TX, NV>(a);                                               \
+    } while (0)
+    if (a.vec_dtype == WPC_F64) {
+        if (a.val_dtype == WPC_F64) GT(double, double);
+        if (a.val_dtype == WPC_F32) GT(float, double);
+        if (a.val_dtype == WPC_F16) GT(f16s, double);
+        if (a.val_dtype == WPC_BF16) GT(bf16s, double);
+    } else if (a.vec_dtype == WPC_F32) {
+        if (a.val_dtype == WPC_F32) GT(float, float);
+        if (a.val_dtype == WPC_F16) GT(f16s, float);
+        if (a.val_dtype == WPC_BF16) GT(bf16s, float);
+    }
+#undef GT
+    return WPC_EDTYPE;
 }
 
 // Fixed-order fold of per-block partials by one workgroup of 1024 threads.
@@ -422,6 +548,9 @@ int stream_for_nv(const StreamArgs& a) {
 #define WPC_STREAM_CAT(a, b) WPC_STREAM_CAT2(a, b)
 int WPC_STREAM_CAT(stream_entry_, WPC_STREAM_NV)(const StreamArgs& a) {
     return stream_for_nv<WPC_STREAM_NV>(a);
+}
+int WPC_STREAM_CAT(stream_grouped_entry_, WPC_STREAM_NV)(const GroupedArgs& a) {
+    return grouped_for_nv<WPC_STREAM_NV>(a);
 }
 
 }  // namespace wpc
