@@ -120,16 +120,26 @@ int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* rows
                   void* stream);
 int wpc_sell_values(long count, const int* sell_map, const void* values, int elem_bytes,
                     void* sell_values, void* stream);
+/* int16 copy of the sliced columns as differences col - row (the row of each
+ * position, through `rows`); padding repeats the row's last difference, an empty
+ * row points at itself.  overflow[0] (zeroed by the caller) counts entries whose
+ * difference is outside [-32768, 32767] (and empty rows >= ncol), in which case
+ * the int32 columns must be used.  Halves the index stream of banded
+ * (mesh-ordered) matrices. */
+int wpc_sell_col16(int n, int ncol, const int* offsets, const int* rows, const int* slice_ptr,
+                   const int* slice_len, const int* sell_columns, short* sell_col16, int* overflow,
+                   void* stream);
 
 /* The product of wpc_csr_stream on the sliced copy: one thread per row, storage
  * order serial sums (bit-identical to the streamed mapping), same epilogue and
- * fused inner product; rows as given to wpc_sell_fill; partials: ceil(n / 256)
- * workgroups (workgroup b covers positions 256b..256b+255). */
+ * fused inner product; rows as given to wpc_sell_fill; sell_columns are the int32
+ * columns (col16 = 0) or the int16 differences of wpc_sell_col16 (col16 = 1);
+ * partials: ceil(n / 256) workgroups (workgroup b covers positions 256b..+255). */
 int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* rows,
-                    const int* sell_columns, const void* sell_values, int val_dtype, const void* s,
-                    int s_dtype, const void* d, const void* u, const void* x, const void* y,
-                    void* z, int vec_dtype, int nvec, double alpha, double beta, const void* w,
-                    double* partials, double* out, void* stream);
+                    const void* sell_columns, int col16, const void* sell_values, int val_dtype,
+                    const void* s, int s_dtype, const void* d, const void* u, const void* x,
+                    const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
+                    const void* w, double* partials, double* out, void* stream);
 
 /* ------------------------------------------------------------ FSAI setup */
 
@@ -445,19 +455,19 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, in
  * No reference counterpart: a serving extension that advances the fused CG of
  * several systems (uneven sizes, same vector format and count, same product
  * flags) with one launch per pass instead of one per system.  `tasks` is a device
- * table of 14 int64 words per system; systems own consecutive global workgroups
+ * table of 15 int64 words per system; systems own consecutive global workgroups
  * starting at their block0 (ascending, the first 0), total_blocks in all; tasks
  * with n = 0 (block0 = the end of the real ones) pad a launch captured for a
  * larger group and do nothing.  Each system computes exactly what its own
  * launch computes, bit for bit.
  *
  * wpc_sell_stream_grouped: per task {slice_ptr, slice_len, rows, sell_columns,
- *   sell_values, s, d, u, x, z, w, partials, n, block0} -- the wpc_sell_stream
+ *   sell_values, s, d, u, x, z, w, partials, n, block0, col16} -- the wpc_sell_stream
  *   product with beta = 0 on each system (ceil(n/256) workgroups); s_dtype 0 means
  *   no row scale, has_d selects the d .* u term, dot the w . z partials.
  * wpc_cg_fold_grouped: step 1 = wpc_cg_update_fold, step 2 = wpc_cg_direction_fold
  *   per task {x, r, p, ap, z, part_a, part_b, state, partials, n, nblocks, block0,
- *   pa, pb} with nblocks = wpc_krylov_partials(n); update folds pa partials of
+ *   pa, pb, 0} with nblocks = wpc_krylov_partials(n); update folds pa partials of
  *   p.Ap from part_a and leaves r.r partials in `partials`; direction folds pa
  *   r.z partials (part_a) and pb r.r partials (part_b). */
 int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int val_dtype,

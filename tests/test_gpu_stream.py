@@ -211,3 +211,32 @@ def test_sliced_copy_follows_refit():
     zr = torch.empty_like(x)
     ref.apply_batch(x, x, zr, 1.0, 0.0, 1)
     np.testing.assert_array_equal(host(z), host(zr))
+
+
+@pytest.mark.parametrize("far", [False, True])
+def test_sliced_int16_columns_and_fallback(oracle, far):
+    # Column differences within int16 are stored as such (a banded matrix); one
+    # entry 69,999 columns from its row keeps the int32 columns.  Both give the
+    # oracle's product bit for bit.
+    from warp_preconditioners.sparse import StreamPlan
+
+    n = 70000
+    rng = np.random.default_rng(21)
+    off = rng.integers(-300, 300, size=(n, 5))
+    rows = np.repeat(np.arange(n), 5)
+    cols = np.clip(rows + off.ravel(), 0, n - 1)
+    a = ss.csr_matrix((rng.normal(size=rows.size), (rows, cols)), shape=(n, n))
+    a[17, :] = 0  # an empty row
+    if far:
+        a[3, n - 1] = 2.5
+    a = a.tocsr()
+    a.eliminate_zeros()
+    A = upload(a)
+    plan = StreamPlan(A, lanes=StreamPlan.SLICED)
+    x = rng.normal(size=(n, 2))
+    z = torch.empty(n * 2, dtype=torch.float64, device=dev())
+    plan.run(torch.tensor(x.ravel(), device=dev()), None, z, 1.0, 0.0, 2)
+    assert plan._sell[6] == (not far)
+    ref = np.stack([oracle.spmv(oracle_csr(oracle, a), np.ascontiguousarray(x[:, v]))
+                    for v in range(2)], axis=1)
+    np.testing.assert_array_equal(host(z).reshape(n, 2), ref)

@@ -438,8 +438,9 @@ struct CgTask {
     long block0;
     long pa;
     long pb;
+    long unused;  // tables of both kinds are 15 words wide
 };
-static_assert(sizeof(CgTask) == 14 * 8, "CgTask is 14 64-bit words");
+static_assert(sizeof(CgTask) == 15 * 8, "CgTask is 15 64-bit words");
 
 __device__ __forceinline__ int task_of(const CgTask* __restrict__ tasks, int ntasks, int g,
                                        int* s_task) {

@@ -53,6 +53,43 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(int n, int nslices, cons
     }
 }
 
+// int16 copy of the columns as differences col - row; overflow[0] counts entries
+// whose difference does not fit (the int32 copy must then be used).  Padding
+// repeats the row's last difference; an empty row pads with its own index (a
+// valid column of a square matrix, row < ncol).
+__global__ __launch_bounds__(256) void sell_col16_kernel(int n, int ncol, int nslices,
+                                                         const int* __restrict__ off,
+                                                         const int* __restrict__ rows,
+                                                         const int* __restrict__ slice_ptr,
+                                                         const int* __restrict__ slice_len,
+                                                         const int* __restrict__ scol,
+                                                         short* __restrict__ out, int* overflow) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nslices * kSlice) return;
+    const int s = (int)(idx / kSlice);
+    const long base = (long)slice_ptr[s] * kSlice + (idx % kSlice);
+    const int len = slice_len[s];
+    const long row = idx < n ? (rows != nullptr ? rows[idx] : idx) : -1;
+    const int own = row >= 0 ? off[row + 1] - off[row] : 0;
+    int bad = 0;
+    long last = 0;
+    if (own == 0 && row >= ncol) bad = 1;  // no column of its own to point at
+    for (int k = 0; k < len; ++k) {
+        const long p = base + (long)k * kSlice;
+        long diff = last;
+        if (k < own) {
+            diff = (long)scol[p] - (row >= 0 ? row : 0);
+            if (diff < -32768 || diff > 32767) {
+                ++bad;
+                diff = 0;
+            }
+            last = diff;
+        }
+        out[p] = (short)diff;
+    }
+    if (bad) atomicAdd(overflow, bad);
+}
+
 template <class E>
 __global__ __launch_bounds__(256) void sell_values_kernel(long count, const int* __restrict__ smap,
                                                           const E* __restrict__ src,
@@ -88,6 +125,17 @@ int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* rows
     hipLaunchKernelGGL(sell_fill_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
                        as_stream(stream), n, ns, offsets, columns, rows, slice_ptr, slice_len,
                        sell_columns, sell_map);
+    return launch_status();
+}
+
+int wpc_sell_col16(int n, int ncol, const int* offsets, const int* rows, const int* slice_ptr,
+                   const int* slice_len, const int* sell_columns, short* sell_col16, int* overflow,
+                   void* stream) {
+    const int ns = wpc_sell_slices(n);
+    if (ns == 0) return WPC_OK;
+    hipLaunchKernelGGL(sell_col16_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
+                       as_stream(stream), n, ncol, ns, offsets, rows, slice_ptr, slice_len,
+                       sell_columns, sell_col16, overflow);
     return launch_status();
 }
 

@@ -31,6 +31,7 @@ struct StreamArgs {
     const int* blocks;  // streamed mapping: row partition; sliced mapping: slice_ptr
     const int* slice_len;  // sliced mapping only
     const int* rows;       // sliced mapping: row order of the slices (NULL: natural)
+    bool col16;            // sliced mapping: int16 column differences
     int n;
     int lg;  // log2(lanes per row), -1 for the streamed mapping, -2 for the sliced one
     const int* off;
@@ -69,7 +70,7 @@ struct SellTask {
     const int* slice_ptr;
     const int* slice_len;
     const int* rows;
-    const int* col;
+    const void* col;
     const void* val;
     const void* s;
     const void* d;
@@ -80,8 +81,9 @@ struct SellTask {
     double* partials;
     long n;
     long block0;
+    long col16;  // columns stored as int16 differences from the row
 };
-static_assert(sizeof(SellTask) == 14 * 8, "SellTask is 14 64-bit words");
+static_assert(sizeof(SellTask) == 15 * 8, "SellTask is 15 64-bit words");
 
 struct GroupedArgs {
     int ntasks;
@@ -288,14 +290,56 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 // fully coalesced index/value streams, issues its gathers four entries at a time,
 // and sums in storage order, so results equal the streamed mapping's bit for bit
 // (padding adds exact zeros).  Same epilogue and fused inner product.
+// Column index of entry p of the row `row`: stored as int32, or as the int16
+// difference col - row when every entry of the matrix is within 32767 of its row
+// (banded meshes; 2 bytes per entry less to stream).
+template <class TI> __device__ __forceinline__ long sell_col(const TI* col, long p, int row);
+template <> __device__ __forceinline__ long sell_col<int>(const int* col, long p, int) {
+    return col[p];
+}
+template <> __device__ __forceinline__ long sell_col<short>(const short* col, long p, int row) {
+    return (long)row + col[p];
+}
+
+template <class TI, class TV, class TX, int NV>
+__device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long base, int len,
+                                           int row, const TI* __restrict__ col,
+                                           const TV* __restrict__ val, const TX* __restrict__ x) {
+    using TA = typename AccOf<TX>::type;
+    int k = 0;
+    for (; k + 4 <= len; k += 4) {
+        long c[4];
+        TA a[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long p = base + (long)(k + j) * 64;
+            c[j] = sell_col<TI>(col, p, row);
+            a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * NV + v);
+        }
+    }
+    for (; k < len; ++k) {
+        const long p = base + (long)k * 64;
+        const long c = sell_col<TI>(col, p, row);
+        const TA a = lds_val<TA, TV>(raw_val<TV>(val, p));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += a * St<TX>::template ld<TA>(x, c * NV + v);
+    }
+}
+
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __device__ __forceinline__ void sell_body(
     int blk, int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
-    const int* __restrict__ rows, const int* __restrict__ col, const TV* __restrict__ val,
-    const TS* __restrict__ s, const TX* __restrict__ d, const TX* __restrict__ u,
-    const TX* __restrict__ x, const TX* y, TX* z, typename AccOf<TX>::type alpha,
-    typename AccOf<TX>::type beta, const TX* __restrict__ w, double* __restrict__ partials,
-    double* red) {
+    const int* __restrict__ rows, const void* __restrict__ col, bool col16,
+    const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
+    const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
+    typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
+    double* __restrict__ partials, double* red) {
     using TA = typename AccOf<TX>::type;
     const int idx = blk * kStreamRows + threadIdx.x;
     const bool live = idx < n;
@@ -308,30 +352,10 @@ __device__ __forceinline__ void sell_body(
         const int slice = idx >> 6;
         const long base = (long)slice_ptr[slice] * 64 + (idx & 63);
         const int len = slice_len[slice];
-        int k = 0;
-        for (; k + 4 <= len; k += 4) {
-            long c[4];
-            TA a[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const long p = base + (long)(k + j) * 64;
-                c[j] = col[p];
-                a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * NV + v);
-            }
-        }
-        for (; k < len; ++k) {
-            const long p = base + (long)k * 64;
-            const long c = col[p];
-            const TA a = lds_val<TA, TV>(raw_val<TV>(val, p));
-#pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] += a * St<TX>::template ld<TA>(x, c * NV + v);
-        }
+        if (col16)
+            sell_accum<short, TV, TX, NV>(acc, base, len, row, (const short*)col, val, x);
+        else
+            sell_accum<int, TV, TX, NV>(acc, base, len, row, (const int*)col, val, x);
     }
     double dot[NV];
 #pragma unroll
@@ -361,13 +385,15 @@ __device__ __forceinline__ void sell_body(
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
-    const int* __restrict__ rows, const int* __restrict__ col, const TV* __restrict__ val,
-    const TS* __restrict__ s, const TX* __restrict__ d, const TX* __restrict__ u,
-    const TX* __restrict__ x, const TX* y, TX* z, typename AccOf<TX>::type alpha,
-    typename AccOf<TX>::type beta, const TX* __restrict__ w, double* __restrict__ partials) {
+    const int* __restrict__ rows, const void* __restrict__ col, int col16,
+    const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
+    const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
+    typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
+    double* __restrict__ partials) {
     __shared__ double red[kStreamRows];
-    sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, rows, col, val, s,
-                                           d, u, x, y, z, alpha, beta, w, partials, red);
+    sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, rows, col,
+                                           col16 != 0, val, s, d, u, x, y, z, alpha, beta, w,
+                                           partials, red);
 }
 
 // The same product on several systems in one launch: workgroup g (after the XCD
@@ -396,7 +422,8 @@ __global__ __launch_bounds__(kStreamRows) void grouped_sell_kernel(
     if (t.n == 0) return;  // capacity padding of a captured launch
     const int blk = g - (int)t.block0;
     sell_body<TV, TX, TS, NV, HS, HD, DOT>(
-        blk, (int)t.n, t.slice_ptr, t.slice_len, t.rows, t.col, (const TV*)t.val, (const TS*)t.s,
+        blk, (int)t.n, t.slice_ptr, t.slice_len, t.rows, t.col, t.col16 != 0, (const TV*)t.val,
+        (const TS*)t.s,
         (const TX*)t.d, (const TX*)t.u, (const TX*)t.x, nullptr, (TX*)t.z, alpha, TA(0),
         (const TX*)t.w, t.partials, red);
 }
@@ -464,8 +491,8 @@ __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
 }
 
 template <class TV, class TX, class TS, int NV>
-int stream_launch(int nb, const int* blocks, const int* slice_len, const int* rows, int n, int lg,
-                  const int* off,
+int stream_launch(int nb, const int* blocks, const int* slice_len, const int* rows, bool col16,
+                  int n, int lg, const int* off,
                   const int* col, const void* val, const void* s, const void* d, const void* u,
                   const void* x, const void* y, void* z, double alpha, double beta, const void* w,
                   double* partials, double* out, hipStream_t st) {
@@ -477,7 +504,8 @@ int stream_launch(int nb, const int* blocks, const int* slice_len, const int* ro
     do {                                                                                        \
         if (lg == -2)                                                                           \
             hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(grid),           \
-                               dim3(kStreamRows), 0, st, n, blocks, slice_len, rows, col,       \
+                               dim3(kStreamRows), 0, st, n, blocks, slice_len, rows,            \
+                               (const void*)col, (int)col16,                                    \
                                (const TV*)val, (const TS*)s, (const TX*)d, (const TX*)u,        \
                                (const TX*)x, (const TX*)y, (TX*)z, (TA)alpha, (TA)beta,         \
                                (const TX*)w, partials);                                         \
@@ -516,13 +544,15 @@ int stream_for_nv(const StreamArgs& a) {
 #define ST(TV, TX)                                                                              \
     do {                                                                                        \
         if (a.s32)                                                                              \
-            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.rows, a.n,  \
+            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.rows,       \
+                                                     a.col16, a.n,                               \
                                                      a.lg,                                       \
                                                      a.off, a.col,                               \
                                                      a.val, a.s, a.d, a.u, a.x, a.y, a.z,        \
                                                      a.alpha, a.beta, a.w, a.partials, a.out,    \
                                                      a.st);                                      \
-        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.rows, a.n, a.lg,     \
+        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.rows, a.col16,     \
+                                             a.n, a.lg,                                          \
                                              a.off,                                              \
                                              a.col, a.val,                                       \
                                              a.s, a.d, a.u, a.x, a.y, a.z, a.alpha, a.beta, a.w, \

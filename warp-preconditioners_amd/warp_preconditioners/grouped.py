@@ -29,7 +29,7 @@ from .sparse import StreamPlan
 from .sparse_operator import SparseOperator
 from .squared_laplacian import SquaredLaplacianOperator
 
-_WORDS = 14  # 64-bit words per task (include/wpc.h, grouped CG steps)
+_WORDS = 15  # 64-bit words per task (include/wpc.h, grouped CG steps)
 
 
 def _p(t):
@@ -96,6 +96,10 @@ def _sliced(plan):
     return [_p(sp), _p(sl), _p(plan._sell[5]), _p(sc), _p(sv)]
 
 
+def _col16(plan):
+    return int(plan._sell[6])
+
+
 class GroupedCG:
     """Captured grouped CG chunks for one ``group_key``."""
 
@@ -130,23 +134,24 @@ class GroupedCG:
             p_, r_, z_, ap_ = (_p(t) for t in (w.p, w.r, w.z, w.ap))
             a_sl = _sliced(plan_a)
             pg, pgt = M.plans()
+            ca, cg, cgt = _col16(plan_a), _col16(pg), _col16(pgt)
             if self.kind == "factored":
-                k1 = a_sl + [_p(A.inverse_mass), 0, 0, p_, _p(sc.tmp_a), 0, 0, n, b_sell]
+                k1 = a_sl + [_p(A.inverse_mass), 0, 0, p_, _p(sc.tmp_a), 0, 0, n, b_sell, ca]
                 k2 = a_sl + [0, _p(A._wm) if self.weight else 0, p_ if self.weight else 0,
-                             _p(sc.tmp_a), ap_, p_, _p(sc.part_ap), n, b_sell]
+                             _p(sc.tmp_a), ap_, p_, _p(sc.part_ap), n, b_sell, ca]
                 ops.append((k1, k2))
             else:
-                ops.append((a_sl + [0, 0, 0, p_, ap_, p_, _p(sc.part_ap), n, b_sell],))
+                ops.append((a_sl + [0, 0, 0, p_, ap_, p_, _p(sc.part_ap), n, b_sell, ca],))
             g = _sliced(pg) + [_p(M.row_scale2) if self.scaled else 0, 0, 0, r_,
-                               _p(sc.tmp_m), 0, 0, n, b_sell]
+                               _p(sc.tmp_m), 0, 0, n, b_sell, cg]
             gt = _sliced(pgt) + [0, 0, 0, _p(sc.tmp_m), z_, r_, _p(sc.part_rz), n,
-                                 b_sell]
+                                 b_sell, cgt]
             sell.append((g, gt))
             pr = _p(w.parts[1])
             st = _p(w.state)
             fold_u.append([_p(w._group_x), r_, p_, ap_, 0, _p(sc.part_ap), 0, st, pr,
-                           n, nb, b_fold, pa, 0])
-            fold_d.append([0, 0, p_, 0, z_, _p(sc.part_rz), pr, st, 0, n, nb, b_fold, pa, nb])
+                           n, nb, b_fold, pa, 0, 0])
+            fold_d.append([0, 0, p_, 0, z_, _p(sc.part_rz), pr, st, 0, n, nb, b_fold, pa, nb, 0])
             b_sell += pa
             b_fold += nb
         nops = len(ops[0])
