@@ -277,12 +277,18 @@ def kernel_times(s, reps=30):
     nnzG, nnzK = P.G.nnz, L.nnz
     scale_b = 4 * n if P.row_scale2 is not None else 0
     idx = 4 * (n + 1) + 4 * (pg.nblocks + 1)  # offsets + row partition
+    def ib(plan, mode):
+        """Bytes of one stored column index in the layout the pass reads."""
+        col16 = mode == plan.SLICED and plan._sell is not None and plan._sell[6]
+        return 2 if col16 else 4
+
+    ig, igt, ik = ib(pg, tuned["G"]), ib(pgt, tuned["GT"]), ib(pl, tuned["K"])
     bytes_ = {
         # values + columns streamed once, x gathered once, z written once (+ extras)
-        "fsai_G_pass": nnzG * (vb_g + 4) + idx + 2 * n * k * xb + scale_b,
-        "fsai_GT_pass_dot": nnzG * (vb_g + 4) + idx + 3 * n * k * xb,
-        "sqlap_forward": nnzK * (vb_k + 4) + idx + 2 * n * k * xb + n * vb_k,
-        "sqlap_backward_dot": nnzK * (vb_k + 4) + idx + 3 * n * k * xb + n * vb_k,
+        "fsai_G_pass": nnzG * (vb_g + ig) + idx + 2 * n * k * xb + scale_b,
+        "fsai_GT_pass_dot": nnzG * (vb_g + igt) + idx + 3 * n * k * xb,
+        "sqlap_forward": nnzK * (vb_k + ik) + idx + 2 * n * k * xb + n * vb_k,
+        "sqlap_backward_dot": nnzK * (vb_k + ik) + idx + 3 * n * k * xb + n * vb_k,
         "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
         "cg_update_fold": 6 * n * k * xb + 8 * pk * k,
