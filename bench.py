@@ -277,18 +277,15 @@ def kernel_times(s, reps=30):
     nnzG, nnzK = P.G.nnz, L.nnz
     scale_b = 4 * n if P.row_scale2 is not None else 0
     idx = 4 * (n + 1) + 4 * (pg.nblocks + 1)  # offsets + row partition
-    def ib(plan, mode):
-        """Bytes of one stored column index in the layout the pass reads."""
-        col16 = mode == plan.SLICED and plan._sell is not None and plan._sell[6]
-        return 2 if col16 else 4
-
-    ig, igt, ik = ib(pg, tuned["G"]), ib(pgt, tuned["GT"]), ib(pl, tuned["K"])
+    # Algorithmic bytes count 4-byte column indices (the CSR format of the path);
+    # the sliced layout may store them as 2-byte differences, which the measured
+    # traffic shows as fewer bytes than these.
     bytes_ = {
         # values + columns streamed once, x gathered once, z written once (+ extras)
-        "fsai_G_pass": nnzG * (vb_g + ig) + idx + 2 * n * k * xb + scale_b,
-        "fsai_GT_pass_dot": nnzG * (vb_g + igt) + idx + 3 * n * k * xb,
-        "sqlap_forward": nnzK * (vb_k + ik) + idx + 2 * n * k * xb + n * vb_k,
-        "sqlap_backward_dot": nnzK * (vb_k + ik) + idx + 3 * n * k * xb + n * vb_k,
+        "fsai_G_pass": nnzG * (vb_g + 4) + idx + 2 * n * k * xb + scale_b,
+        "fsai_GT_pass_dot": nnzG * (vb_g + 4) + idx + 3 * n * k * xb,
+        "sqlap_forward": nnzK * (vb_k + 4) + idx + 2 * n * k * xb + n * vb_k,
+        "sqlap_backward_dot": nnzK * (vb_k + 4) + idx + 3 * n * k * xb + n * vb_k,
         "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
         "cg_update_fold": 6 * n * k * xb + 8 * pk * k,
@@ -491,12 +488,13 @@ def block_bench(name):
                       "methods": records}), flush=True)
 
 
-def apply_bench(sides=(64, 96), nvecs=(1, 3, 8), reps=50):
+def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8), reps=50):
     """FSAI-apply HBM throughput: z = G^T (G x) with the width-8 factor of the
     assembled biharmonic Q (device-assembled on the jittered grid, float32 factor,
     float64 vectors), both passes timed together with HIP events on the launch
-    stream; algorithmic bytes 2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (+ the
-    r.z dot read when fused).  Prints one JSON line."""
+    stream; algorithmic bytes 2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (4-byte
+    column indices, whatever the layout stores: `index_bytes` reports 2 where a pass
+    reads int16 column differences).  Prints one JSON line."""
     import torch
 
     import warp_preconditioners as wp
@@ -524,11 +522,14 @@ def apply_bench(sides=(64, 96), nvecs=(1, 3, 8), reps=50):
             e1.record(stream)
             e1.synchronize()
             us = e0.elapsed_time(e1) / reps * 1e3
-            nbytes = 2 * nnz * (4 + 4) + 8 * (n + 1) + 4 * n * k * 8
             pg, pgt = P.plans()
+            ib = [2 if (pl.mode(k, torch.float64) == pl.SLICED and pl._sell is not None
+                        and pl._sell[6]) else 4 for pl in (pg, pgt)]
+            nbytes = 2 * nnz * (4 + 4) + 8 * (n + 1) + 4 * n * k * 8
             rows.append(dict(side=side, n=n, nnzG=nnz, nvec=k, us=us, GBs=nbytes / us / 1e3,
                              frac=nbytes / us / 1e3 / HBM_PEAK_GBS,
-                             mapping=[pg.mode(k, torch.float64), pgt.mode(k, torch.float64)]))
+                             mapping=[pg.mode(k, torch.float64), pgt.mode(k, torch.float64)],
+                             index_bytes=ib))
             log(json.dumps(rows[-1]))
         del P, Q, K, m
     print(json.dumps({"metric": "FSAI-apply HBM GB/s", "unit": "GB/s", "peak": HBM_PEAK_GBS,
