@@ -1193,6 +1193,10 @@ int orc_bsr_spmv(int dt, int nb, int b, const int* off, const int* col, const vo
 // the first converged iteration rounded up to a multiple of check_every.  With
 // this rule the reference's recorded grid32/grid64 counts are reproduced exactly.
 // Returns iterations; the residual norm (not squared) through *residual.
+// Threads of the PCG baseline's row loops (orc_set_krylov_threads).  Rows are
+// independent and each is summed serially, so any thread count gives the same bits.
+static int g_krylov_threads = 1;
+
 struct OrcOp {
     int kind;
     int n;
@@ -1201,6 +1205,7 @@ struct OrcOp {
     std::vector<double> tmp;
     void apply(const double* x, double* y) {
         if (kind == 0) {
+#pragma omp parallel for num_threads(g_krylov_threads) schedule(static)
             for (int r = 0; r < n; ++r) {
                 double s = 0.0;
                 for (int e = off[r]; e < off[r + 1]; ++e) s += val[e] * x[col[e]];
@@ -1208,11 +1213,14 @@ struct OrcOp {
             }
         } else {
             tmp.resize(n);
+            double* t = tmp.data();
+#pragma omp parallel for num_threads(g_krylov_threads) schedule(static)
             for (int r = 0; r < n; ++r) {
                 double s = 0.0;
                 for (int e = off[r]; e < off[r + 1]; ++e) s += val[e] * x[col[e]];
-                tmp[r] = (1.0 / mass[r]) * s;
+                t[r] = (1.0 / mass[r]) * s;
             }
+#pragma omp parallel for num_threads(g_krylov_threads) schedule(static)
             for (int r = 0; r < n; ++r) {
                 double s = 0.0;
                 for (int e = off[r]; e < off[r + 1]; ++e) s += val[e] * tmp[col[e]];
@@ -1233,14 +1241,17 @@ struct OrcPrec {
         if (kind == 0) { for (int i = 0; i < n; ++i) z[i] = r[i]; return; }
         if (kind == 2) { for (int i = 0; i < n; ++i) z[i] = r[i] / diag[i]; return; }
         tmp.resize(n);
+        double* t = tmp.data();
+#pragma omp parallel for num_threads(g_krylov_threads) schedule(static)
         for (int i = 0; i < n; ++i) {
             double s = 0.0;
             for (int e = goff[i]; e < goff[i + 1]; ++e) s += gval[e] * r[gcol[e]];
-            tmp[i] = s;
+            t[i] = s;
         }
+#pragma omp parallel for num_threads(g_krylov_threads) schedule(static)
         for (int i = 0; i < n; ++i) {
             double s = 0.0;
-            for (int e = toff[i]; e < toff[i + 1]; ++e) s += tval[e] * tmp[tcol[e]];
+            for (int e = toff[i]; e < toff[i + 1]; ++e) s += tval[e] * t[tcol[e]];
             z[i] = s;
         }
     }
@@ -1251,6 +1262,8 @@ static double dot(const std::vector<double>& a, const std::vector<double>& b) {
     for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
     return s;
 }
+
+void orc_set_krylov_threads(int threads) { g_krylov_threads = threads > 0 ? threads : 1; }
 
 int orc_krylov(int method, int n, int op_kind, const int* off, const int* col, const double* val,
                const double* mass, double weight, int prec_kind, const int* goff, const int* gcol,

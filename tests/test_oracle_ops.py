@@ -224,3 +224,30 @@ def test_cg_and_cr_converge_to_direct_solve(oracle):
         x, it, _ = oracle.krylov(method, w * mass * target, K=csr(oracle, lap), mass=mass, weight=w,
                                  x0=target, tol=1e-12, maxiter=300)
         np.testing.assert_allclose(x, expected, rtol=1e-10, atol=1e-11)
+
+
+def test_threaded_pcg_rows_give_identical_iterates(oracle):
+    # The CPU baseline threads the PCG row loops; rows are summed serially, so the
+    # iterates are the single-thread ones bit for bit.
+    import scipy.sparse as ss
+
+    side = 40
+    d = ss.diags([-np.ones(side - 1), 2 * np.ones(side), -np.ones(side - 1)], [-1, 0, 1])
+    k = ss.kronsum(d, d).tocsr()
+    n = k.shape[0]
+    rng = np.random.default_rng(3)
+    mass = rng.uniform(0.5, 1.5, n) / side ** 2
+    q = (k @ ss.diags(1 / mass) @ k + ss.diags(mass)).tocsr()
+    P = oracle.fsai(oracle.CSR.from_scipy(q), max_row_size=8)
+    K = oracle.CSR.from_scipy(k)
+    b = rng.normal(size=n)
+    out = []
+    for threads in (1, 4):
+        oracle.lib().orc_set_krylov_threads(threads)
+        try:
+            out.append(oracle.krylov("cg", b, K=K, mass=mass, weight=1.0, P=P, tol=1e-10,
+                                     maxiter=400, check_every=25))
+        finally:
+            oracle.lib().orc_set_krylov_threads(1)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
