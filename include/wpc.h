@@ -46,7 +46,7 @@ const char* wpc_version(void);
  * Replaces sparse_operator.py:90-104 (_matvec: the grouped multi-lane kernel
  * :10-40, the mixed-storage kernel :64-87 and Warp's bsr_mv).
  * `lanes` (1,2,4,8,16,32,64; 0 = pick by mean row length) lanes cooperate on
- * each row.  `nvec` (1,2,3,4,8) vectors are stored interleaved: x[col*nvec+v].
+ * each row.  `nvec` (1..8) vectors are stored interleaved: x[col*nvec+v].
  * row_scale (nullable, float32[n]) multiplies each row sum before alpha.
  * alpha == 0 never reads A or x; beta == 0 never reads y.  x must not alias z;
  * y may alias z.  Accumulation is in the vector format (float for binary16
