@@ -641,6 +641,16 @@ def serving_bench(args):
     print(json.dumps(record), flush=True)
 
 
+def force_mapping(s, mapping):
+    """Fix the row mappings of the G, G^T and K products (0 streamed, L lanes, -1 sliced)."""
+    g, gt, k = mapping
+    pg, pgt = s.P.plans()
+    pg.fixed, pgt.fixed = g, gt
+    for op in (getattr(s, "op64", None), getattr(s, "op32", None)):
+        if op is not None:
+            op._stream().fixed = k
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -652,7 +662,9 @@ def main():
                              "batched-mixed-bf16"])
     ap.add_argument("--rtol", type=float, default=1e-5,
                     help="recursive relative-residual stopping tolerance")
-    ap.add_argument("--compare", action="store_true", help="also time the plain one-RHS path")
+    ap.add_argument("--compare", action="store_true",
+                    help="also time the plain one-RHS-at-a-time paths (float32 and float64 "
+                         "Krylov vectors; serving: the one-at-a-time engine)")
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mapping", default=None,
@@ -689,13 +701,10 @@ def main():
     wl = build_workload(args.side)
     s = Solver(wl, args.mode)
     log(f"mode {args.mode}: FSAI setup {s.setup_s:.3f}s nnzG={s.nnzG}")
+    forced = None
     if args.mapping:
-        g, gt, k = (int(v) for v in args.mapping.split(","))
-        pg, pgt = s.P.plans()
-        pg.fixed, pgt.fixed = g, gt
-        for op in (getattr(s, "op64", None), getattr(s, "op32", None)):
-            if op is not None:
-                op._stream().fixed = k
+        forced = tuple(int(v) for v in args.mapping.split(","))
+        force_mapping(s, forced)
     if args.profile_kernels:
         profile_kernels(s)
         return
@@ -757,6 +766,29 @@ def main():
         "converged": ok,
         "setup_s": s.setup_s,
     }
+    if args.compare:
+        # the plain one-right-hand-side-at-a-time paths on the same GPU and stopping rule
+        record["compare"] = {}
+        for mode in ("plain-f32", "plain-f64"):
+            p = Solver(wl, mode)
+            if forced:
+                force_mapping(p, forced)
+            p.solve(tol=args.rtol)  # warm
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            p.solve(tol=args.rtol)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            perr = field_error(p, p.X, Xref)
+            reached = bool((perr <= 1e-8).all())
+            record["compare"][mode] = {
+                "solves_per_s": p.k / dt, "value_at_1e-8": p.k / dt if reached else 0.0,
+                "field_error": [float(e) for e in perr], "iterations": p.iterations,
+                "reaches_1e-8": reached}
+            log(f"compare {mode}: {p.k / dt:.2f} solves/s, field error {perr}")
+            del p
+        f64 = record["compare"]["plain-f64"]["solves_per_s"]
+        record["compare"]["batched_vs_plain_f64"] = record["value"] / f64
     if not args.no_cpu:
         try:
             record["cpu_baseline"] = cpu_baseline(wl, s, max(s.iterations))
