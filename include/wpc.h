@@ -453,28 +453,28 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, in
 
 /* ------------------------------------------------------------------ grouped CG steps
  * No reference counterpart: a serving extension that advances the fused CG of
- * several systems (uneven sizes, same vector format and count, same product
- * flags) with one launch per pass instead of one per system.  `tasks` is a device
- * table of 15 int64 words per system; systems own consecutive global workgroups
+ * several systems (uneven sizes and right-hand-side counts, same vector format
+ * and product flags) with one launch per pass instead of one per system.  `tasks`
+ * is a device table of 16 int64 words per system; systems own consecutive global workgroups
  * starting at their block0 (ascending, the first 0), total_blocks in all; tasks
  * with n = 0 (block0 = the end of the real ones) pad a launch captured for a
  * larger group and do nothing.  Each system computes exactly what its own
  * launch computes, bit for bit.
  *
  * wpc_sell_stream_grouped: per task {slice_ptr, slice_len, rows, sell_columns,
- *   sell_values, s, d, u, x, z, w, partials, n, block0, col16} -- the wpc_sell_stream
+ *   sell_values, s, d, u, x, z, w, partials, n, block0, col16, nvec} -- the wpc_sell_stream
  *   product with beta = 0 on each system (ceil(n/256) workgroups); s_dtype 0 means
  *   no row scale, has_d selects the d .* u term, dot the w . z partials.
  * wpc_cg_fold_grouped: step 1 = wpc_cg_update_fold, step 2 = wpc_cg_direction_fold
  *   per task {x, r, p, ap, z, part_a, part_b, state, partials, n, nblocks, block0,
- *   pa, pb, 0} with nblocks = wpc_krylov_partials(n); update folds pa partials of
+ *   pa, pb, nvec, 0} with nblocks = wpc_krylov_partials(n); update folds pa partials of
  *   p.Ap from part_a and leaves r.r partials in `partials`; direction folds pa
  *   r.z partials (part_a) and pb r.r partials (part_b). */
 int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int val_dtype,
-                            int s_dtype, int has_d, int dot, int vec_dtype, int nvec,
-                            double alpha, void* stream);
+                            int s_dtype, int has_d, int dot, int vec_dtype, double alpha,
+                            void* stream);
 int wpc_cg_fold_grouped(int step, int ntasks, const void* tasks, int total_blocks, int dtype,
-                        int nvec, void* stream);
+                        void* stream);
 
 /* ------------------------------------------------------------------ certified residuals
  * No single reference routine: these supply the float64 true residual b - A x

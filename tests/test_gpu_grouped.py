@@ -49,17 +49,18 @@ def _plain(side, seed, dtype=torch.float32, factor=torch.bfloat16):
 
 
 @pytest.mark.parametrize("case", ["factored-f64", "plain-f32-bf16"])
-@pytest.mark.parametrize("nvec", [1, 3])
-def test_grouped_steps_equal_own_steps(case, nvec):
+@pytest.mark.parametrize("nvecs", [(1, 1, 1, 1), (3, 1, 8, 2)])
+def test_grouped_steps_equal_own_steps(case, nvecs):
+    # uneven sizes and (second case) uneven right-hand-side counts in one batch
     from warp_preconditioners import grouped
     from warp_preconditioners.linear import KrylovWork
 
     dtype = torch.float64 if case == "factored-f64" else torch.float32
     systems = [(_factored if case == "factored-f64" else _plain)(side, seed)
                for side, seed in ((9, 1), (40, 2), (23, 3), (57, 4))]
-    rng = np.random.default_rng(nvec)
+    rng = np.random.default_rng(sum(nvecs))
     own, mine, xs_own, xs_mine = [], [], [], []
-    for n, A, P in systems:
+    for (n, A, P), nvec in zip(systems, nvecs):
         B = _vec(rng.normal(size=(n, nvec)), dtype)
         for works, xs in ((own, xs_own), (mine, xs_mine)):
             w = KrylovWork("cg", A, P, n, nvec, dtype, dev())

@@ -13,14 +13,19 @@ extern "C" {
 
 int wpc_graph_begin(void* stream) {
     hipError_t e = hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) (void)hipGetLastError();  // leave no error for the next launch check
     return e == hipSuccess ? WPC_OK : (int)e;
 }
 
+// A failed capture (e.g. invalidated) ends here with no graph, the stream out of
+// capture mode and the thread's error state cleared, so the caller can launch
+// the same work directly.
 int wpc_graph_end(void* stream, void** graph_exec) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(as_stream(stream), &g);
     if (e != hipSuccess) {
         if (g != nullptr) hipGraphDestroy(g);
+        (void)hipGetLastError();
         return (int)e;
     }
     hipGraphExec_t x = nullptr;

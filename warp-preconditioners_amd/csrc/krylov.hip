@@ -269,18 +269,20 @@ template <class T, int NV>
 __device__ __forceinline__ void cg_update_fold_body(
     int blk, int nblocks, int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
-    double* __restrict__ partials, double* red, double* sums) {
-    constexpr int PF = kPrefetch<NV>;
+    double* __restrict__ partials, double* red, double* sums, int nv_rt = NV) {
+    constexpr int PF = NV > 0 ? kPrefetch<NV> : 1;
+    constexpr int NVM = NV > 0 ? NV : 8;
+    const int nv = NV > 0 ? NV : nv_rt;
     long begin, end;
     chunk_of(n, blk, nblocks, &begin, &end);
-    T xv[PF][NV], rv[PF][NV], pv[PF][NV], av[PF][NV];
+    T xv[PF][NVM], rv[PF][NVM], pv[PF][NVM], av[PF][NVM];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
         if (i < end) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const long k = i * NV + v;
+            for (int v = 0; v < NVM; ++v) if (v < nv) {
+                const long k = i * nv + v;
                 xv[j][v] = x[k];
                 rv[j][v] = r[k];
                 pv[j][v] = p[k];
@@ -288,31 +290,31 @@ __device__ __forceinline__ void cg_update_fold_body(
             }
         }
     }
-    fold_all<kKrylovThreads>(part_denom, P, NV, sums);
-    T alpha[NV];
+    fold_all<kKrylovThreads>(part_denom, P, nv, sums);
+    T alpha[NVM];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+    for (int v = 0; v < NVM; ++v) if (v < nv) {
         const double denom = sums[v];
-        const bool active = state[WPC_KS_RR * NV + v] > state[WPC_KS_TARGET * NV + v];
-        const double a = active ? state[WPC_KS_RZ * NV + v] / denom : 0.0;
+        const bool active = state[WPC_KS_RR * nv + v] > state[WPC_KS_TARGET * nv + v];
+        const double a = active ? state[WPC_KS_RZ * nv + v] / denom : 0.0;
         alpha[v] = (T)a;
         if (blk == 0 && threadIdx.x == v) {
-            state[WPC_KS_RZ_OLD * NV + v] = state[WPC_KS_RZ * NV + v];
-            state[WPC_KS_DENOM * NV + v] = denom;
-            state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
-            state[WPC_KS_ALPHA * NV + v] = a;
+            state[WPC_KS_RZ_OLD * nv + v] = state[WPC_KS_RZ * nv + v];
+            state[WPC_KS_DENOM * nv + v] = denom;
+            state[WPC_KS_ACTIVE * nv + v] = active ? 1.0 : 0.0;
+            state[WPC_KS_ALPHA * nv + v] = a;
         }
     }
-    double acc[NV];
+    double acc[NVM];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    for (int v = 0; v < NVM; ++v) if (v < nv) acc[v] = 0.0;
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
         if (i < end) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const long k = i * NV + v;
+            for (int v = 0; v < NVM; ++v) if (v < nv) {
+                const long k = i * nv + v;
                 T rk = rv[j][v];
                 if (alpha[v] != T(0)) {
                     x[k] = xv[j][v] + alpha[v] * pv[j][v];
@@ -325,8 +327,8 @@ __device__ __forceinline__ void cg_update_fold_body(
     }
     for (long i = begin + threadIdx.x + (long)PF * kKrylovThreads; i < end; i += kKrylovThreads) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const long k = i * NV + v;
+        for (int v = 0; v < NVM; ++v) if (v < nv) {
+            const long k = i * nv + v;
             T rk = r[k];
             if (alpha[v] != T(0)) {
                 x[k] += alpha[v] * p[k];
@@ -336,8 +338,13 @@ __device__ __forceinline__ void cg_update_fold_body(
             acc[v] += (double)rk * (double)rk;
         }
     }
-    block_sum<NV>(acc, red);
-    write_partials<NV>(acc, partials, blk);
+    if constexpr (NV > 0) {
+        block_sum<NV>(acc, red);
+        write_partials<NV>(acc, partials, blk);
+    } else {
+        block_sum_n(acc, nv, red);
+        write_partials_n(acc, nv, partials, blk);
+    }
 }
 
 template <class T, int NV>
@@ -355,39 +362,41 @@ template <class T, int NV>
 __device__ __forceinline__ void cg_direction_fold_body(
     int blk, int nblocks, int n, const T* __restrict__ z, T* __restrict__ p, int P_rz,
     const double* __restrict__ part_rz, int P_rr, const double* __restrict__ part_rr,
-    double* state, double* s_rz, double* s_rr) {
-    constexpr int PF = kPrefetch<NV>;
+    double* state, double* s_rz, double* s_rr, int nv_rt = NV) {
+    constexpr int PF = NV > 0 ? kPrefetch<NV> : 1;
+    constexpr int NVM = NV > 0 ? NV : 8;
+    const int nv = NV > 0 ? NV : nv_rt;
     long begin, end;
     chunk_of(n, blk, nblocks, &begin, &end);
-    T zv[PF][NV], pv[PF][NV];
+    T zv[PF][NVM], pv[PF][NVM];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
         if (i < end) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                zv[j][v] = z[i * NV + v];
-                pv[j][v] = p[i * NV + v];
+            for (int v = 0; v < NVM; ++v) if (v < nv) {
+                zv[j][v] = z[i * nv + v];
+                pv[j][v] = p[i * nv + v];
             }
         }
     }
-    fold_all<kKrylovThreads>(part_rz, P_rz, NV, s_rz);
-    fold_all<kKrylovThreads>(part_rr, P_rr, NV, s_rr);
-    T beta[NV];
-    bool act[NV];
+    fold_all<kKrylovThreads>(part_rz, P_rz, nv, s_rz);
+    fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
+    T beta[NVM];
+    bool act[NVM];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        act[v] = state[WPC_KS_ACTIVE * NV + v] != 0.0;
-        const double b = act[v] ? s_rz[v] / state[WPC_KS_RZ_OLD * NV + v] : 0.0;
+    for (int v = 0; v < NVM; ++v) if (v < nv) {
+        act[v] = state[WPC_KS_ACTIVE * nv + v] != 0.0;
+        const double b = act[v] ? s_rz[v] / state[WPC_KS_RZ_OLD * nv + v] : 0.0;
         beta[v] = (T)b;
         if (blk == 0 && threadIdx.x == v) {
-            state[WPC_KS_RZ_NEW * NV + v] = s_rz[v];
-            state[WPC_KS_RR_NEW * NV + v] = s_rr[v];
-            state[WPC_KS_BETA * NV + v] = b;
+            state[WPC_KS_RZ_NEW * nv + v] = s_rz[v];
+            state[WPC_KS_RR_NEW * nv + v] = s_rr[v];
+            state[WPC_KS_BETA * nv + v] = b;
             if (act[v]) {
-                state[WPC_KS_RZ * NV + v] = s_rz[v];
-                state[WPC_KS_RR * NV + v] = s_rr[v];
-                state[WPC_KS_ITER * NV + v] += 1.0;
+                state[WPC_KS_RZ * nv + v] = s_rz[v];
+                state[WPC_KS_RR * nv + v] = s_rr[v];
+                state[WPC_KS_ITER * nv + v] += 1.0;
             }
         }
     }
@@ -396,15 +405,15 @@ __device__ __forceinline__ void cg_direction_fold_body(
         const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
         if (i < end) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                if (act[v]) p[i * NV + v] = zv[j][v] + beta[v] * pv[j][v];
+            for (int v = 0; v < NVM; ++v) if (v < nv)
+                if (act[v]) p[i * nv + v] = zv[j][v] + beta[v] * pv[j][v];
         }
     }
     for (long i = begin + threadIdx.x + (long)PF * kKrylovThreads; i < end; i += kKrylovThreads) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
+        for (int v = 0; v < NVM; ++v) if (v < nv) {
             if (!act[v]) continue;
-            const long k = i * NV + v;
+            const long k = i * nv + v;
             p[k] = z[k] + beta[v] * p[k];
         }
     }
@@ -438,9 +447,10 @@ struct CgTask {
     long block0;
     long pa;
     long pb;
-    long unused;  // tables of both kinds are 15 words wide
+    long nv;      // interleaved vectors of this system (1..8)
+    long unused;  // tables of both kinds are 16 words wide
 };
-static_assert(sizeof(CgTask) == 15 * 8, "CgTask is 15 64-bit words");
+static_assert(sizeof(CgTask) == 16 * 8, "CgTask is 16 64-bit words");
 
 __device__ __forceinline__ int task_of(const CgTask* __restrict__ tasks, int ntasks, int g,
                                        int* s_task) {
@@ -471,7 +481,7 @@ __global__ __launch_bounds__(kKrylovThreads) void grouped_update_fold_kernel(
     if (t.n == 0) return;  // capacity padding
     cg_update_fold_body<T, NV>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (T*)t.x, (T*)t.r,
                                (const T*)t.p, (const T*)t.ap, (int)t.pa, t.part_a, t.state,
-                               t.partials, red, sums);
+                               t.partials, red, sums, (int)t.nv);
 }
 
 template <class T, int NV>
@@ -484,7 +494,7 @@ __global__ __launch_bounds__(kKrylovThreads) void grouped_direction_fold_kernel(
     if (t.n == 0) return;  // capacity padding
     cg_direction_fold_body<T, NV>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (const T*)t.z,
                                   (T*)t.p, (int)t.pa, t.part_a, (int)t.pb, t.part_b, t.state,
-                                  s_rz, s_rr);
+                                  s_rz, s_rr, (int)t.nv);
 }
 
 // p = z + BETA p for slots that stepped.
@@ -632,15 +642,16 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, in
 }
 
 int wpc_cg_fold_grouped(int step, int ntasks, const void* tasks, int total_blocks, int dtype,
-                        int nvec, void* stream) {
+                        void* stream) {
     if (ntasks <= 0) return WPC_OK;
     if (tasks == nullptr || total_blocks <= 0 || (step != 1 && step != 2)) return WPC_EARG;
     hipStream_t s = as_stream(stream);
     const CgTask* t = (const CgTask*)tasks;
+    // one instantiation for every vector count: each task carries its own
     if (step == 1) {
-        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((grouped_update_fold_kernel<T, NV>), dim3(total_blocks), dim3(kKrylovThreads), 0, s, t, ntasks)))
+        VT_SWITCH(dtype, hipLaunchKernelGGL((grouped_update_fold_kernel<T, 0>), dim3(total_blocks), dim3(kKrylovThreads), 0, s, t, ntasks))
     } else {
-        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((grouped_direction_fold_kernel<T, NV>), dim3(total_blocks), dim3(kKrylovThreads), 0, s, t, ntasks)))
+        VT_SWITCH(dtype, hipLaunchKernelGGL((grouped_direction_fold_kernel<T, 0>), dim3(total_blocks), dim3(kKrylovThreads), 0, s, t, ntasks))
     }
     return launch_status();
 }

@@ -52,6 +52,32 @@ __device__ __forceinline__ void write_partials(const double* v, double* partials
     }
 }
 
+// The same two steps for a count nv <= 8 known only at run time (batched launches
+// over systems with different right-hand-side counts); per vector they perform
+// exactly the operations of block_sum<NV> / write_partials<NV>.
+__device__ inline void block_sum_n(double* v, int nv, double* red) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (q >= nv) break;
+        double x = v[q];
+        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
+        __syncthreads();
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < kStreamRows / kWave; ++w) s += red[w];
+            v[q] = s;
+        }
+    }
+}
+
+__device__ inline void write_partials_n(const double* v, int nv, double* partials, int block) {
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < nv; ++q) partials[(long)block * nv + q] = v[q];
+    }
+}
+
 // Fixed-order fold of partials[0..P) (stride nvec, vector v) by one workgroup;
 // the result is returned to every thread.
 __device__ inline double fold_block(const double* __restrict__ partials, int P, int nvec, int v,

@@ -68,25 +68,15 @@ int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int
 }
 
 int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int val_dtype,
-                            int s_dtype, int has_d, int dot, int vec_dtype, int nvec,
-                            double alpha, void* stream) {
+                            int s_dtype, int has_d, int dot, int vec_dtype, double alpha,
+                            void* stream) {
     if (ntasks <= 0) return WPC_OK;
     if (tasks == nullptr || total_blocks <= 0) return WPC_EARG;
     if (s_dtype != 0 && s_dtype != WPC_F32 && s_dtype != vec_dtype) return WPC_EDTYPE;
     const GroupedArgs a{ntasks, (const SellTask*)tasks, total_blocks, val_dtype, s_dtype != 0,
                         s_dtype == WPC_F32, has_d != 0, dot != 0, vec_dtype, alpha,
                         as_stream(stream)};
-    switch (nvec) {
-        case 1: return stream_grouped_entry_1(a);
-        case 2: return stream_grouped_entry_2(a);
-        case 3: return stream_grouped_entry_3(a);
-        case 4: return stream_grouped_entry_4(a);
-        case 5: return stream_grouped_entry_5(a);
-        case 6: return stream_grouped_entry_6(a);
-        case 7: return stream_grouped_entry_7(a);
-        case 8: return stream_grouped_entry_8(a);
-        default: return WPC_EARG;
-    }
+    return stream_grouped_entry(a);
 }
 
 }  // extern "C"

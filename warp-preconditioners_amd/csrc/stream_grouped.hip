@@ -1,0 +1,4 @@
+// Batched sliced products over several systems, each with its own number of
+// interleaved vectors (one runtime-count instantiation of stream_impl.h).
+#define WPC_STREAM_NV 0
+#include "stream_impl.h"

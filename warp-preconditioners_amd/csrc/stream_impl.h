@@ -82,8 +82,9 @@ struct SellTask {
     long n;
     long block0;
     long col16;  // columns stored as int16 differences from the row
+    long nv;     // interleaved vectors of this system (1..8)
 };
-static_assert(sizeof(SellTask) == 15 * 8, "SellTask is 15 64-bit words");
+static_assert(sizeof(SellTask) == 16 * 8, "SellTask is 16 64-bit words");
 
 struct GroupedArgs {
     int ntasks;
@@ -99,14 +100,8 @@ struct GroupedArgs {
     hipStream_t st;
 };
 
-int stream_grouped_entry_1(const GroupedArgs& a);
-int stream_grouped_entry_2(const GroupedArgs& a);
-int stream_grouped_entry_3(const GroupedArgs& a);
-int stream_grouped_entry_4(const GroupedArgs& a);
-int stream_grouped_entry_5(const GroupedArgs& a);
-int stream_grouped_entry_6(const GroupedArgs& a);
-int stream_grouped_entry_7(const GroupedArgs& a);
-int stream_grouped_entry_8(const GroupedArgs& a);
+// Batched products, each task with its own vector count (stream_grouped.hip).
+int stream_grouped_entry(const GroupedArgs& a);
 
 }  // namespace wpc
 
@@ -301,11 +296,18 @@ template <> __device__ __forceinline__ long sell_col<short>(const short* col, lo
     return (long)row + col[p];
 }
 
+// NV > 0: NV interleaved vectors; NV == 0: nv_rt (<= 8) vectors, the count a
+// batched launch reads per system.  Both perform the same operations per vector.
+template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
+
 template <class TI, class TV, class TX, int NV>
 __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long base, int len,
                                            int row, const TI* __restrict__ col,
-                                           const TV* __restrict__ val, const TX* __restrict__ x) {
+                                           const TV* __restrict__ val, const TX* __restrict__ x,
+                                           int nv_rt) {
     using TA = typename AccOf<TX>::type;
+    constexpr int NVM = kNvMax<NV>;
+    const int nv = NV > 0 ? NV : nv_rt;
     int k = 0;
     for (; k + 4 <= len; k += 4) {
         long c[4];
@@ -319,8 +321,8 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * NV + v);
+            for (int v = 0; v < NVM; ++v)
+                if (v < nv) acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * nv + v);
         }
     }
     for (; k < len; ++k) {
@@ -328,7 +330,8 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
         const long c = sell_col<TI>(col, p, row);
         const TA a = lds_val<TA, TV>(raw_val<TV>(val, p));
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] += a * St<TX>::template ld<TA>(x, c * NV + v);
+        for (int v = 0; v < NVM; ++v)
+            if (v < nv) acc[v] += a * St<TX>::template ld<TA>(x, c * nv + v);
     }
 }
 
@@ -339,31 +342,34 @@ __device__ __forceinline__ void sell_body(
     const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
     const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
     typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
-    double* __restrict__ partials, double* red) {
+    double* __restrict__ partials, double* red, int nv_rt = NV) {
     using TA = typename AccOf<TX>::type;
+    constexpr int NVM = kNvMax<NV>;
+    const int nv = NV > 0 ? NV : nv_rt;
     const int idx = blk * kStreamRows + threadIdx.x;
     const bool live = idx < n;
     const int row = live && rows != nullptr ? rows[idx] : idx;
     const bool use_a = alpha != TA(0);
-    TA acc[NV];
+    TA acc[NVM];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    for (int v = 0; v < NVM; ++v) acc[v] = TA(0);
     if (live && use_a) {
         const int slice = idx >> 6;
         const long base = (long)slice_ptr[slice] * 64 + (idx & 63);
         const int len = slice_len[slice];
         if (col16)
-            sell_accum<short, TV, TX, NV>(acc, base, len, row, (const short*)col, val, x);
+            sell_accum<short, TV, TX, NV>(acc, base, len, row, (const short*)col, val, x, nv);
         else
-            sell_accum<int, TV, TX, NV>(acc, base, len, row, (const int*)col, val, x);
+            sell_accum<int, TV, TX, NV>(acc, base, len, row, (const int*)col, val, x, nv);
     }
-    double dot[NV];
+    double dot[NVM];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) dot[v] = 0.0;
+    for (int v = 0; v < NVM; ++v) dot[v] = 0.0;
     if (live) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const long k = (long)row * NV + v;
+        for (int v = 0; v < NVM; ++v) {
+            if (v >= nv) break;
+            const long k = (long)row * nv + v;
             TA r = TA(0);
             if (use_a) {
                 r = acc[v];
@@ -377,8 +383,13 @@ __device__ __forceinline__ void sell_body(
         }
     }
     if (DOT) {
-        block_sum<NV>(dot, red);
-        write_partials<NV>(dot, partials, blk);
+        if constexpr (NV > 0) {
+            block_sum<NV>(dot, red);
+            write_partials<NV>(dot, partials, blk);
+        } else {
+            block_sum_n(dot, nv, red);
+            write_partials_n(dot, nv, partials, blk);
+        }
     }
 }
 
@@ -425,7 +436,7 @@ __global__ __launch_bounds__(kStreamRows) void grouped_sell_kernel(
         blk, (int)t.n, t.slice_ptr, t.slice_len, t.rows, t.col, t.col16 != 0, (const TV*)t.val,
         (const TS*)t.s,
         (const TX*)t.d, (const TX*)t.u, (const TX*)t.x, nullptr, (TX*)t.z, alpha, TA(0),
-        (const TX*)t.w, t.partials, red);
+        (const TX*)t.w, t.partials, red, (int)t.nv);
 }
 
 template <class TV, class TX, class TS, int NV>
@@ -576,12 +587,13 @@ int stream_for_nv(const StreamArgs& a) {
 
 #define WPC_STREAM_CAT2(a, b) a##b
 #define WPC_STREAM_CAT(a, b) WPC_STREAM_CAT2(a, b)
+#if WPC_STREAM_NV > 0
 int WPC_STREAM_CAT(stream_entry_, WPC_STREAM_NV)(const StreamArgs& a) {
     return stream_for_nv<WPC_STREAM_NV>(a);
 }
-int WPC_STREAM_CAT(stream_grouped_entry_, WPC_STREAM_NV)(const GroupedArgs& a) {
-    return grouped_for_nv<WPC_STREAM_NV>(a);
-}
+#else
+int stream_grouped_entry(const GroupedArgs& a) { return grouped_for_nv<0>(a); }
+#endif
 
 }  // namespace wpc
 

@@ -1,7 +1,7 @@
 """One batched CG step over a table of systems (serving extension).
 
 ``GroupedCG`` advances the fused preconditioned CG of several ``KrylovWork``s --
-systems of uneven size, the same vector format and right-hand-side count -- with
+systems of uneven size and right-hand-side count, one vector format -- with
 one launch per pass for all of them (``wpc_sell_stream_grouped``,
 ``wpc_cg_fold_grouped``): the operator pass(es), the update, the two FSAI passes
 and the direction step.  Each system runs exactly the workgroups its own launch
@@ -29,7 +29,7 @@ from .sparse import StreamPlan
 from .sparse_operator import SparseOperator
 from .squared_laplacian import SquaredLaplacianOperator
 
-_WORDS = 15  # 64-bit words per task (include/wpc.h, grouped CG steps)
+_WORDS = 16  # 64-bit words per task (include/wpc.h, grouped CG steps)
 
 
 def _p(t):
@@ -67,7 +67,7 @@ def group_key(work: KrylovWork):
         return None
     a_vals = (A.L if kind == "factored" else A.scalar).values.dtype
     weight = kind == "factored" and A.mass_weight != 0.0
-    return (kind, a_vals, weight, M.G.values.dtype, M.row_scale2 is not None, work.dtype, nv)
+    return (kind, a_vals, weight, M.G.values.dtype, M.row_scale2 is not None, work.dtype)
 
 
 class _Scratch:
@@ -106,7 +106,7 @@ class GroupedCG:
     def __init__(self, key, steps):
         self.key = key
         self.steps = steps
-        self.kind, self.a_dtype, self.weight, self.g_dtype, self.scaled, self.dtype, self.nvec = key
+        self.kind, self.a_dtype, self.weight, self.g_dtype, self.scaled, self.dtype = key
         self.graph = None
         self.cap_tasks = 0
         self.cap_sell = 0
@@ -116,6 +116,7 @@ class GroupedCG:
         self._ring_next = 0
         self.warm = False
         self.capture_failures = 0
+        self._cap_stream = None
 
     # ------------------------------------------------------------ tables
     def _passes(self, works):
@@ -135,23 +136,25 @@ class GroupedCG:
             a_sl = _sliced(plan_a)
             pg, pgt = M.plans()
             ca, cg, cgt = _col16(plan_a), _col16(pg), _col16(pgt)
+            nv = w.nvec
             if self.kind == "factored":
-                k1 = a_sl + [_p(A.inverse_mass), 0, 0, p_, _p(sc.tmp_a), 0, 0, n, b_sell, ca]
+                k1 = a_sl + [_p(A.inverse_mass), 0, 0, p_, _p(sc.tmp_a), 0, 0, n, b_sell, ca, nv]
                 k2 = a_sl + [0, _p(A._wm) if self.weight else 0, p_ if self.weight else 0,
-                             _p(sc.tmp_a), ap_, p_, _p(sc.part_ap), n, b_sell, ca]
+                             _p(sc.tmp_a), ap_, p_, _p(sc.part_ap), n, b_sell, ca, nv]
                 ops.append((k1, k2))
             else:
-                ops.append((a_sl + [0, 0, 0, p_, ap_, p_, _p(sc.part_ap), n, b_sell, ca],))
+                ops.append((a_sl + [0, 0, 0, p_, ap_, p_, _p(sc.part_ap), n, b_sell, ca, nv],))
             g = _sliced(pg) + [_p(M.row_scale2) if self.scaled else 0, 0, 0, r_,
-                               _p(sc.tmp_m), 0, 0, n, b_sell, cg]
+                               _p(sc.tmp_m), 0, 0, n, b_sell, cg, nv]
             gt = _sliced(pgt) + [0, 0, 0, _p(sc.tmp_m), z_, r_, _p(sc.part_rz), n,
-                                 b_sell, cgt]
+                                 b_sell, cgt, nv]
             sell.append((g, gt))
             pr = _p(w.parts[1])
             st = _p(w.state)
             fold_u.append([_p(w._group_x), r_, p_, ap_, 0, _p(sc.part_ap), 0, st, pr,
-                           n, nb, b_fold, pa, 0, 0])
-            fold_d.append([0, 0, p_, 0, z_, _p(sc.part_rz), pr, st, 0, n, nb, b_fold, pa, nb, 0])
+                           n, nb, b_fold, pa, 0, nv, 0])
+            fold_d.append([0, 0, p_, 0, z_, _p(sc.part_rz), pr, st, 0, n, nb, b_fold, pa, nb, nv,
+                           0])
             b_sell += pa
             b_fold += nb
         nops = len(ops[0])
@@ -173,6 +176,9 @@ class GroupedCG:
             out[len(rows):, 11 if is_fold else 13] = b_fold if is_fold else b_sell
             if is_fold:
                 out[len(rows):, 10] = 1  # nblocks
+                out[len(rows):, 14] = 1  # nv
+            else:
+                out[len(rows):, 15] = 1  # nv
             hosts.append(out)
         # pinned staging, reused once its previous copy has completed
         k = self._ring_next
@@ -196,40 +202,41 @@ class GroupedCG:
         code = nat.code(self.dtype)
         a_code = nat.code(self.a_dtype)
         g_code = nat.code(self.g_dtype)
-        nv = self.nvec
+        sell = lambda i, vals, s_code, has_d, dot: nat.call(  # noqa: E731
+            "wpc_sell_stream_grouped", nt, nat.ptr(t[i]), self.cap_sell, vals, s_code, has_d, dot,
+            code, 1.0, s)
+        fold = lambda step, i: nat.call(  # noqa: E731
+            "wpc_cg_fold_grouped", step, nt, nat.ptr(t[i]), self.cap_fold, code, s)
         i = 0
         if self.kind == "factored":
-            nat.call("wpc_sell_stream_grouped", nt, nat.ptr(t[i]), self.cap_sell, a_code, code, 0, 0,
-                     code, nv, 1.0, s)
-            i += 1
-            nat.call("wpc_sell_stream_grouped", nt, nat.ptr(t[i]), self.cap_sell, a_code, 0,
-                     int(self.weight), 1, code, nv, 1.0, s)
-            i += 1
+            sell(0, a_code, code, 0, 0)  # K pass 1, inverse-mass scale
+            sell(1, a_code, 0, int(self.weight), 1)  # K pass 2 (+ weight M p, p.Ap partials)
+            i = 2
         else:
-            nat.call("wpc_sell_stream_grouped", nt, nat.ptr(t[i]), self.cap_sell, a_code, 0, 0, 1,
-                     code, nv, 1.0, s)
-            i += 1
-        nat.call("wpc_cg_fold_grouped", 1, nt, nat.ptr(t[i]), self.cap_fold, code, nv, s)
-        i += 1
-        nat.call("wpc_sell_stream_grouped", nt, nat.ptr(t[i]), self.cap_sell, g_code,
-                 nat.F32 if self.scaled else 0, 0, 0, code, nv, 1.0, s)
-        i += 1
-        nat.call("wpc_sell_stream_grouped", nt, nat.ptr(t[i]), self.cap_sell, g_code, 0, 0, 1, code,
-                 nv, 1.0, s)
-        i += 1
-        nat.call("wpc_cg_fold_grouped", 2, nt, nat.ptr(t[i]), self.cap_fold, code, nv, s)
+            sell(0, a_code, 0, 0, 1)  # A p (+ p.Ap partials)
+            i = 1
+        fold(1, i)  # update
+        sell(i + 1, g_code, nat.F32 if self.scaled else 0, 0, 0)  # G pass
+        sell(i + 2, g_code, 0, 0, 1)  # G^T pass (+ r.z partials)
+        fold(2, i + 3)  # direction
 
     def _capture(self):
+        """Record `steps` steps into a graph on a private stream that never runs
+        work (a stream with pending cross-stream waits is not captured), to be
+        launched on the caller's stream."""
         if self.graph is not None:
             nat.lib().wpc_graph_destroy(self.graph)
             self.graph = None
+        if self._cap_stream is None:
+            self._cap_stream = torch.cuda.Stream()
         handle = ctypes.c_void_p()
-        nat.call("wpc_graph_begin", nat.stream())
-        try:
-            for _ in range(self.steps):
-                self._step()
-        finally:
-            rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
+        with torch.cuda.stream(self._cap_stream):
+            nat.call("wpc_graph_begin", nat.stream())
+            try:
+                for _ in range(self.steps):
+                    self._step()
+            finally:
+                rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
         if rc != 0:
             raise nat.NativeError(f"wpc_graph_end failed: {rc}")
         self.graph = handle.value

@@ -174,6 +174,7 @@ class System:
         self.kind, self.spec, self.alpha, self.stream = kind, spec, float(alpha), stream
         self.policy = cfg.precision[kind]
         self.group = cfg.group
+        self._capture_stream = None
         self.slots = {}
         self.active = 0
         self.bytes = 0
@@ -233,6 +234,12 @@ class System:
         self.P._scratch(self.vdtype, n * 8)
         if isinstance(self.inner, SquaredLaplacianOperator):
             self.inner._scratch(self.vdtype, n * 8)
+
+    def capture_stream(self):
+        """A stream that only records graphs (never runs work)."""
+        if self._capture_stream is None:
+            self._capture_stream = torch.cuda.Stream(self.stream.device)
+        return self._capture_stream
 
     def slot(self, nvec):
         for s in self.slots.setdefault(nvec, []):
@@ -294,19 +301,28 @@ class Slot:
                 self.work.step(self.x_iter)
             self.warm = True
             return
+        # recorded on a private stream (no pending cross-stream waits), launched here
         handle = ctypes.c_void_p()
-        nat.call("wpc_graph_begin", nat.stream())
-        try:
+        launch_stream = nat.stream()
+        recorded = True
+        with torch.cuda.stream(self.system.capture_stream()):
+            nat.call("wpc_graph_begin", nat.stream())
+            try:
+                for _ in range(steps):
+                    self.work.step(self.x_iter)
+            except nat.NativeError:
+                recorded = False
+            finally:
+                rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
+        if not recorded or rc != 0:
+            # invalidated from outside: nothing recorded ran; run this chunk directly
             for _ in range(steps):
                 self.work.step(self.x_iter)
-        finally:
-            rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
-        if rc != 0:
-            raise nat.NativeError(f"wpc_graph_end failed: {rc}")
+            return
         if self.graph is not None:
             nat.lib().wpc_graph_destroy(self.graph)
         self.graph, self.graph_steps = handle.value, steps
-        nat.call("wpc_graph_launch", self.graph, nat.stream())
+        nat.call("wpc_graph_launch", self.graph, launch_stream)
 
     def release(self):
         if self.graph is not None:
