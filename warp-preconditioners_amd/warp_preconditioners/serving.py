@@ -124,12 +124,12 @@ class ServingConfig:
     keep_solutions: bool = True
     # one right-hand side per job, for the one-solve-at-a-time comparison path
     split_rhs: bool = False
-    # advance the CG of the small solves of one kind/format/RHS count together
+    # advance the CG of the solves of one kind and format together
     # (grouped.GroupedCG: one launch per pass for the whole batch).  Off by
-    # default: on the 64-request stream (sides 16-64) batches averaged 2.1 jobs
-    # and tied small solves to their partners' step time (27.0 vs 27.3-28.1
-    # solves/s, p50 2.2-2.6 s vs 1.3-1.6 s); it pays when many same-shape small
-    # solves are in flight at once.
+    # default: on the 64-request stream (sides 16-64) batches averaged 2.1-2.4
+    # jobs, tied small solves to their partners' step time and took them off
+    # their own streams (14.7-27.0 vs 27.2-28.1 solves/s); it is meant for many
+    # same-size small solves in flight at once.
     group: bool = False
 
 
