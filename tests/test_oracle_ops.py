@@ -251,3 +251,34 @@ def test_threaded_pcg_rows_give_identical_iterates(oracle):
             oracle.lib().orc_set_krylov_threads(1)
     np.testing.assert_array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8])
+def test_cg_and_cr_iterates_are_krylov_optimal(oracle, k):
+    # From x0 = 0 the k-th preconditioned iterate lies in x0 + K_k(MA, M r0), M = G^T G,
+    # and is the minimizer there of ||x* - x||_A (CG) or of ||b - A x||_M (CR): a
+    # dropped term or wrong sign in either recurrence breaks the optimality.
+    import scipy.sparse as ss
+
+    from gpu_helpers import random_spd
+
+    rng = np.random.default_rng(1)
+    n = 60
+    a = random_spd(rng, n, 0.1, (0.5, 2))
+    A = oracle.CSR.from_scipy(ss.csr_matrix(a))
+    P = oracle.fsai(A, max_row_size=4)
+    g = P["G"].to_dense()
+    m = g.T @ g
+    b = rng.normal(size=n)
+    v, basis = m @ b, []
+    for _ in range(k):
+        basis.append(v)
+        v = m @ (a @ v)
+    q, _ = np.linalg.qr(np.array(basis).T)
+    best_cg = q @ np.linalg.solve(q.T @ a @ q, q.T @ b)
+    lt = np.linalg.cholesky(m).T
+    y, *_ = np.linalg.lstsq(lt @ a @ q, lt @ b, rcond=None)
+    best_cr = q @ y
+    for method, best in (("cg", best_cg), ("cr", best_cr)):
+        x, _, _ = oracle.krylov(method, b, A=A, P=P, tol=0.0, maxiter=k, check_every=1)
+        np.testing.assert_allclose(x, best, rtol=1e-10, atol=1e-12 * np.abs(best).max())
