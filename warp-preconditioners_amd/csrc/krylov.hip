@@ -261,6 +261,31 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
 // The fields each kernel reads and the fields workgroup 0 writes are disjoint:
 //   update:    reads RZ, RR, TARGET      writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA
 //   direction: reads RZ_OLD, ACTIVE      writes RZ_NEW, RR_NEW, BETA, RZ, RR, ITER
+// State fields F0, F1 (, F2) of all nv vectors, held by the lanes of every
+// wavefront (lane k * nv + v holds field k of vector v) and loaded with ordinary
+// vector loads, so that they travel with a step's other loads (uniform reads
+// would become scalar loads issued only after the rows have arrived).
+template <int F0, int F1, int F2 = -1>
+struct StateLanes {
+    static constexpr int kFields = F2 < 0 ? 2 : 3;
+    double val;
+
+    __device__ __forceinline__ void load(const double* state, int nv) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int k = lane / nv, v = lane - k * nv;
+        const int f = k == 0 ? F0 : (k == 1 ? F1 : F2);
+        val = k < kFields ? state[f * nv + v] : 0.0;
+    }
+
+    __device__ __forceinline__ double get(int k, int nv, int v) const {
+        const int lane = k * nv + v;
+        const long long bits = __double_as_longlong(val);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, lane);
+        const long long hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+        return __longlong_as_double((hi << 32) | lo);
+    }
+};
+
 // Rows each thread loads before the fold, so that the HBM reads overlap the fold's
 // latency (bounded so the prefetched values stay in registers).
 template <int NV> constexpr int kPrefetch = NV <= 2 ? 4 : (NV <= 4 ? 2 : 1);
@@ -275,6 +300,13 @@ __device__ __forceinline__ void cg_update_fold_body(
     const int nv = NV > 0 ? NV : nv_rt;
     long begin, end;
     chunk_of(n, blk, nblocks, &begin, &end);
+    // Loads in flight together: the state fields read, the partials (compile-time
+    // counts), then the prefetched rows.  The fields read and written are disjoint,
+    // so reading them before workgroup 0's writes changes nothing.
+    StateLanes<WPC_KS_RZ, WPC_KS_RR, WPC_KS_TARGET> st;
+    st.load(state, nv);
+    FoldRegs<kKrylovThreads, (NV > 0 ? NV : 1)> fold_a;
+    if constexpr (NV > 0) fold_a.load(part_denom, P);
     T xv[PF][NVM], rv[PF][NVM], pv[PF][NVM], av[PF][NVM];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -290,16 +322,20 @@ __device__ __forceinline__ void cg_update_fold_body(
             }
         }
     }
-    fold_all<kKrylovThreads>(part_denom, P, nv, sums);
+    if constexpr (NV > 0)
+        fold_a.finish(part_denom, P, sums);
+    else
+        fold_all<kKrylovThreads>(part_denom, P, nv, sums);
     T alpha[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; ++v) if (v < nv) {
         const double denom = sums[v];
-        const bool active = state[WPC_KS_RR * nv + v] > state[WPC_KS_TARGET * nv + v];
-        const double a = active ? state[WPC_KS_RZ * nv + v] / denom : 0.0;
+        const double rz = st.get(0, nv, v);
+        const bool active = st.get(1, nv, v) > st.get(2, nv, v);
+        const double a = active ? rz / denom : 0.0;
         alpha[v] = (T)a;
         if (blk == 0 && threadIdx.x == v) {
-            state[WPC_KS_RZ_OLD * nv + v] = state[WPC_KS_RZ * nv + v];
+            state[WPC_KS_RZ_OLD * nv + v] = rz;
             state[WPC_KS_DENOM * nv + v] = denom;
             state[WPC_KS_ACTIVE * nv + v] = active ? 1.0 : 0.0;
             state[WPC_KS_ALPHA * nv + v] = a;
@@ -368,6 +404,14 @@ __device__ __forceinline__ void cg_direction_fold_body(
     const int nv = NV > 0 ? NV : nv_rt;
     long begin, end;
     chunk_of(n, blk, nblocks, &begin, &end);
+    // As in the update: state fields, partials, then rows, all in flight together.
+    StateLanes<WPC_KS_ACTIVE, WPC_KS_RZ_OLD> st;
+    st.load(state, nv);
+    FoldRegs<kKrylovThreads, (NV > 0 ? NV : 1)> fold_z, fold_r;
+    if constexpr (NV > 0) {
+        fold_z.load(part_rz, P_rz);
+        fold_r.load(part_rr, P_rr);
+    }
     T zv[PF][NVM], pv[PF][NVM];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -380,14 +424,19 @@ __device__ __forceinline__ void cg_direction_fold_body(
             }
         }
     }
-    fold_all<kKrylovThreads>(part_rz, P_rz, nv, s_rz);
-    fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
+    if constexpr (NV > 0) {
+        fold_z.finish(part_rz, P_rz, s_rz);
+        fold_r.finish(part_rr, P_rr, s_rr);
+    } else {
+        fold_all<kKrylovThreads>(part_rz, P_rz, nv, s_rz);
+        fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
+    }
     T beta[NVM];
     bool act[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; ++v) if (v < nv) {
-        act[v] = state[WPC_KS_ACTIVE * nv + v] != 0.0;
-        const double b = act[v] ? s_rz[v] / state[WPC_KS_RZ_OLD * nv + v] : 0.0;
+        act[v] = st.get(0, nv, v) != 0.0;
+        const double b = act[v] ? s_rz[v] / st.get(1, nv, v) : 0.0;
         beta[v] = (T)b;
         if (blk == 0 && threadIdx.x == v) {
             state[WPC_KS_RZ_NEW * nv + v] = s_rz[v];

@@ -95,17 +95,13 @@ __device__ inline double fold_block(const double* __restrict__ partials, int P, 
     return out;
 }
 
-// Fixed-order fold of partials[0..P) for all nvec (<= 8) vectors at once by a
-// workgroup of NT threads; results land in out_s[0..nvec) (shared memory).
+// Cross-wave end of the folds below: per-thread sums acc[0..nvec) -> out_s.
 template <int NT>
-__device__ void fold_all(const double* __restrict__ partials, int P, int nvec, double* out_s) {
+__device__ void fold_reduce(const double* acc, int nvec, double* out_s) {
     __shared__ double red[NT / kWave][8];
-    double acc[8];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
-    for (int b = threadIdx.x; b < P; b += NT)
-        for (int v = 0; v < nvec; ++v) acc[v] += partials[(long)b * nvec + v];
-    for (int v = 0; v < nvec; ++v) {
+    for (int v = 0; v < 8; ++v) {
+        if (v >= nvec) break;
         double x = acc[v];
         for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
         if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][v] = x;
@@ -118,5 +114,61 @@ __device__ void fold_all(const double* __restrict__ partials, int P, int nvec, d
     }
     __syncthreads();
 }
+
+// Fixed-order fold of partials[0..P) for all nvec (<= 8) vectors at once by a
+// workgroup of NT threads; results land in out_s[0..nvec) (shared memory).
+// Thread t sums rows t, t + NT, t + 2 NT, ... in that order.
+template <int NT>
+__device__ void fold_all(const double* __restrict__ partials, int P, int nvec, double* out_s) {
+    double acc[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+    for (int b = threadIdx.x; b < P; b += NT)
+        for (int v = 0; v < nvec; ++v) acc[v] += partials[(long)b * nvec + v];
+    fold_reduce<NT>(acc, nvec, out_s);
+}
+
+// Partial counts P * NV up to this many are staged in registers by FoldRegs.
+constexpr int kFoldStaged = 4096;
+
+// The same fold split in two so that its loads overlap other work: load() issues
+// every load of the thread's rows at once (one memory round trip instead of one
+// per row), finish() adds them in fold_all's order (identical results) and
+// reduces across the workgroup.  Rows beyond the staged capacity are read in
+// finish(), so any P is folded correctly.
+template <int NT, int NV>
+struct FoldRegs {
+    static constexpr int K = (kFoldStaged / NV + NT - 1) / NT;
+    double val[K][NV];
+
+    __device__ __forceinline__ void load(const double* __restrict__ partials, int P) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int b = threadIdx.x + k * NT;
+            if (b < P) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) val[k][v] = partials[(long)b * NV + v];
+            }
+        }
+    }
+
+    __device__ __forceinline__ void finish(const double* __restrict__ partials, int P,
+                                           double* out_s) {
+        double acc[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (threadIdx.x + k * NT < P) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] += val[k][v];
+            }
+        }
+        for (int b = threadIdx.x + K * NT; b < P; b += NT)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += partials[(long)b * NV + v];
+        fold_reduce<NT>(acc, NV, out_s);
+    }
+};
 
 }  // namespace wpc
