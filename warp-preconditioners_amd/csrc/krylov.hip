@@ -342,22 +342,30 @@ __device__ __forceinline__ void cg_update_fold_body(
         }
     }
     double acc[NVM];
-#pragma unroll
-    for (int v = 0; v < NVM; ++v) if (v < nv) acc[v] = 0.0;
+    bool moves = false;  // some slot steps: rows are stored whole (a resting slot's
+#pragma unroll           // values unchanged), so the stores of a row merge
+    for (int v = 0; v < NVM; ++v) if (v < nv) {
+        acc[v] = 0.0;
+        moves = moves || alpha[v] != T(0);
+    }
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
         if (i < end) {
+            T xn[NVM], rn[NVM];
 #pragma unroll
             for (int v = 0; v < NVM; ++v) if (v < nv) {
-                const long k = i * nv + v;
-                T rk = rv[j][v];
-                if (alpha[v] != T(0)) {
-                    x[k] = xv[j][v] + alpha[v] * pv[j][v];
-                    rk -= alpha[v] * av[j][v];
-                    r[k] = rk;
+                const bool step = alpha[v] != T(0);
+                xn[v] = step ? xv[j][v] + alpha[v] * pv[j][v] : xv[j][v];
+                rn[v] = step ? rv[j][v] - alpha[v] * av[j][v] : rv[j][v];
+                acc[v] += (double)rn[v] * (double)rn[v];
+            }
+            if (moves) {
+#pragma unroll
+                for (int v = 0; v < NVM; ++v) if (v < nv) {
+                    x[i * nv + v] = xn[v];
+                    r[i * nv + v] = rn[v];
                 }
-                acc[v] += (double)rk * (double)rk;
             }
         }
     }
@@ -449,13 +457,18 @@ __device__ __forceinline__ void cg_direction_fold_body(
             }
         }
     }
+    bool moves = false;  // as in the update: whole rows, stored when some slot steps
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {
-        const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
-        if (i < end) {
+    for (int v = 0; v < NVM; ++v) if (v < nv) moves = moves || act[v];
+    if (moves) {
 #pragma unroll
-            for (int v = 0; v < NVM; ++v) if (v < nv)
-                if (act[v]) p[i * nv + v] = zv[j][v] + beta[v] * pv[j][v];
+        for (int j = 0; j < PF; ++j) {
+            const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
+            if (i < end) {
+#pragma unroll
+                for (int v = 0; v < NVM; ++v) if (v < nv)
+                    p[i * nv + v] = act[v] ? zv[j][v] + beta[v] * pv[j][v] : pv[j][v];
+            }
         }
     }
     for (long i = begin + threadIdx.x + (long)PF * kKrylovThreads; i < end; i += kKrylovThreads) {

@@ -20,21 +20,57 @@ __device__ __forceinline__ int xcd_block() {
     return x * per + min(x, extra) + q;
 }
 
-// Deterministic block sum of v[NV] (all threads) into red[0..NV) (thread 0 reads).
+// 64-bit DPP move (both halves with the same lane pattern CTRL).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const long long hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((hi << 32) | lo);
+}
+
+__device__ __forceinline__ double read_lane(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);
+    const long long hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((hi << 32) | lo);
+}
+
+// Sum of x over the 64 lanes of a wavefront (all lanes active), the same value
+// in every lane: pair and quad sums (DPP quad permutations), 8- and 16-lane rows
+// (DPP half-row and row mirrors) -- VALU moves, no LDS round trips -- and the
+// four row sums as (r0 + r1) + (r2 + r3).  Every step adds the same two values
+// in both lanes it pairs, so all lanes agree bit for bit.
+__device__ __forceinline__ double wave_sum(double x) {
+    x += dpp_move<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dpp_move<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dpp_move<0x141>(x);  // row_half_mirror
+    x += dpp_move<0x140>(x);  // row_mirror
+    return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
+}
+
+constexpr int kBlockWaves = kStreamRows / kWave;
+
+// Deterministic block sum of v[NV] (all threads) into v[0..NV) of thread 0.
+// red: kStreamRows doubles of scratch (NV * kBlockWaves are used, NV <= 16).
 template <int NV>
 __device__ void block_sum(double* v, double* red) {
-    // red: kStreamRows doubles of scratch
+    static_assert(NV * kBlockWaves <= kStreamRows, "scratch too small");
+    double s[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double x = v[q];
-        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
-        __syncthreads();
-        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = x;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int w = 0; w < kStreamRows / kWave; ++w) s += red[w];
-            v[q] = s;
+    for (int q = 0; q < NV; ++q) s[q] = wave_sum(v[q]);
+    __syncthreads();  // red may still be read by an earlier use
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) red[q * kBlockWaves + threadIdx.x / kWave] = s[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double t = 0.0;
+            for (int w = 0; w < kBlockWaves; ++w) t += red[q * kBlockWaves + w];
+            v[q] = t;
         }
     }
 }
@@ -56,18 +92,23 @@ __device__ __forceinline__ void write_partials(const double* v, double* partials
 // over systems with different right-hand-side counts); per vector they perform
 // exactly the operations of block_sum<NV> / write_partials<NV>.
 __device__ inline void block_sum_n(double* v, int nv, double* red) {
+    double s[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        if (q >= nv) break;
-        double x = v[q];
-        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
-        __syncthreads();
-        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = x;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int w = 0; w < kStreamRows / kWave; ++w) s += red[w];
-            v[q] = s;
+    for (int q = 0; q < 8; ++q) s[q] = q < nv ? wave_sum(v[q]) : 0.0;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q < nv) red[q * kBlockWaves + threadIdx.x / kWave] = s[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q >= nv) break;
+            double t = 0.0;
+            for (int w = 0; w < kBlockWaves; ++w) t += red[q * kBlockWaves + w];
+            v[q] = t;
         }
     }
 }
@@ -102,8 +143,7 @@ __device__ void fold_reduce(const double* acc, int nvec, double* out_s) {
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
         if (v >= nvec) break;
-        double x = acc[v];
-        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
+        const double x = wave_sum(acc[v]);
         if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][v] = x;
     }
     __syncthreads();

@@ -156,8 +156,7 @@ __global__ __launch_bounds__(1024) void dd_fold_kernel(int P, int nq,
     for (int b = threadIdx.x; b < P; b += 1024)
         for (int k = 0; k < nq; ++k) acc[k] += partials[(long)b * nq + k];
     for (int k = 0; k < nq; ++k) {
-        double s = acc[k];
-        for (int m = kWave / 2; m > 0; m >>= 1) s += __shfl_xor(s, m, kWave);
+        const double s = wave_sum(acc[k]);
         if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][k] = s;
     }
     __syncthreads();

@@ -489,8 +489,7 @@ __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
     for (int b = threadIdx.x; b < P; b += 1024)
         for (int v = 0; v < nvec; ++v) acc[v] += partials[(long)b * nvec + v];
     for (int v = 0; v < nvec; ++v) {
-        double x = acc[v];
-        for (int m = kWave / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, kWave);
+        const double x = wave_sum(acc[v]);
         if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][v] = x;
     }
     __syncthreads();
