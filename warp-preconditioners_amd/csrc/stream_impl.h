@@ -282,8 +282,8 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 // the slice sits at (slice_ptr[s] + k) * 64 + r -- and every row is padded to the
 // slice's longest row with zero values on a column the row already holds.
 // Position i holds row rows[i] (rows == NULL: row i).  One thread per row reads
-// fully coalesced index/value streams, issues its gathers four entries at a time,
-// and sums in storage order, so results equal the streamed mapping's bit for bit
+// fully coalesced index/value streams, issues its gathers a group of entries at a
+// time, and sums in storage order, so results equal the streamed mapping's bit for bit
 // (padding adds exact zeros).  Same epilogue and fused inner product.
 // Column index of entry p of the row `row`: stored as int32, or as the int16
 // difference col - row when every entry of the matrix is within 32767 of its row
@@ -300,39 +300,50 @@ template <> __device__ __forceinline__ long sell_col<short>(const short* col, lo
 // batched launch reads per system.  Both perform the same operations per vector.
 template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
 
+// Entries of a row handled per round trip: a group's index and value loads go out
+// together, then its gathers.  A slice's length (uniform across the wavefront) is
+// covered by groups of 8 and then one each of 4, 2 and 1 as needed, so no entry
+// past the slice is read and at most four round trips follow the full groups.
+template <class TI, class TV, class TX, int NV, int G>
+__device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long base, int k, int row,
+                                           const TI* __restrict__ col,
+                                           const TV* __restrict__ val, const TX* __restrict__ x,
+                                           int nv) {
+    using TA = typename AccOf<TX>::type;
+    constexpr int NVM = kNvMax<NV>;
+    long c[G];
+    TA a[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const long p = base + (long)(k + j) * 64;
+        c[j] = sell_col<TI>(col, p, row);
+        a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+#pragma unroll
+        for (int v = 0; v < NVM; ++v)
+            if (v < nv) acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * nv + v);
+    }
+}
+
 template <class TI, class TV, class TX, int NV>
 __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long base, int len,
                                            int row, const TI* __restrict__ col,
                                            const TV* __restrict__ val, const TX* __restrict__ x,
                                            int nv_rt) {
-    using TA = typename AccOf<TX>::type;
-    constexpr int NVM = kNvMax<NV>;
     const int nv = NV > 0 ? NV : nv_rt;
     int k = 0;
-    for (; k + 4 <= len; k += 4) {
-        long c[4];
-        TA a[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long p = base + (long)(k + j) * 64;
-            c[j] = sell_col<TI>(col, p, row);
-            a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-#pragma unroll
-            for (int v = 0; v < NVM; ++v)
-                if (v < nv) acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * nv + v);
-        }
+    for (; k + 8 <= len; k += 8) sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
+    if (k + 4 <= len) {
+        sell_group<TI, TV, TX, NV, 4>(acc, base, k, row, col, val, x, nv);
+        k += 4;
     }
-    for (; k < len; ++k) {
-        const long p = base + (long)k * 64;
-        const long c = sell_col<TI>(col, p, row);
-        const TA a = lds_val<TA, TV>(raw_val<TV>(val, p));
-#pragma unroll
-        for (int v = 0; v < NVM; ++v)
-            if (v < nv) acc[v] += a * St<TX>::template ld<TA>(x, c * nv + v);
+    if (k + 2 <= len) {
+        sell_group<TI, TV, TX, NV, 2>(acc, base, k, row, col, val, x, nv);
+        k += 2;
     }
+    if (k < len) sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
 }
 
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
