@@ -260,27 +260,51 @@ class StreamPlan:
     def _tune(self, nvec, dtype, reps=10, rounds=3):
         """Time every candidate mapping; candidates are interleaved over `rounds`
         rounds and each keeps its best round, so a transient stall on the device
-        does not decide the choice."""
+        does not decide the choice.  Each candidate's `reps` launches are recorded
+        into a graph and replayed, as the solvers replay their iterations, so the
+        host's launch cost (which can exceed a small product's run time) does not
+        hide the difference between mappings."""
         n = self.A.nrow
         x = torch.ones(n * nvec, dtype=dtype, device=self.A.device)
         z = torch.empty_like(x)
-        stream = torch.cuda.current_stream()
-        for m in self.CANDIDATES:
-            for _ in range(2):
-                self._launch(m, x, None, z, 1.0, 0.0, nvec)
-        best, best_t = 0, float("inf")
-        for _ in range(rounds):
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        graphs = {}
+        with torch.cuda.stream(side):
             for m in self.CANDIDATES:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                for _ in range(reps):
+                for _ in range(2):
                     self._launch(m, x, None, z, 1.0, 0.0, nvec)
-                e1.record(stream)
-                e1.synchronize()
-                t = e0.elapsed_time(e1) / reps
-                key = (nvec, dtype, m)
-                self.timings[key] = min(t, self.timings.get(key, float("inf")))
+        side.synchronize()
+        try:
+            for m in self.CANDIDATES:
+                g = torch.cuda.CUDAGraph()
+                # (thread-local: work other threads issue meanwhile is not disturbed)
+                with torch.cuda.graph(g, stream=side, capture_error_mode="thread_local"):
+                    for _ in range(reps):
+                        self._launch(m, x, None, z, 1.0, 0.0, nvec)
+                graphs[m] = g
+        except RuntimeError:
+            graphs = {}  # capture refused: time direct launches instead
+            side.synchronize()
+        with torch.cuda.stream(side):
+            for _ in range(rounds):
+                for m in self.CANDIDATES:
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(side)
+                    if graphs:
+                        graphs[m].replay()
+                    else:
+                        for _ in range(reps):
+                            self._launch(m, x, None, z, 1.0, 0.0, nvec)
+                    e1.record(side)
+                    e1.synchronize()
+                    t = e0.elapsed_time(e1) / reps
+                    key = (nvec, dtype, m)
+                    self.timings[key] = min(t, self.timings.get(key, float("inf")))
+        torch.cuda.current_stream().wait_stream(side)
+        del graphs
+        best, best_t = 0, float("inf")
         for m in self.CANDIDATES:
             t = self.timings[(nvec, dtype, m)]
             if t < best_t:
