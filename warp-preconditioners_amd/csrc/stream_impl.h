@@ -302,28 +302,37 @@ template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
 
 // Entries of a row handled per round trip: a group's index and value loads go out
 // together, then its gathers.  A slice's length (uniform across the wavefront) is
-// covered by groups of 8 and then one each of 4, 2 and 1 as needed, so no entry
-// past the slice is read and at most four round trips follow the full groups.
-template <class TI, class TV, class TX, int NV, int G>
+// covered by groups of 8, then the remainder r < 8: r = 6, 7 as one masked group
+// of 8, r = 3 as a masked group of 4, otherwise groups of 4, 2 and 1.  A masked
+// group's entries past the slice re-read its last entry (a cache hit; nothing past
+// the slice is read) and their products are discarded by a select, so the sums are
+// those of storage order.  Masking only where it wastes at most two gathers keeps
+// the gathers (which bound these passes) close to the stored entries.
+template <class TI, class TV, class TX, int NV, int G, bool FULL = true>
 __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long base, int k, int row,
                                            const TI* __restrict__ col,
                                            const TV* __restrict__ val, const TX* __restrict__ x,
-                                           int nv) {
+                                           int nv, int m = G) {
     using TA = typename AccOf<TX>::type;
     constexpr int NVM = kNvMax<NV>;
     long c[G];
     TA a[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        const long p = base + (long)(k + j) * 64;
+        const int e = FULL || j < m ? j : m - 1;
+        const long p = base + (long)(k + e) * 64;
         c[j] = sell_col<TI>(col, p, row);
         a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
     }
 #pragma unroll
     for (int j = 0; j < G; ++j) {
 #pragma unroll
-        for (int v = 0; v < NVM; ++v)
-            if (v < nv) acc[v] += a[j] * St<TX>::template ld<TA>(x, c[j] * nv + v);
+        for (int v = 0; v < NVM; ++v) {
+            if (v < nv) {
+                const TA t = acc[v] + a[j] * St<TX>::template ld<TA>(x, c[j] * nv + v);
+                acc[v] = FULL || j < m ? t : acc[v];
+            }
+        }
     }
 }
 
@@ -335,15 +344,26 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
     const int nv = NV > 0 ? NV : nv_rt;
     int k = 0;
     for (; k + 8 <= len; k += 8) sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
-    if (k + 4 <= len) {
+    int r = len - k;
+    if (r >= 6) {
+        sell_group<TI, TV, TX, NV, 8, false>(acc, base, k, row, col, val, x, nv, r);
+        return;
+    }
+    if (r >= 4) {
         sell_group<TI, TV, TX, NV, 4>(acc, base, k, row, col, val, x, nv);
         k += 4;
+        r -= 4;
     }
-    if (k + 2 <= len) {
+    if (r == 3) {
+        sell_group<TI, TV, TX, NV, 4, false>(acc, base, k, row, col, val, x, nv, 3);
+        return;
+    }
+    if (r >= 2) {
         sell_group<TI, TV, TX, NV, 2>(acc, base, k, row, col, val, x, nv);
         k += 2;
+        r -= 2;
     }
-    if (k < len) sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
+    if (r == 1) sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
 }
 
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
@@ -359,15 +379,28 @@ __device__ __forceinline__ void sell_body(
     const int nv = NV > 0 ? NV : nv_rt;
     const int idx = blk * kStreamRows + threadIdx.x;
     const bool live = idx < n;
-    const int row = live && rows != nullptr ? rows[idx] : idx;
+    // Loads without branches (lanes past n read row n-1's data and discard it), so
+    // the slice header and the epilogue operands are in flight together.
+    const int idc = live ? idx : n - 1;
+    const int row = rows != nullptr ? rows[idc] : idc;
     const bool use_a = alpha != TA(0);
+    const int slice = idc >> 6;
+    const long base = (long)slice_ptr[slice] * 64 + (idx & 63);
+    const int len = live && use_a ? slice_len[slice] : 0;
+    TA sv = TA(1), dv = TA(0), uv[NVM], wv[NVM];
+    if (HS) sv = (TA)s[row];
+    if (HD) dv = St<TX>::template ld<TA>(d, row);
+#pragma unroll
+    for (int v = 0; v < NVM; ++v) {
+        if (v < nv) {
+            if (HD) uv[v] = St<TX>::template ld<TA>(u, (long)row * nv + v);
+            if (DOT) wv[v] = St<TX>::template ld<TA>(w, (long)row * nv + v);
+        }
+    }
     TA acc[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; ++v) acc[v] = TA(0);
     if (live && use_a) {
-        const int slice = idx >> 6;
-        const long base = (long)slice_ptr[slice] * 64 + (idx & 63);
-        const int len = slice_len[slice];
         if (col16)
             sell_accum<short, TV, TX, NV>(acc, base, len, row, (const short*)col, val, x, nv);
         else
@@ -384,13 +417,13 @@ __device__ __forceinline__ void sell_body(
             TA r = TA(0);
             if (use_a) {
                 r = acc[v];
-                if (HS) r = (TA)s[row] * r;
-                if (HD) r += St<TX>::template ld<TA>(d, row) * St<TX>::template ld<TA>(u, k);
+                if (HS) r = sv * r;
+                if (HD) r += dv * uv[v];
                 r *= alpha;
             }
             if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
             St<TX>::template st<TA>(z, k, r);
-            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) * (double)St<TX>::template ld<TA>(z, k);
+            if (DOT) dot[v] = (double)wv[v] * (double)St<TX>::template ld<TA>(z, k);
         }
     }
     if (DOT) {
