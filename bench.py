@@ -526,8 +526,8 @@ def block_bench(name):
 def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8), reps=50):
     """FSAI-apply HBM throughput: z = G^T (G x) with the width-8 factor of the
     assembled biharmonic Q (device-assembled on the jittered grid, float32 factor,
-    float64 vectors), both passes timed together with HIP events on the launch
-    stream; algorithmic bytes 2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (4-byte
+    float64 vectors), both passes timed together: `reps` applies recorded into a
+    graph and replayed, HIP events on the replay stream; algorithmic bytes 2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (4-byte
     column indices, whatever the layout stores: `index_bytes` reports 2 where a pass
     reads int16 column differences).  Prints one JSON line."""
     import torch
@@ -547,16 +547,27 @@ def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8), reps=50):
         for k in nvecs:
             x = torch.randn(n * k, dtype=torch.float64, device=dev)
             z = torch.empty_like(x)
-            for _ in range(3):
-                P.apply_batch(x, x, z, 1.0, 0.0, k)
-            stream = torch.cuda.current_stream()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                P.apply_batch(x, x, z, 1.0, 0.0, k)
-            e1.record(stream)
-            e1.synchronize()
-            us = e0.elapsed_time(e1) / reps * 1e3
+            rs = torch.cuda.Stream()
+            rs.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(rs):
+                for _ in range(3):
+                    P.apply_batch(x, x, z, 1.0, 0.0, k)
+            rs.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=rs):
+                for _ in range(reps):
+                    P.apply_batch(x, x, z, 1.0, 0.0, k)
+            us = float("inf")
+            with torch.cuda.stream(rs):
+                for _ in range(3):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(rs)
+                    graph.replay()
+                    e1.record(rs)
+                    e1.synchronize()
+                    us = min(us, e0.elapsed_time(e1) / reps * 1e3)
+            del graph
             pg, pgt = P.plans()
             ib = [2 if (pl.mode(k, torch.float64) == pl.SLICED and pl._sell is not None
                         and pl._sell[6]) else 4 for pl in (pg, pgt)]

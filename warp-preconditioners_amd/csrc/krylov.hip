@@ -432,7 +432,16 @@ __device__ __forceinline__ void cg_direction_fold_body(
             }
         }
     }
-    if constexpr (NV > 0) {
+    if constexpr (NV > 0 && NV <= 4) {
+        // both folds through one cross-wavefront pass (one barrier pair, not two)
+        double acc[8];
+        __shared__ double both[8];
+        fold_z.sums(part_rz, P_rz, acc);
+        fold_r.sums(part_rr, P_rr, acc + NV);
+        fold_reduce<kKrylovThreads>(acc, 2 * NV, both);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { s_rz[v] = both[v]; s_rr[v] = both[NV + v]; }
+    } else if constexpr (NV > 0) {
         fold_z.finish(part_rz, P_rz, s_rz);
         fold_r.finish(part_rr, P_rr, s_rr);
     } else {

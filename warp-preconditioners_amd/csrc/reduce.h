@@ -192,11 +192,11 @@ struct FoldRegs {
         }
     }
 
-    __device__ __forceinline__ void finish(const double* __restrict__ partials, int P,
-                                           double* out_s) {
-        double acc[8];
+    // Per-thread sums acc[0..NV) of the staged rows (and of any rows beyond the
+    // staged capacity), in fold_all's order.
+    __device__ __forceinline__ void sums(const double* __restrict__ partials, int P, double* acc) {
 #pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (threadIdx.x + k * NT < P) {
@@ -207,6 +207,12 @@ struct FoldRegs {
         for (int b = threadIdx.x + K * NT; b < P; b += NT)
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[v] += partials[(long)b * NV + v];
+    }
+
+    __device__ __forceinline__ void finish(const double* __restrict__ partials, int P,
+                                           double* out_s) {
+        double acc[8];
+        sums(partials, P, acc);
         fold_reduce<NT>(acc, NV, out_s);
     }
 };
