@@ -240,3 +240,52 @@ def test_sliced_int16_columns_and_fallback(oracle, far):
     ref = np.stack([oracle.spmv(oracle_csr(oracle, a), np.ascontiguousarray(x[:, v]))
                     for v in range(2)], axis=1)
     np.testing.assert_array_equal(host(z).reshape(n, 2), ref)
+
+
+@pytest.mark.parametrize("col16", [True, False])
+def test_sliced_every_slice_length_bitwise(oracle, monkeypatch, col16):
+    # Slices of every length 0..24 (the row in each slice's first position is the
+    # longest, the others random and shorter): every remainder of the entry groups
+    # (full groups of 8, masked groups of 8 and 4, groups of 4, 2, 1) against the
+    # oracle's serial product, bit for bit, on int16 and int32 columns; with three
+    # right-hand sides and the fused epilogue against the streamed mapping.
+    from warp_preconditioners.sparse import StreamPlan
+
+    monkeypatch.setattr(StreamPlan, "COL16", col16)
+    monkeypatch.setattr(StreamPlan, "SORT_GAIN", 1e9)
+    rng = np.random.default_rng(24)
+    n = 25 * 64 + 13
+    rows, cols = [], []
+    for r in range(n):
+        top = (r // 64) % 25
+        m = top if r % 64 == 0 else int(rng.integers(0, top + 1))
+        lo, hi = max(0, r - 300), min(n, r + 300)
+        rows += [r] * m
+        cols += sorted(rng.choice(np.arange(lo, hi), size=m, replace=False).tolist())
+    a = ss.csr_matrix((rng.normal(size=len(rows)), (rows, cols)), shape=(n, n))
+    a.sort_indices()
+    A = upload(a)
+    plan = StreamPlan(A, lanes=StreamPlan.SLICED)
+    x = rng.normal(size=n)
+    y = rng.normal(size=n)
+    z = torch.empty(n, dtype=torch.float64, device=dev())
+    plan.run(torch.tensor(x, device=dev()), torch.tensor(y, device=dev()), z, 0.7, -0.3)
+    assert bool(plan._sell[6]) == col16
+    lens = host(plan._sliced()[1])
+    assert set(range(25)) <= set(lens.tolist())
+    np.testing.assert_array_equal(host(z), oracle.spmv(oracle_csr(oracle, a, np.float64), x, y, 0.7, -0.3))
+    k = 3
+    X = torch.tensor(rng.normal(size=(n, k)), device=dev())
+    U = torch.tensor(rng.normal(size=(n, k)), device=dev())
+    S = torch.tensor(rng.uniform(0.5, 2, size=n), dtype=torch.float32, device=dev())
+    D = torch.tensor(rng.uniform(0.5, 2, size=n), device=dev())
+    outs = []
+    for mode in (0, StreamPlan.SLICED):
+        p = StreamPlan(A, lanes=mode)
+        Z = torch.empty_like(X)
+        dots = torch.zeros(k, dtype=torch.float64, device=dev())
+        p.run(X.reshape(-1), None, Z.reshape(-1), 0.9, 0.0, k, s=S, d=D, u=U.reshape(-1),
+              w=X.reshape(-1), out=dots)
+        outs.append((host(Z), host(dots)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-12)

@@ -29,6 +29,21 @@ __device__ __forceinline__ double dpp_move(double x) {
     return __longlong_as_double((hi << 32) | lo);
 }
 
+// x of the lane whose index differs in bit log2(M) (M = 1 or 2, inside a quad):
+// the DPP form of __shfl_xor(x, M) with no LDS round trip.
+template <int M>
+__device__ __forceinline__ double quad_xor(double x) {
+    static_assert(M == 1 || M == 2, "quad strides only");
+    return dpp_move<M == 1 ? 0xB1 : 0x4E>(x);
+}
+
+template <int M>
+__device__ __forceinline__ float quad_xor(float x) {
+    static_assert(M == 1 || M == 2, "quad strides only");
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(x), M == 1 ? 0xB1 : 0x4E, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ double read_lane(double x, int lane) {
     const long long b = __double_as_longlong(x);
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);

@@ -236,21 +236,59 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
     const int sub = threadIdx.x & (L - 1);
     const bool live = row < n;
     const bool use_a = alpha != TA(0);
+    // The row's extent and the epilogue operands are loaded together, without
+    // branches (lanes past n read row n-1's and discard them).
+    const long rc = live ? row : n - 1;
+    const int o0 = off[rc], o1 = off[rc + 1];
+    const int e0 = o0, e1 = live && use_a ? o1 : o0;
+    TA sv = TA(1), dv = TA(0), uv[NV], wv[NV];
+    if (HS) sv = (TA)s[rc];
+    if (HD) dv = St<TX>::template ld<TA>(d, rc);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (HD) uv[v] = St<TX>::template ld<TA>(u, rc * NV + v);
+        if (DOT) wv[v] = St<TX>::template ld<TA>(w, rc * NV + v);
+    }
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    if (live && use_a) {
-        const int e1 = off[row + 1];
-        for (int e = off[row] + sub; e < e1; e += L) {
-            const long c = col[e];
-            const TA av = lds_val<TA, TV>(raw_val<TV>(val, e));
+    // Two of the lane's entries per round trip (both index/value loads, then both
+    // gathers); each lane still sums its entries e0+sub, e0+sub+L, ... in order.
+    int e = e0 + sub;
+    for (; e + L < e1; e += 2 * L) {
+        const long c0 = col[e], c1 = col[e + L];
+        const TA a0 = lds_val<TA, TV>(raw_val<TV>(val, e));
+        const TA a1 = lds_val<TA, TV>(raw_val<TV>(val, e + L));
+        TA g0[NV], g1[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] += av * St<TX>::template ld<TA>(x, c * NV + v);
+        for (int v = 0; v < NV; ++v) {
+            g0[v] = St<TX>::template ld<TA>(x, c0 * NV + v);
+            g1[v] = St<TX>::template ld<TA>(x, c1 * NV + v);
         }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += a0 * g0[v];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += a1 * g1[v];
     }
-    for (int m = L >> 1; m > 0; m >>= 1) {
+    if (e < e1) {
+        const long c = col[e];
+        const TA av = lds_val<TA, TV>(raw_val<TV>(val, e));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += av * St<TX>::template ld<TA>(x, c * NV + v);
+    }
+    // Lane reduction, xor butterfly from the widest stride down; strides 2 and 1
+    // stay inside a quad and run as DPP quad permutations (the same pairs).
+    for (int m = L >> 1; m > 2; m >>= 1) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] += shfl_xor(acc[v], m, kWave);
+    }
+    if (L >= 4) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += quad_xor<2>(acc[v]);
+    }
+    if (L >= 2) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += quad_xor<1>(acc[v]);
     }
     double dot[NV];
 #pragma unroll
@@ -262,13 +300,13 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
             TA r = TA(0);
             if (use_a) {
                 r = acc[v];
-                if (HS) r = (TA)s[row] * r;
-                if (HD) r += St<TX>::template ld<TA>(d, row) * St<TX>::template ld<TA>(u, k);
+                if (HS) r = sv * r;
+                if (HD) r += dv * uv[v];
                 r *= alpha;
             }
             if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
             St<TX>::template st<TA>(z, k, r);
-            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) * (double)St<TX>::template ld<TA>(z, k);
+            if (DOT) dot[v] = (double)wv[v] * (double)St<TX>::template ld<TA>(z, k);
         }
     }
     if (DOT) {
