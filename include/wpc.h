@@ -575,6 +575,10 @@ int wpc_graph_begin(void* stream);
 int wpc_graph_end(void* stream, void** graph_exec);
 int wpc_graph_launch(void* graph_exec, void* stream);
 int wpc_graph_destroy(void* graph_exec);
+/* A non-blocking stream of the caller's own, default priority (a framework's
+ * pooled streams can alias; a capture needs a stream no other work uses). */
+int wpc_stream_create(void** stream);
+int wpc_stream_destroy(void* stream);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,27 @@ int wpc_graph_launch(void* graph_exec, void* stream) {
     return e == hipSuccess ? WPC_OK : (int)e;
 }
 
+// A stream of the caller's own (non-blocking, default priority).  Frameworks that
+// hand out streams from a shared pool can give two owners the same stream; a
+// capture must run on a stream that no other work uses.
+int wpc_stream_create(void** stream) {
+    if (stream == nullptr) return WPC_EARG;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return (int)e;
+    }
+    *stream = (void*)s;
+    return WPC_OK;
+}
+
+int wpc_stream_destroy(void* stream) {
+    if (stream == nullptr) return WPC_OK;
+    hipError_t e = hipStreamDestroy(as_stream(stream));
+    return e == hipSuccess ? WPC_OK : (int)e;
+}
+
 int wpc_graph_destroy(void* graph_exec) {
     if (graph_exec == nullptr) return WPC_OK;
     hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph_exec);

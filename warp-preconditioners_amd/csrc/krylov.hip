@@ -50,7 +50,9 @@ __device__ void block_partial(const double* acc, double* red, double* partials) 
 }
 
 __device__ __forceinline__ void chunk_of(int n, int blk, int nblocks, long* begin, long* end) {
-    const long chunk = ((long)n + nblocks - 1) / nblocks;
+    // (32-bit unsigned: n < 2^31, so n + nblocks - 1 < 2^32; a 64-bit division
+    // here costs a long scalar sequence ahead of every load)
+    const unsigned chunk = ((unsigned)n + (unsigned)nblocks - 1u) / (unsigned)nblocks;
     *begin = (long)blk * chunk;
     *end = *begin + chunk < n ? *begin + chunk : n;
 }

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import re
+import threading
 from pathlib import Path
 
 import torch
@@ -93,6 +94,39 @@ def ptr(t):
 
 def stream():
     return torch.cuda.current_stream().cuda_stream
+
+
+class PrivateStream:
+    """A HIP stream of one owner's own (wpc_stream_create), usable as a torch stream
+    (``.torch``).  torch hands out streams from a pool of 32 per device, so two
+    torch streams can be the same HIP stream; a stream that graphs are captured on
+    must be one that no other work (another thread's, another system's) runs on."""
+
+    def __init__(self, device=None):
+        handle = ctypes.c_void_p()
+        call("wpc_stream_create", ctypes.addressof(handle))
+        self.handle = handle.value
+        self.torch = torch.cuda.ExternalStream(self.handle, device=device)
+
+    def release(self):
+        if self.handle is not None:
+            lib().wpc_stream_destroy(self.handle)
+            self.handle = None
+
+
+_capture_streams = threading.local()
+
+
+def capture_stream() -> torch.cuda.Stream:
+    """This thread's private capture stream on the current device (created once,
+    kept for the process)."""
+    dev = torch.cuda.current_device()
+    cache = getattr(_capture_streams, "by_device", None)
+    if cache is None:
+        cache = _capture_streams.by_device = {}
+    if dev not in cache:
+        cache[dev] = PrivateStream(dev)
+    return cache[dev].torch
 
 
 def require_device(t: torch.Tensor, what: str = "array"):

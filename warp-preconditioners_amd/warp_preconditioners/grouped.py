@@ -116,7 +116,6 @@ class GroupedCG:
         self._ring_next = 0
         self.warm = False
         self.capture_failures = 0
-        self._cap_stream = None
 
     # ------------------------------------------------------------ tables
     def _passes(self, works):
@@ -221,16 +220,15 @@ class GroupedCG:
         fold(2, i + 3)  # direction
 
     def _capture(self):
-        """Record `steps` steps into a graph on a private stream that never runs
-        work (a stream with pending cross-stream waits is not captured), to be
-        launched on the caller's stream."""
+        """Record `steps` steps into a graph on this thread's private capture
+        stream, which never runs work (a stream with pending cross-stream waits, or
+        one another owner shares, is not captured cleanly), to be launched on the
+        caller's stream."""
         if self.graph is not None:
             nat.lib().wpc_graph_destroy(self.graph)
             self.graph = None
-        if self._cap_stream is None:
-            self._cap_stream = torch.cuda.Stream()
         handle = ctypes.c_void_p()
-        with torch.cuda.stream(self._cap_stream):
+        with torch.cuda.stream(nat.capture_stream()):
             nat.call("wpc_graph_begin", nat.stream())
             try:
                 for _ in range(self.steps):

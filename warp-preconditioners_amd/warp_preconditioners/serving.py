@@ -174,7 +174,6 @@ class System:
         self.kind, self.spec, self.alpha, self.stream = kind, spec, float(alpha), stream
         self.policy = cfg.precision[kind]
         self.group = cfg.group
-        self._capture_stream = None
         self.slots = {}
         self.active = 0
         self.bytes = 0
@@ -236,10 +235,10 @@ class System:
             self.inner._scratch(self.vdtype, n * 8)
 
     def capture_stream(self):
-        """A stream that only records graphs (never runs work)."""
-        if self._capture_stream is None:
-            self._capture_stream = torch.cuda.Stream(self.stream.device)
-        return self._capture_stream
+        """A stream that only records graphs (never runs work): the capturing
+        thread's private stream (a torch stream could be another system's or the
+        setup stream, torch hands them out from a shared pool)."""
+        return nat.capture_stream()
 
     def slot(self, nvec):
         for s in self.slots.setdefault(nvec, []):

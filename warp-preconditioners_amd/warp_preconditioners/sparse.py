@@ -12,6 +12,8 @@ always compact and canonical: sorted columns, no duplicate blocks.
 
 from __future__ import annotations
 
+import ctypes
+
 from dataclasses import dataclass
 
 import numpy as np
@@ -260,50 +262,58 @@ class StreamPlan:
     def _tune(self, nvec, dtype, reps=10, rounds=3):
         """Time every candidate mapping; candidates are interleaved over `rounds`
         rounds and each keeps its best round, so a transient stall on the device
-        does not decide the choice.  Each candidate's `reps` launches are recorded
-        into a graph and replayed, as the solvers replay their iterations, so the
-        host's launch cost (which can exceed a small product's run time) does not
-        hide the difference between mappings."""
+        does not decide the choice.  Each candidate's `reps` launches are captured
+        into a graph (the library's thread-local capture: no device-wide sync) and
+        replayed, as the solvers replay their iterations, so the host's launch cost
+        (which can exceed a small product's run time) does not hide the difference
+        between mappings; direct launches are timed if a capture fails."""
         n = self.A.nrow
         x = torch.ones(n * nvec, dtype=dtype, device=self.A.device)
         z = torch.empty_like(x)
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        graphs = {}
-        with torch.cuda.stream(side):
+        work = torch.cuda.Stream()
+        work.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(work):
             for m in self.CANDIDATES:
                 for _ in range(2):
                     self._launch(m, x, None, z, 1.0, 0.0, nvec)
-        side.synchronize()
-        try:
+        work.synchronize()
+        # captured on this thread's private stream, which runs nothing
+        graphs = {}
+        with torch.cuda.stream(nat.capture_stream()):
             for m in self.CANDIDATES:
-                g = torch.cuda.CUDAGraph()
-                # (thread-local: work other threads issue meanwhile is not disturbed)
-                with torch.cuda.graph(g, stream=side, capture_error_mode="thread_local"):
-                    for _ in range(reps):
-                        self._launch(m, x, None, z, 1.0, 0.0, nvec)
-                graphs[m] = g
-        except RuntimeError:
-            graphs = {}  # capture refused: time direct launches instead
-            side.synchronize()
-        with torch.cuda.stream(side):
-            for _ in range(rounds):
-                for m in self.CANDIDATES:
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e0.record(side)
-                    if graphs:
-                        graphs[m].replay()
-                    else:
-                        for _ in range(reps):
-                            self._launch(m, x, None, z, 1.0, 0.0, nvec)
-                    e1.record(side)
-                    e1.synchronize()
-                    t = e0.elapsed_time(e1) / reps
-                    key = (nvec, dtype, m)
-                    self.timings[key] = min(t, self.timings.get(key, float("inf")))
-        torch.cuda.current_stream().wait_stream(side)
-        del graphs
+                if nat.lib().wpc_graph_begin(nat.stream()) != 0:
+                    break
+                for _ in range(reps):
+                    self._launch(m, x, None, z, 1.0, 0.0, nvec)
+                handle = ctypes.c_void_p()
+                if nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle)) != 0:
+                    break
+                graphs[m] = handle.value
+        if len(graphs) < len(self.CANDIDATES):
+            for g in graphs.values():
+                nat.lib().wpc_graph_destroy(g)
+            graphs = {}
+        try:
+            with torch.cuda.stream(work):
+                for _ in range(rounds):
+                    for m in self.CANDIDATES:
+                        e0 = torch.cuda.Event(enable_timing=True)
+                        e1 = torch.cuda.Event(enable_timing=True)
+                        e0.record(work)
+                        if graphs:
+                            nat.call("wpc_graph_launch", graphs[m], nat.stream())
+                        else:
+                            for _ in range(reps):
+                                self._launch(m, x, None, z, 1.0, 0.0, nvec)
+                        e1.record(work)
+                        e1.synchronize()
+                        t = e0.elapsed_time(e1) / reps
+                        key = (nvec, dtype, m)
+                        self.timings[key] = min(t, self.timings.get(key, float("inf")))
+        finally:
+            for g in graphs.values():
+                nat.lib().wpc_graph_destroy(g)
+        torch.cuda.current_stream().wait_stream(work)
         best, best_t = 0, float("inf")
         for m in self.CANDIDATES:
             t = self.timings[(nvec, dtype, m)]
