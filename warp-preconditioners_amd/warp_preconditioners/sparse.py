@@ -145,10 +145,11 @@ class StreamPlan:
     # Rows are ordered by length inside windows of SORT_WINDOW rows only when that
     # removes more than SORT_GAIN * nnz padded entries.  Natural order keeps the
     # gathers of neighbouring rows together, which outweighs moderate padding: on
-    # the 64^3 biharmonic G^T (39.6% padding) the 3-vector pass takes 11.9 us in
-    # natural order and 12.4 / 13.2 / 14.5 us ordered in windows of 128 / 512 /
-    # 1024 rows (padding 23 / 7 / 4%).  The order pays for rows that are far longer
-    # than their slice neighbours (padding above the matrix's own size).
+    # the 64^3 biharmonic G^T (28% of the stored entries are padding) the 3-vector
+    # pass with its dot takes 9.7 us in natural order and 10.2 / 9.9 / 10.5 / 11.7 us
+    # ordered in windows of 128 / 256 / 512 / 1024 rows (padding 18 / 11 / 6 / 4%).
+    # The order pays for rows that are far longer than their slice neighbours
+    # (padding above the matrix's own size).
     SORT_WINDOW = 256
     SORT_GAIN = 1.0
     # Store the sliced columns as int16 differences from the row when they all fit.
