@@ -227,6 +227,44 @@ def test_native_graph_capture_replays():
         nat.call("wpc_graph_destroy", handle.value)
 
 
+def test_private_capture_streams():
+    # torch hands out streams from a pool of 32 per device, so 40 torch streams
+    # repeat; the capture streams come from the library, differ from every pooled
+    # stream and per thread, and a graph recorded on one replays on another stream.
+    import ctypes
+    import threading
+
+    from warp_preconditioners import _native as nat
+
+    pooled = {torch.cuda.Stream().cuda_stream for _ in range(40)}
+    mine = nat.capture_stream()
+    assert mine is nat.capture_stream()
+    other = {}
+    t = threading.Thread(target=lambda: other.setdefault("s", nat.capture_stream().cuda_stream))
+    t.start()
+    t.join()
+    assert mine.cuda_stream not in pooled
+    assert other["s"] not in pooled and other["s"] != mine.cuda_stream
+    n = 1000
+    x = torch.arange(n, dtype=torch.float64, device=dev())
+    y = torch.ones(n, dtype=torch.float64, device=dev())
+    run = torch.cuda.Stream()
+    handle = ctypes.c_void_p()
+    with torch.cuda.stream(mine):
+        nat.call("wpc_graph_begin", nat.stream())
+        nat.call("wpc_axpby", n, 2.0, nat.ptr(x), 1.0, nat.ptr(y), nat.ptr(y),
+                 nat.code(torch.float64), nat.stream())
+        assert nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle)) == 0
+    for rep in range(1, 4):
+        nat.call("wpc_graph_launch", handle.value, run.cuda_stream)
+        run.synchronize()
+        assert torch.equal(y, 1.0 + 2.0 * rep * x)
+    nat.call("wpc_graph_destroy", handle.value)
+    extra = nat.PrivateStream()
+    assert extra.handle not in pooled and extra.handle != mine.cuda_stream
+    extra.release()
+
+
 @pytest.mark.parametrize("factor", [torch.float32, torch.bfloat16])
 def test_refit_clone_equals_update(factor):
     from warp_preconditioners import FSAI, assembly
