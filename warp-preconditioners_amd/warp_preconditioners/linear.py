@@ -247,6 +247,19 @@ class KrylovWork:
         s = self.state.cpu().view(-1, self.nvec)
         return s
 
+    def snapshot_state(self, index):
+        """Copy the state to pinned host memory on the current stream, without
+        waiting; returns (event, host tensor).  Two buffers alternate by `index`:
+        a buffer is reused only after its previous copy was waited for."""
+        if getattr(self, "_snapshots", None) is None:
+            self._snapshots = [torch.empty(self.state.shape, dtype=self.state.dtype,
+                                           pin_memory=True) for _ in range(2)]
+        host = self._snapshots[index % 2]
+        host.copy_(self.state, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        return ev, host
+
 
 def nat_state_fields() -> int:
     return 12
@@ -333,12 +346,32 @@ def solve_batched(method, A, B, X, M=None, tol=1.0e-8, atol=0.0, maxiter=10000, 
     st = work.host_state()
     chunks = 0
     iters = 0
-    while iters < maxiter and bool((st[1] > st[2]).any()):
-        steps = min(check_every, maxiter - iters)
-        work.run(X.reshape(-1), steps, use_cuda_graph and steps == check_every and chunks > 0)
-        iters += steps
-        chunks += 1
-        st = work.host_state()
+    xf = X.reshape(-1)
+
+    def active(state):
+        return bool((state[1] > state[2]).any())
+
+    # The next chunk is launched before the previous chunk's state is read (an
+    # asynchronous copy to pinned memory), so the device does not idle while the
+    # host checks.  A chunk launched after the last slot converged takes no steps
+    # (a resting slot is unchanged), so results and counts are those of checking
+    # after every chunk; it costs one chunk's launches at the end of the solve.
+    pending = []
+    while iters < maxiter and (pending or active(st)):
+        while iters < maxiter and len(pending) < 2:
+            steps = min(check_every, maxiter - iters)
+            work.run(xf, steps, use_cuda_graph and steps == check_every and chunks > 0)
+            iters += steps
+            pending.append(work.snapshot_state(chunks))
+            chunks += 1
+        ev, host = pending.pop(0)
+        ev.synchronize()
+        st = host.view(-1, k)
+        if not active(st):
+            break
+    for ev, _ in pending:
+        ev.synchronize()
+    st = work.host_state()
     return dict(iterations=[int(v) for v in st[4]], residuals=[float(v) ** 0.5 for v in st[1]],
                 targets=[float(v) ** 0.5 for v in st[2]], chunks=chunks, work=work)
 
