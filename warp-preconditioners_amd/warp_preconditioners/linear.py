@@ -297,16 +297,32 @@ def _solve(method, A, b, x, tol, atol, maxiter, M, callback, check_every, use_cu
     if callback is not None:
         callback(0, math.sqrt(rr), abs_tol)
     if rr > target:
-        while iters < maxiter:
-            steps = min(chunk, maxiter - iters)
-            work.run(xf, steps, use_cuda_graph and steps == chunk and iters > 0)
-            iters += steps
-            st = work.host_state()
-            rr = float(st[1, 0])
+        # As in solve_batched, the next chunk goes out before the previous chunk's
+        # state is read; the count returned is that of the check that saw
+        # convergence (a later chunk takes no steps).  With a callback, which may
+        # look at x, each chunk is checked before the next one starts.
+        depth = 1 if callback is not None else 2
+        pending = []
+        launched = 0
+        while True:
+            while launched < maxiter and len(pending) < depth:
+                steps = min(chunk, maxiter - launched)
+                work.run(xf, steps, use_cuda_graph and steps == chunk and launched > 0)
+                ev, host = work.snapshot_state(launched // chunk)
+                launched += steps
+                pending.append((ev, host, launched))
+            if not pending:
+                break
+            ev, host, iters = pending.pop(0)
+            ev.synchronize()
+            rr = float(host.view(-1, 1)[1, 0])
             if callback is not None:
                 callback(iters, math.sqrt(rr), abs_tol)
             if rr <= target:
                 break
+        for ev, _, _ in pending:
+            ev.synchronize()
+        st = work.host_state()
     if exact:
         taken = int(st[4, 0])
         return (torch.tensor([taken], dtype=torch.int32, device=bf.device),
