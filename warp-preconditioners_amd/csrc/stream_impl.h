@@ -341,7 +341,8 @@ template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
 // Entries of a row handled per round trip: a group's index and value loads go out
 // together, then its gathers.  A slice's length (uniform across the wavefront) is
 // covered by groups of 8, then the remainder r < 8: r = 6, 7 as one masked group
-// of 8, r = 3 as a masked group of 4, otherwise groups of 4, 2 and 1.  A masked
+// of 8, r = 3 as a masked group of 4, otherwise groups of 4, 2 and 1 (with many
+// vectors, groups of 4 and the remainder r = 3 masked, 2 or 1: kSellMain).  A masked
 // group's entries past the slice re-read its last entry (a cache hit; nothing past
 // the slice is read) and their products are discarded by a select, so the sums are
 // those of storage order.  Masking only where it wastes at most two gathers keeps
@@ -374,6 +375,12 @@ __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long b
     }
 }
 
+// Entries per full group: 8 while a group's gathered values fit in 64 registers
+// (up to 4 double or 8 float vectors), else 4 (8 entries of 8 double vectors take
+// 130 registers, too many for four wavefronts per SIMD).
+template <class TX, int NV>
+constexpr int kSellMain = kNvMax<NV> * sizeof(typename AccOf<TX>::type) <= 32 ? 8 : 4;
+
 template <class TI, class TV, class TX, int NV>
 __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long base, int len,
                                            int row, const TI* __restrict__ col,
@@ -381,6 +388,18 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
                                            int nv_rt) {
     const int nv = NV > 0 ? NV : nv_rt;
     int k = 0;
+    if constexpr (kSellMain<TX, NV> == 4) {
+        for (; k + 4 <= len; k += 4)
+            sell_group<TI, TV, TX, NV, 4>(acc, base, k, row, col, val, x, nv);
+        const int r = len - k;
+        if (r == 3)
+            sell_group<TI, TV, TX, NV, 4, false>(acc, base, k, row, col, val, x, nv, 3);
+        else if (r == 2)
+            sell_group<TI, TV, TX, NV, 2>(acc, base, k, row, col, val, x, nv);
+        else if (r == 1)
+            sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
+        return;
+    }
     for (; k + 8 <= len; k += 8) sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
     int r = len - k;
     if (r >= 6) {
