@@ -269,6 +269,36 @@ def _default_tol(dtype):
     return {torch.float32: 1.0e-6, torch.float64: 1.0e-12}.get(dtype, 1.0e-3)
 
 
+def _chunk_loop(work, xf, chunk, maxiter, use_graph, depth, check):
+    """Run chunks of `chunk` steps (at most `maxiter` steps in all) until
+    check(host_state, steps) is true for the state after some chunk.  With depth 2
+    the next chunk is launched before the previous chunk's state is read (an
+    asynchronous copy to pinned memory), so the device does not idle while the host
+    checks; a chunk launched after every slot converged takes no steps (a resting
+    slot is unchanged), so the outcome is that of checking after every chunk, at
+    the cost of one chunk's launches.  Returns (steps at the deciding check, or at
+    the last chunk, chunks launched)."""
+    pending = []
+    launched = chunks = at = 0
+    while True:
+        while launched < maxiter and len(pending) < depth:
+            steps = min(chunk, maxiter - launched)
+            work.run(xf, steps, use_graph and steps == chunk and chunks > 0)
+            launched += steps
+            ev, host = work.snapshot_state(chunks)
+            pending.append((ev, host, launched))
+            chunks += 1
+        if not pending:
+            break
+        ev, host, at = pending.pop(0)
+        ev.synchronize()
+        if check(host, at):
+            break
+    for ev, _, _ in pending:
+        ev.synchronize()
+    return at, chunks
+
+
 def _solve(method, A, b, x, tol, atol, maxiter, M, callback, check_every, use_cuda_graph):
     A = aslinearoperator(A)
     M = aslinearoperator(M)
@@ -297,31 +327,17 @@ def _solve(method, A, b, x, tol, atol, maxiter, M, callback, check_every, use_cu
     if callback is not None:
         callback(0, math.sqrt(rr), abs_tol)
     if rr > target:
-        # As in solve_batched, the next chunk goes out before the previous chunk's
-        # state is read; the count returned is that of the check that saw
-        # convergence (a later chunk takes no steps).  With a callback, which may
-        # look at x, each chunk is checked before the next one starts.
-        depth = 1 if callback is not None else 2
-        pending = []
-        launched = 0
-        while True:
-            while launched < maxiter and len(pending) < depth:
-                steps = min(chunk, maxiter - launched)
-                work.run(xf, steps, use_cuda_graph and steps == chunk and launched > 0)
-                ev, host = work.snapshot_state(launched // chunk)
-                launched += steps
-                pending.append((ev, host, launched))
-            if not pending:
-                break
-            ev, host, iters = pending.pop(0)
-            ev.synchronize()
-            rr = float(host.view(-1, 1)[1, 0])
+        def check(host, at):
+            nonlocal rr, iters
+            rr, iters = float(host.view(-1, 1)[1, 0]), at
             if callback is not None:
-                callback(iters, math.sqrt(rr), abs_tol)
-            if rr <= target:
-                break
-        for ev, _, _ in pending:
-            ev.synchronize()
+                callback(at, math.sqrt(rr), abs_tol)
+            return rr <= target
+
+        # (with a callback, which may look at x, every chunk is checked before the
+        # next one starts; the count returned is that of the deciding check)
+        _chunk_loop(work, xf, chunk, maxiter, use_cuda_graph, 1 if callback is not None else 2,
+                    check)
         st = work.host_state()
     if exact:
         taken = int(st[4, 0])
@@ -361,32 +377,14 @@ def solve_batched(method, A, B, X, M=None, tol=1.0e-8, atol=0.0, maxiter=10000, 
     work.start(B.reshape(-1), X.reshape(-1), tol, atol)
     st = work.host_state()
     chunks = 0
-    iters = 0
-    xf = X.reshape(-1)
 
-    def active(state):
-        return bool((state[1] > state[2]).any())
+    def converged(host, _steps):
+        h = host.view(-1, k)
+        return not bool((h[1] > h[2]).any())
 
-    # The next chunk is launched before the previous chunk's state is read (an
-    # asynchronous copy to pinned memory), so the device does not idle while the
-    # host checks.  A chunk launched after the last slot converged takes no steps
-    # (a resting slot is unchanged), so results and counts are those of checking
-    # after every chunk; it costs one chunk's launches at the end of the solve.
-    pending = []
-    while iters < maxiter and (pending or active(st)):
-        while iters < maxiter and len(pending) < 2:
-            steps = min(check_every, maxiter - iters)
-            work.run(xf, steps, use_cuda_graph and steps == check_every and chunks > 0)
-            iters += steps
-            pending.append(work.snapshot_state(chunks))
-            chunks += 1
-        ev, host = pending.pop(0)
-        ev.synchronize()
-        st = host.view(-1, k)
-        if not active(st):
-            break
-    for ev, _ in pending:
-        ev.synchronize()
+    if not converged(st.reshape(-1), 0):
+        _, chunks = _chunk_loop(work, X.reshape(-1), check_every, maxiter, use_cuda_graph, 2,
+                                converged)
     st = work.host_state()
     return dict(iterations=[int(v) for v in st[4]], residuals=[float(v) ** 0.5 for v in st[1]],
                 targets=[float(v) ** 0.5 for v in st[2]], chunks=chunks, work=work)
