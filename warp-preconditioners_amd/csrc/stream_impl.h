@@ -340,9 +340,11 @@ template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
 
 // Entries of a row handled per round trip: a group's index and value loads go out
 // together, then its gathers.  A slice's length (uniform across the wavefront) is
-// covered by groups of 8, then the remainder r < 8: r = 6, 7 as one masked group
-// of 8, r = 3 as a masked group of 4, otherwise groups of 4, 2 and 1 (with many
-// vectors, groups of 4 and the remainder r = 3 masked, 2 or 1: kSellMain).  A masked
+// covered by groups of 8 until at most 12 entries remain; a remainder of 9-12 is
+// one masked group of 12 (where 12 entries' gathers fit in registers), then 8 a
+// group of 8, 6-7 a masked group of 8, 3 a masked group of 4, otherwise groups of
+// 4, 2 and 1 (with many vectors, groups of 4 and the remainder 3 masked, 2 or 1:
+// kSellMain).  Column indices are held as 32-bit values (a column fits an int).  A masked
 // group's entries past the slice re-read its last entry (a cache hit; nothing past
 // the slice is read) and their products are discarded by a select, so the sums are
 // those of storage order.  Masking only where it wastes at most two gathers keeps
@@ -354,13 +356,13 @@ __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long b
                                            int nv, int m = G) {
     using TA = typename AccOf<TX>::type;
     constexpr int NVM = kNvMax<NV>;
-    long c[G];
+    int c[G];
     TA a[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
         const int e = FULL || j < m ? j : m - 1;
         const long p = base + (long)(k + e) * 64;
-        c[j] = sell_col<TI>(col, p, row);
+        c[j] = (int)sell_col<TI>(col, p, row);
         a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
     }
 #pragma unroll
@@ -368,7 +370,7 @@ __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long b
 #pragma unroll
         for (int v = 0; v < NVM; ++v) {
             if (v < nv) {
-                const TA t = acc[v] + a[j] * St<TX>::template ld<TA>(x, c[j] * nv + v);
+                const TA t = acc[v] + a[j] * St<TX>::template ld<TA>(x, (long)c[j] * nv + v);
                 acc[v] = FULL || j < m ? t : acc[v];
             }
         }
@@ -400,8 +402,19 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
             sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
         return;
     }
-    for (; k + 8 <= len; k += 8) sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
+    // 12 entries' gathers: at most 72 registers (3 double or 6 float vectors)
+    constexpr bool kTwelve = kNvMax<NV> * sizeof(typename AccOf<TX>::type) <= 24;
+    for (; len - k > (kTwelve ? 12 : 7); k += 8)
+        sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
     int r = len - k;
+    if (kTwelve && r > 8) {
+        sell_group<TI, TV, TX, NV, 12, false>(acc, base, k, row, col, val, x, nv, r);
+        return;
+    }
+    if (r == 8) {
+        sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
+        return;
+    }
     if (r >= 6) {
         sell_group<TI, TV, TX, NV, 8, false>(acc, base, k, row, col, val, x, nv, r);
         return;
