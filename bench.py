@@ -437,6 +437,17 @@ def block_bench(name):
         torch.cuda.synchronize()
         return out, time.perf_counter() - t
 
+    def ready(P, dtype):
+        """A preconditioner after its first application: the product layouts built
+        and row mappings tuned (first-use work counted as setup, not as solve)."""
+        v = torch.ones(n, dtype=dtype, device=dev)
+        P.matvec(v, v, torch.empty_like(v), 1.0, 0.0)
+        return P
+
+    # the operators' first products (layout, tuning) outside every timed solve
+    for op, dt in ((op64, torch.float64), (op32, torch.float32)):
+        v = torch.ones(n, dtype=dt, device=dev)
+        op.matvec(v, v, torch.empty_like(v), 1.0, 0.0)
     Pref, _ = timed(lambda: wp.FSAI(A64, max_row_size=16 if name == "kron48" else 8))
     xref = torch.zeros(n, dtype=torch.float64, device=dev)
     log("float64 reference solve")
@@ -472,19 +483,20 @@ def block_bench(name):
     for label, prec, method, mode, fdt in methods:
         log(f"running {label}")
         A = A32 if mode == "f32" or (mode == "mixed" and prec != "fsai16" and prec != "fsai8") else A64
+        vt = torch.float32 if mode in ("f32", "mixed") else torch.float64
         if prec == "jacobi":
             # the tolerance the reference passes for K (x) C (block_challenge.py:101)
-            P, setup = timed(lambda: wp.BlockJacobi(A, pivot_tolerance=1e-15))
+            P, setup = timed(lambda: ready(wp.BlockJacobi(A, pivot_tolerance=1e-15), vt))
         elif prec == "ilu0":
-            P, setup = timed(lambda: wp.BlockILU0(A, factor_sweeps=3))
+            P, setup = timed(lambda: ready(wp.BlockILU0(A, factor_sweeps=3), vt))
         elif prec.startswith("blockfsai"):
             # block FSAI: 5 blocks of 3 = 15 scalars per row (block_fsai_candidate.py)
-            P, setup = timed(lambda: wp.BlockFSAI(A, max_blocks=int(prec[9:]),
-                                                  factor_dtype=torch.float32))
+            P, setup = timed(lambda: ready(wp.BlockFSAI(A, max_blocks=int(prec[9:]),
+                                                        factor_dtype=torch.float32), vt))
         else:
             width = int(prec[4:])
-            P, setup = timed(lambda: wp.FSAI(A, max_row_size=width,
-                                              factor_dtype=fdt or torch.float32))
+            P, setup = timed(lambda: ready(wp.FSAI(A, max_row_size=width,
+                                                    factor_dtype=fdt or torch.float32), vt))
         x = torch.zeros(n, dtype=torch.float64, device=dev)
         if method == "gmres":
             from warp_preconditioners.refinement import TrueResidual
