@@ -219,8 +219,9 @@ __global__ __launch_bounds__(kStreamRows) void stream_kernel(
 }
 
 // Multi-lane mapping of the same product: L = 1 << lg lanes own one row, stride
-// over its entries and reduce with xor shuffles (no LDS staging).  Same epilogue
-// and fused inner product as stream_kernel.
+// over its entries and reduce with an xor butterfly (DPP quad permutations for
+// strides 2 and 1, shuffles above; no LDS staging).  Same epilogue and fused inner
+// product as stream_kernel.
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
     int n, int lg, const int* __restrict__ off, const int* __restrict__ col,
