@@ -122,12 +122,14 @@ __global__ __launch_bounds__(256) void build_kernel(
             if (skip) continue;
             const T sc = ldv<T, S>(scale, c);
             T residual = T(0);
-            for (int q0 = 0; q0 < size; q0 += 8) {
-                // A[c, pattern[q0..q0+8)] by lockstep searches, summed in q order
-                T e[8];
-                entries<T, S, 8>(off, col, val, c, pattern + q0, min(8, size - q0), e);
+            for (int q0 = 0; q0 < size; q0 += 4) {
+                // A[c, pattern[q0..q0+4)] by lockstep searches, summed in q order
+                // (4 at a time: 8 needed registers that cost more than the round
+                // trips they saved)
+                T e[4];
+                entries<T, S, 4>(off, col, val, c, pattern + q0, min(4, size - q0), e);
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
+                for (int q = 0; q < 4; ++q)
                     if (q0 + q < size) residual += e[q] * sc * scl[q0 + q] * z[q0 + q];
             }
             const T score = nabs(residual);
