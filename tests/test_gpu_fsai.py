@@ -284,3 +284,17 @@ def test_empty_matrix(wp):
         v = torch.empty(0, dtype=dtype, device=dev())
         pre.matvec(v, v, v, 1.0, 0.0)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("count", [1, 1023, 1024, 8191, 8192, 8193, 100003, 262145])
+def test_scan_inclusive_exact(count):
+    # The setup scan (tiles of 8192: thread-serial runs, wavefront and workgroup
+    # carries) equals numpy's cumulative sum on counts across tile boundaries.
+    from warp_preconditioners import _native as nat
+
+    rng = np.random.default_rng(count)
+    a = rng.integers(0, 70, size=count).astype(np.int32)
+    src = torch.tensor(a, device=dev())
+    out = torch.full((count,), -1, dtype=torch.int32, device=dev())
+    nat.call("wpc_scan_inclusive", count, nat.ptr(src), nat.ptr(out), nat.stream())
+    np.testing.assert_array_equal(host(out), np.cumsum(a, dtype=np.int64).astype(np.int32))

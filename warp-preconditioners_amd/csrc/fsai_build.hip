@@ -318,26 +318,51 @@ __global__ __launch_bounds__(256) void sort_rows_kernel(int n, int W, int* colum
     counts[i + 1] = size;
 }
 
-// One-workgroup inclusive scan (setup only; wp.utils.array_scan).
-__global__ __launch_bounds__(1024) void scan_kernel(int count, const int* in, int* out) {
-    __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < count; base += 1024) {
-        const int idx = base + threadIdx.x;
-        int x = idx < count ? in[idx] : 0;
-        buf[threadIdx.x] = x;
-        __syncthreads();
-        for (int d = 1; d < 1024; d <<= 1) {
-            int y = threadIdx.x >= d ? buf[threadIdx.x - d] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += y;
-            __syncthreads();
+// One-workgroup inclusive scan (setup only; wp.utils.array_scan).  Tiles of 8192
+// entries: coalesced loads through LDS, each thread scans 8 consecutive entries,
+// the 1024 thread totals are scanned across the workgroup (wavefront shuffles, then
+// the 16 wavefront totals), and the results leave through LDS again, coalesced.
+constexpr int kScanThreads = 1024, kScanPer = 8, kScanTile = kScanThreads * kScanPer;
+
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(int count, const int* in, int* out) {
+    __shared__ int tile[kScanTile];
+    __shared__ int wave_tot[kScanThreads / kWave];
+    __shared__ int carry_s;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    int carry = 0;
+    for (int base = 0; base < count; base += kScanTile) {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            const int idx = base + k * kScanThreads + t;
+            tile[k * kScanThreads + t] = idx < count ? in[idx] : 0;
         }
-        if (idx < count) out[idx] = buf[threadIdx.x] + carry;
         __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
+        int v[kScanPer];
+        int run = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            run += tile[t * kScanPer + k];
+            v[k] = run;
+        }
+        int pre = run;  // inclusive scan of the thread totals within the wavefront
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int o = __shfl_up(pre, d, kWave);
+            if (lane >= d) pre += o;
+        }
+        if (lane == kWave - 1) wave_tot[wave] = pre;
+        __syncthreads();
+        int before = carry + pre - run;  // everything ahead of this thread's entries
+        for (int w = 0; w < wave; ++w) before += wave_tot[w];
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) tile[t * kScanPer + k] = v[k] + before;
+        if (t == kScanThreads - 1) carry_s = v[kScanPer - 1] + before;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            const int idx = base + k * kScanThreads + t;
+            if (idx < count) out[idx] = tile[k * kScanThreads + t];
+        }
+        carry = carry_s;
         __syncthreads();
     }
 }
@@ -439,7 +464,7 @@ static int transpose_launch(int nrow, int ncol, int nnz, const int* off, const i
     if (nnz > 0)
         hipLaunchKernelGGL(count_columns_kernel, dim3(blocks_for(nnz, 256)), dim3(256), 0, s, nnz,
                            col, work);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, ncol + 1, work, toff);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, ncol + 1, work, toff);
     hipLaunchKernelGGL(copy_i32_kernel, dim3(blocks_for(ncol + 1, 256)), dim3(256), 0, s, ncol + 1,
                        toff, work);
     if (nrow > 0) {
@@ -581,7 +606,7 @@ int wpc_fsai_sort_rows(int n, int width, int* columns, void* values, int dtype, 
 
 int wpc_scan_inclusive(int count, const int* in, int* out, void* stream) {
     if (count <= 0) return WPC_OK;
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, as_stream(stream), count, in, out);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, as_stream(stream), count, in, out);
     return launch_status();
 }
 
