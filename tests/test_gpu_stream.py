@@ -289,3 +289,35 @@ def test_sliced_every_slice_length_bitwise(oracle, monkeypatch, col16):
         outs.append((host(Z), host(dots)))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-12)
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+@pytest.mark.parametrize("nvec", [3, 8])
+def test_lanes_mapping_batched_epilogue(lanes, nvec):
+    # The multi-lane mapping (two entries per lane per round trip, butterfly sums
+    # with DPP quad steps) on several vectors with the fused epilogue and dot,
+    # against the streamed mapping's serial sums (rows reordered: tolerance).
+    from warp_preconditioners.sparse import StreamPlan
+
+    rng = np.random.default_rng(lanes * 10 + nvec)
+    n = 1500
+    a = ss.random(n, n, density=0.01, random_state=rng, format="lil")
+    a[7, :] = 0
+    a[9, : 300] = rng.normal(size=300)  # one long row
+    a = a.tocsr()
+    A = upload(a)
+    X = torch.tensor(rng.normal(size=(n, nvec)), device=dev())
+    Y = torch.tensor(rng.normal(size=(n, nvec)), device=dev())
+    U = torch.tensor(rng.normal(size=(n, nvec)), device=dev())
+    S = torch.tensor(rng.uniform(0.5, 2, size=n), dtype=torch.float32, device=dev())
+    D = torch.tensor(rng.uniform(0.5, 2, size=n), device=dev())
+    outs = []
+    for mode in (0, lanes):
+        p = StreamPlan(A, lanes=mode)
+        Z = torch.empty_like(X)
+        dots = torch.zeros(nvec, dtype=torch.float64, device=dev())
+        p.run(X.reshape(-1), Y.reshape(-1), Z.reshape(-1), 0.9, -0.4, nvec, s=S, d=D,
+              u=U.reshape(-1), w=X.reshape(-1), out=dots)
+        outs.append((host(Z), host(dots)))
+    np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=1e-10)
