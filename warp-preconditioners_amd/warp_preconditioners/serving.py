@@ -280,7 +280,10 @@ class Slot:
         self.bytes = tensor_bytes([self.X, self.B, self.Rin, self.C, self.c64, self.norms_w, r.r,
                                    r.t_hi, r.t_lo, r.partials, r.norms, w.r, w.p, w.ap, w.z,
                                    w.map, w.az, w.parts, w.state, w.dot_parts, gs])
-        self.state_host = torch.empty(12 * nvec, dtype=torch.float64, pin_memory=True)
+        # one per chunk in flight (two at most): a chunk's state is read while the
+        # next chunk's copy may be landing
+        self.state_hosts = [torch.empty(12 * nvec, dtype=torch.float64, pin_memory=True)
+                            for _ in range(2)]
         self.norms_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
         self.norms_w_host = torch.empty(2 * nvec, dtype=torch.float64, pin_memory=True)
         self.x_host = torch.empty(n * nvec, dtype=torch.float64, pin_memory=True)
@@ -338,7 +341,8 @@ class Job:
         self.cfg = eng.cfg
         self.provenance = provenance
         self.t_admit = t_admit
-        self.pending = deque()  # (event, what)
+        self.pending = deque()  # (event, what, host buffer of a state read)
+        self.last_state = None  # host copy of the state after the last chunk read
         self.chunks_out = 0
         self.phase = "init"
         self.outer = 0
@@ -362,10 +366,10 @@ class Job:
         system.active += 1
 
     # ------------------------------------------------------------------ helpers
-    def _record(self, what):
+    def _record(self, what, host=None):
         ev = torch.cuda.Event()
         ev.record(self.system.stream)
-        self.pending.append((ev, what))
+        self.pending.append((ev, what, host))
 
     def _start_solve(self, b, x, tol):
         x.zero_()
@@ -388,8 +392,8 @@ class Job:
         """Consume completed host reads in order."""
         did = False
         while self.pending and self.pending[0][0].query():
-            _, what = self.pending.popleft()
-            self._on(what)
+            _, what, host = self.pending.popleft()
+            self._on(what, host)
             did = True
         return did
 
@@ -434,8 +438,9 @@ class Job:
     def _own_chunk(self):
         s = self.slot
         s.chunk(self.cfg.chunk)
-        s.state_host.copy_(s.work.state, non_blocking=True)
-        self._record("state")
+        host = s.state_hosts[self.chunks_launched % 2]
+        host.copy_(s.work.state, non_blocking=True)
+        self._record("state", host)
         self.chunks_out += 1
         self.chunks_launched += 1
 
@@ -448,11 +453,12 @@ class Job:
         """Bookkeeping after the engine launched this job's chunk in a batch on
         `stream` (the state read is ordered after it on the same stream)."""
         s = self.slot
+        host = s.state_hosts[self.chunks_launched % 2]
         with torch.cuda.stream(stream):
-            s.state_host.copy_(s.work.state, non_blocking=True)
+            host.copy_(s.work.state, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
-        self.pending.append((ev, "state"))
+        self.pending.append((ev, "state", host))
         self.chunks_out += 1
         self.chunks_launched += 1
 
@@ -460,7 +466,7 @@ class Job:
         """Recursive targets met: enqueue the certification step."""
         s = self.slot
         self.want_check = False
-        iters = s.state_host.view(12, -1)[_ITER].numpy()[: self.req.nrhs]
+        iters = self.last_state.view(12, -1)[_ITER].numpy()[: self.req.nrhs]
         self.iterations += iters.astype(np.int64)
         nat.call("wpc_refine_update", self.system.n, self.req.nrhs, nat.ptr(s.X), nat.ptr(s.C),
                  nat.code(s.C.dtype), nat.ptr(s.res.norms), self.mask, nat.ptr(s.c64), nat.stream())
@@ -470,12 +476,13 @@ class Job:
         self._residual()
         self.phase = "wait_residual"
 
-    def _on(self, what):
+    def _on(self, what, host=None):
         s = self.slot
         k = self.req.nrhs
         if what == "state":
             self.chunks_out -= 1
-            st = s.state_host.view(12, -1).numpy()[:, :k]
+            self.last_state = host
+            st = host.view(12, -1).numpy()[:, :k]
             rr, target = st[_RR], st[_TARGET]
             self.far = bool(np.all(rr > 1e4 * target))
             if np.all(rr <= target) or np.max(self.iterations + st[_ITER]) >= self.cfg.maxiter:
