@@ -257,12 +257,6 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
     }
 }
 
-// The same two steps with the scalar folds folded in, for partial counts small
-// enough that every workgroup can afford to fold them: each workgroup folds in the
-// same fixed order, so all agree bit for bit, and workgroup 0 records the scalars.
-// The fields each kernel reads and the fields workgroup 0 writes are disjoint:
-//   update:    reads RZ, RR, TARGET      writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA
-//   direction: reads RZ_OLD, ACTIVE      writes RZ_NEW, RR_NEW, BETA, RZ, RR, ITER
 // State fields F0, F1 (, F2) of all nv vectors, held by the lanes of every
 // wavefront (lane k * nv + v holds field k of vector v) and loaded with ordinary
 // vector loads, so that they travel with a step's other loads (uniform reads
@@ -288,6 +282,12 @@ struct StateLanes {
     }
 };
 
+// The same two steps with the scalar folds folded in, for partial counts small
+// enough that every workgroup can afford to fold them: each workgroup folds in the
+// same fixed order, so all agree bit for bit, and workgroup 0 records the scalars.
+// The fields each kernel reads and the fields workgroup 0 writes are disjoint:
+//   update:    reads RZ, RR, TARGET      writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA
+//   direction: reads RZ_OLD, ACTIVE      writes RZ_NEW, RR_NEW, BETA, RZ, RR, ITER
 // Rows each thread loads before the fold, so that the HBM reads overlap the fold's
 // latency (bounded so the prefetched values stay in registers).
 template <int NV> constexpr int kPrefetch = NV <= 2 ? 4 : (NV <= 4 ? 2 : 1);
