@@ -241,9 +241,10 @@ def kernel_times(s, reps=30):
     fold_a = pk * k <= FOLD_IN_PLACE
     fold_b = fold_a and max(pz, npart) * k <= FOLD_IN_PLACE
     if fold_a:
+        # (with both steps folded, x advances in the direction step)
         kernels["cg_update_fold"] = lambda: nat.call(
-            "wpc_cg_update_fold", n, k, nat.ptr(X), nat.ptr(Y), nat.ptr(T), nat.ptr(T), code, pk,
-            nat.ptr(pl.partials(k)), nat.ptr(state), nat.ptr(parts), cur())
+            "wpc_cg_update_fold", n, k, None if fold_b else nat.ptr(X), nat.ptr(Y), nat.ptr(T),
+            nat.ptr(T), code, pk, nat.ptr(pl.partials(k)), nat.ptr(state), nat.ptr(parts), cur())
     else:
         kernels["cg_alpha"] = lambda: nat.call("wpc_cg_alpha", k, pk, nat.ptr(pl.partials(k)),
                                                nat.ptr(state), cur())
@@ -252,7 +253,7 @@ def kernel_times(s, reps=30):
             nat.ptr(state), nat.ptr(parts), cur())
     if fold_b:
         kernels["cg_direction_fold"] = lambda: nat.call(
-            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), code, pz,
+            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), nat.ptr(X), code, pz,
             nat.ptr(pgt.partials(k)), npart, nat.ptr(parts), nat.ptr(state), cur())
     else:
         kernels["cg_beta"] = lambda: nat.call("wpc_cg_beta", k, pz, nat.ptr(pgt.partials(k)),
@@ -305,10 +306,10 @@ def kernel_times(s, reps=30):
         "sqlap_backward_dot": nnzK * (vb_k + 4) + idx + 3 * n * k * xb + n * vb_k,
         "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
-        "cg_update_fold": 6 * n * k * xb + 8 * pk * k,
+        "cg_update_fold": (3 if fold_b else 6) * n * k * xb + 8 * pk * k,
         "cg_beta": 8 * (pz + npart) * k,
         "cg_direction": 3 * n * k * xb,
-        "cg_direction_fold": 3 * n * k * xb + 8 * (pz + npart) * k,
+        "cg_direction_fold": 5 * n * k * xb + 8 * (pz + npart) * k,
     }
     return out, {name: bytes_[name] for name in out}, tuned
 

@@ -443,13 +443,16 @@ int wpc_cg_direction(int n, int nvec, const void* z, void* p, int dtype, const d
  * partials) in one fixed order, so all workgroups agree on ALPHA/BETA bit for bit
  * and workgroup 0 records the scalars.  Meant for small partial counts
  * (nparts * nvec <= 4096: the folds are repeated per workgroup).  The pair is used
- * together: the update saves RZ in RZ_OLD, which the direction step reads. */
+ * together: the update saves RZ in RZ_OLD, which the direction step reads.
+ * x = NULL in the update leaves x (and p) alone; the direction step given x then
+ * applies the update's x += ALPHA p before it replaces p (one vector read fewer per
+ * iteration, the same arithmetic).  Pass x to exactly one of the two. */
 int wpc_cg_update_fold(int n, int nvec, void* x, void* r, const void* p, const void* ap, int dtype,
                        int nparts, const double* part_denom, double* state, double* partials,
                        void* stream);
-int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, int nparts_rz,
-                          const double* part_rz, int nparts_rr, const double* part_rr,
-                          double* state, void* stream);
+int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int dtype,
+                          int nparts_rz, const double* part_rz, int nparts_rr,
+                          const double* part_rr, double* state, void* stream);
 
 /* ------------------------------------------------------------------ grouped CG steps
  * No reference counterpart: a serving extension that advances the fused CG of
@@ -465,11 +468,12 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, in
  *   sell_values, s, d, u, x, z, w, partials, n, block0, col16, nvec} -- the wpc_sell_stream
  *   product with beta = 0 on each system (ceil(n/256) workgroups); s_dtype 0 means
  *   no row scale, has_d selects the d .* u term, dot the w . z partials.
- * wpc_cg_fold_grouped: step 1 = wpc_cg_update_fold, step 2 = wpc_cg_direction_fold
- *   per task {x, r, p, ap, z, part_a, part_b, state, partials, n, nblocks, block0,
- *   pa, pb, nvec, 0} with nblocks = wpc_krylov_partials(n); update folds pa partials of
- *   p.Ap from part_a and leaves r.r partials in `partials`; direction folds pa
- *   r.z partials (part_a) and pb r.r partials (part_b). */
+ * wpc_cg_fold_grouped: step 1 = wpc_cg_update_fold with x = NULL, step 2 =
+ *   wpc_cg_direction_fold with x, per task {x, r, p, ap, z, part_a, part_b, state,
+ *   partials, n, nblocks, block0, pa, pb, nvec, 0} with nblocks =
+ *   wpc_krylov_partials(n); update folds pa partials of p.Ap from part_a and leaves
+ *   r.r partials in `partials`; direction folds pa r.z partials (part_a) and pb r.r
+ *   partials (part_b) and advances x. */
 int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int val_dtype,
                             int s_dtype, int has_d, int dot, int vec_dtype, double alpha,
                             void* stream);

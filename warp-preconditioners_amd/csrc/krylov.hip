@@ -292,7 +292,11 @@ struct StateLanes {
 // latency (bounded so the prefetched values stay in registers).
 template <int NV> constexpr int kPrefetch = NV <= 2 ? 4 : (NV <= 4 ? 2 : 1);
 
-template <class T, int NV>
+// XD: x is advanced by the direction step (x += ALPHA p there, before p is
+// replaced), so this step reads only r and Ap and leaves x and p alone: one
+// vector stream fewer per iteration than advancing x here, where p must be read
+// for x alone.  Each x element takes the same operation either way.
+template <class T, int NV, bool XD>
 __device__ __forceinline__ void cg_update_fold_body(
     int blk, int nblocks, int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
@@ -317,9 +321,11 @@ __device__ __forceinline__ void cg_update_fold_body(
 #pragma unroll
             for (int v = 0; v < NVM; ++v) if (v < nv) {
                 const long k = i * nv + v;
-                xv[j][v] = x[k];
+                if constexpr (!XD) {
+                    xv[j][v] = x[k];
+                    pv[j][v] = p[k];
+                }
                 rv[j][v] = r[k];
-                pv[j][v] = p[k];
                 av[j][v] = ap[k];
             }
         }
@@ -358,14 +364,14 @@ __device__ __forceinline__ void cg_update_fold_body(
 #pragma unroll
             for (int v = 0; v < NVM; ++v) if (v < nv) {
                 const bool step = alpha[v] != T(0);
-                xn[v] = step ? xv[j][v] + alpha[v] * pv[j][v] : xv[j][v];
+                if constexpr (!XD) xn[v] = step ? xv[j][v] + alpha[v] * pv[j][v] : xv[j][v];
                 rn[v] = step ? rv[j][v] - alpha[v] * av[j][v] : rv[j][v];
                 acc[v] += (double)rn[v] * (double)rn[v];
             }
             if (moves) {
 #pragma unroll
                 for (int v = 0; v < NVM; ++v) if (v < nv) {
-                    x[i * nv + v] = xn[v];
+                    if constexpr (!XD) x[i * nv + v] = xn[v];
                     r[i * nv + v] = rn[v];
                 }
             }
@@ -377,7 +383,7 @@ __device__ __forceinline__ void cg_update_fold_body(
             const long k = i * nv + v;
             T rk = r[k];
             if (alpha[v] != T(0)) {
-                x[k] += alpha[v] * p[k];
+                if constexpr (!XD) x[k] += alpha[v] * p[k];
                 rk -= alpha[v] * ap[k];
                 r[k] = rk;
             }
@@ -393,21 +399,23 @@ __device__ __forceinline__ void cg_update_fold_body(
     }
 }
 
-template <class T, int NV>
+template <class T, int NV, bool XD>
 __global__ __launch_bounds__(kKrylovThreads) void cg_update_fold_kernel(
     int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
     const T* __restrict__ ap, int P, const double* __restrict__ part_denom, double* state,
     double* __restrict__ partials) {
     __shared__ double red[kKrylovThreads];
     __shared__ double sums[8];
-    cg_update_fold_body<T, NV>(blockIdx.x, gridDim.x, n, x, r, p, ap, P, part_denom, state,
-                               partials, red, sums);
+    cg_update_fold_body<T, NV, XD>(blockIdx.x, gridDim.x, n, x, r, p, ap, P, part_denom, state,
+                                   partials, red, sums);
 }
 
-template <class T, int NV>
+// XD: also x += ALPHA p (the update's step, for an update that left x alone),
+// from the p this step replaces.
+template <class T, int NV, bool XD>
 __device__ __forceinline__ void cg_direction_fold_body(
-    int blk, int nblocks, int n, const T* __restrict__ z, T* __restrict__ p, int P_rz,
-    const double* __restrict__ part_rz, int P_rr, const double* __restrict__ part_rr,
+    int blk, int nblocks, int n, const T* __restrict__ z, T* __restrict__ p, T* __restrict__ x,
+    int P_rz, const double* __restrict__ part_rz, int P_rr, const double* __restrict__ part_rr,
     double* state, double* s_rz, double* s_rr, int nv_rt = NV) {
     constexpr int PF = NV > 0 ? kPrefetch<NV> : 1;
     constexpr int NVM = NV > 0 ? NV : 8;
@@ -415,14 +423,14 @@ __device__ __forceinline__ void cg_direction_fold_body(
     long begin, end;
     chunk_of(n, blk, nblocks, &begin, &end);
     // As in the update: state fields, partials, then rows, all in flight together.
-    StateLanes<WPC_KS_ACTIVE, WPC_KS_RZ_OLD> st;
+    StateLanes<WPC_KS_ACTIVE, WPC_KS_RZ_OLD, (XD ? WPC_KS_ALPHA : -1)> st;
     st.load(state, nv);
     FoldRegs<kKrylovThreads, (NV > 0 ? NV : 1)> fold_z, fold_r;
     if constexpr (NV > 0) {
         fold_z.load(part_rz, P_rz);
         fold_r.load(part_rr, P_rr);
     }
-    T zv[PF][NVM], pv[PF][NVM];
+    T zv[PF][NVM], pv[PF][NVM], xv[PF][NVM];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
@@ -431,6 +439,7 @@ __device__ __forceinline__ void cg_direction_fold_body(
             for (int v = 0; v < NVM; ++v) if (v < nv) {
                 zv[j][v] = z[i * nv + v];
                 pv[j][v] = p[i * nv + v];
+                if constexpr (XD) xv[j][v] = x[i * nv + v];
             }
         }
     }
@@ -450,13 +459,14 @@ __device__ __forceinline__ void cg_direction_fold_body(
         fold_all<kKrylovThreads>(part_rz, P_rz, nv, s_rz);
         fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
     }
-    T beta[NVM];
+    T beta[NVM], alpha[NVM];
     bool act[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; ++v) if (v < nv) {
         act[v] = st.get(0, nv, v) != 0.0;
         const double b = act[v] ? s_rz[v] / st.get(1, nv, v) : 0.0;
         beta[v] = (T)b;
+        if constexpr (XD) alpha[v] = (T)st.get(2, nv, v);
         if (blk == 0 && threadIdx.x == v) {
             state[WPC_KS_RZ_NEW * nv + v] = s_rz[v];
             state[WPC_KS_RR_NEW * nv + v] = s_rr[v];
@@ -477,8 +487,12 @@ __device__ __forceinline__ void cg_direction_fold_body(
             const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
             if (i < end) {
 #pragma unroll
-                for (int v = 0; v < NVM; ++v) if (v < nv)
+                for (int v = 0; v < NVM; ++v) if (v < nv) {
                     p[i * nv + v] = act[v] ? zv[j][v] + beta[v] * pv[j][v] : pv[j][v];
+                    if constexpr (XD)
+                        x[i * nv + v] = alpha[v] != T(0) ? xv[j][v] + alpha[v] * pv[j][v]
+                                                         : xv[j][v];
+                }
             }
         }
     }
@@ -487,18 +501,22 @@ __device__ __forceinline__ void cg_direction_fold_body(
         for (int v = 0; v < NVM; ++v) if (v < nv) {
             if (!act[v]) continue;
             const long k = i * nv + v;
-            p[k] = z[k] + beta[v] * p[k];
+            const T pk = p[k];
+            if constexpr (XD)
+                if (alpha[v] != T(0)) x[k] += alpha[v] * pk;
+            p[k] = z[k] + beta[v] * pk;
         }
     }
 }
 
-template <class T, int NV>
+template <class T, int NV, bool XD>
 __global__ __launch_bounds__(kKrylovThreads) void cg_direction_fold_kernel(
-    int n, const T* __restrict__ z, T* __restrict__ p, int P_rz, const double* __restrict__ part_rz,
-    int P_rr, const double* __restrict__ part_rr, double* state) {
+    int n, const T* __restrict__ z, T* __restrict__ p, T* __restrict__ x, int P_rz,
+    const double* __restrict__ part_rz, int P_rr, const double* __restrict__ part_rr,
+    double* state) {
     __shared__ double s_rz[8], s_rr[8];
-    cg_direction_fold_body<T, NV>(blockIdx.x, gridDim.x, n, z, p, P_rz, part_rz, P_rr, part_rr,
-                                  state, s_rz, s_rr);
+    cg_direction_fold_body<T, NV, XD>(blockIdx.x, gridDim.x, n, z, p, x, P_rz, part_rz, P_rr,
+                                      part_rr, state, s_rz, s_rr);
 }
 
 // One CG vector step of a grouped launch (all fields 64-bit, in this order):
@@ -552,9 +570,9 @@ __global__ __launch_bounds__(kKrylovThreads) void grouped_update_fold_kernel(
     const int g = blockIdx.x;
     const CgTask& t = tasks[task_of(tasks, ntasks, g, &s_task)];
     if (t.n == 0) return;  // capacity padding
-    cg_update_fold_body<T, NV>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (T*)t.x, (T*)t.r,
-                               (const T*)t.p, (const T*)t.ap, (int)t.pa, t.part_a, t.state,
-                               t.partials, red, sums, (int)t.nv);
+    cg_update_fold_body<T, NV, true>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (T*)t.x,
+                                     (T*)t.r, (const T*)t.p, (const T*)t.ap, (int)t.pa,
+                                     t.part_a, t.state, t.partials, red, sums, (int)t.nv);
 }
 
 template <class T, int NV>
@@ -565,9 +583,9 @@ __global__ __launch_bounds__(kKrylovThreads) void grouped_direction_fold_kernel(
     const int g = blockIdx.x;
     const CgTask& t = tasks[task_of(tasks, ntasks, g, &s_task)];
     if (t.n == 0) return;  // capacity padding
-    cg_direction_fold_body<T, NV>(g - (int)t.block0, (int)t.nblocks, (int)t.n, (const T*)t.z,
-                                  (T*)t.p, (int)t.pa, t.part_a, (int)t.pb, t.part_b, t.state,
-                                  s_rz, s_rr, (int)t.nv);
+    cg_direction_fold_body<T, NV, true>(g - (int)t.block0, (int)t.nblocks, (int)t.n,
+                                        (const T*)t.z, (T*)t.p, (T*)t.x, (int)t.pa, t.part_a,
+                                        (int)t.pb, t.part_b, t.state, s_rz, s_rr, (int)t.nv);
 }
 
 // p = z + BETA p for slots that stepped.
@@ -701,16 +719,24 @@ int wpc_cg_update_fold(int n, int nvec, void* x, void* r, const void* p, const v
                        void* stream) {
     const int P = krylov_blocks(n);
     hipStream_t s = as_stream(stream);
-    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_update_fold_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, nparts, part_denom, state, partials)))
+    if (x == nullptr) {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_update_fold_kernel<T, NV, true>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, nparts, part_denom, state, partials)))
+    } else {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_update_fold_kernel<T, NV, false>), dim3(P), dim3(kKrylovThreads), 0, s, n, (T*)x, (T*)r, (const T*)p, (const T*)ap, nparts, part_denom, state, partials)))
+    }
     return launch_status();
 }
 
-int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, int dtype, int nparts_rz,
-                          const double* part_rz, int nparts_rr, const double* part_rr,
-                          double* state, void* stream) {
+int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int dtype,
+                          int nparts_rz, const double* part_rz, int nparts_rr,
+                          const double* part_rr, double* state, void* stream) {
     const int P = krylov_blocks(n);
     hipStream_t s = as_stream(stream);
-    VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_fold_kernel<T, NV>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, nparts_rz, part_rz, nparts_rr, part_rr, state)))
+    if (x != nullptr) {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_fold_kernel<T, NV, true>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, (T*)x, nparts_rz, part_rz, nparts_rr, part_rr, state)))
+    } else {
+        VT_SWITCH(dtype, NV_SWITCH(nvec, hipLaunchKernelGGL((cg_direction_fold_kernel<T, NV, false>), dim3(P), dim3(kKrylovThreads), 0, s, n, (const T*)z, (T*)p, (T*)x, nparts_rz, part_rz, nparts_rr, part_rr, state)))
+    }
     return launch_status();
 }
 

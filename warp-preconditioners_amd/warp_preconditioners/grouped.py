@@ -150,10 +150,12 @@ class GroupedCG:
             sell.append((g, gt))
             pr = _p(w.parts[1])
             st = _p(w.state)
-            fold_u.append([_p(w._group_x), r_, p_, ap_, 0, _p(sc.part_ap), 0, st, pr,
+            # (the update leaves x to the direction step, which advances it before
+            # replacing p)
+            fold_u.append([0, r_, 0, ap_, 0, _p(sc.part_ap), 0, st, pr,
                            n, nb, b_fold, pa, 0, nv, 0])
-            fold_d.append([0, 0, p_, 0, z_, _p(sc.part_rz), pr, st, 0, n, nb, b_fold, pa, nb, nv,
-                           0])
+            fold_d.append([_p(w._group_x), 0, p_, 0, z_, _p(sc.part_rz), pr, st, 0, n, nb, b_fold,
+                           pa, nb, nv, 0])
             b_sell += pa
             b_fold += nb
         nops = len(ops[0])

@@ -134,6 +134,7 @@ class KrylovWork:
         self.code = nat.code(dtype)
         self.graph = None
         self.graph_steps = 0
+        self._m_parts = None  # the preconditioner's partial count (fused steps)
         self.use_fused = (self.method == 0 and M is not None and bool(getattr(A, "fused", False))
                           and bool(getattr(M, "fused", False)) and hasattr(A, "apply_batch")
                           and hasattr(M, "apply_batch"))
@@ -185,20 +186,29 @@ class KrylovWork:
             # | beta | direction
             part, nparts = self.A.apply_batch(self.p, self.ap, self.ap, 1.0, 0.0, self.nvec,
                                               dot_w=self.p)
-            if nparts * self.nvec <= FOLD_IN_PLACE:
-                nat.call("wpc_cg_update_fold", self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
-                         nat.ptr(self.p), nat.ptr(self.ap), self.code, nparts, nat.ptr(part), st,
-                         pr, s)
+            fold_a = nparts * self.nvec <= FOLD_IN_PLACE
+            # With both steps folded, x advances in the direction step (x += ALPHA p
+            # before p is replaced) and the update reads only r and Ap.  Whether the
+            # direction step folds depends on the preconditioner's partial count, known
+            # from its previous application (the first step advances x in the update;
+            # a folded direction step is exact for any count, only slower for large).
+            xd = fold_a and self._m_parts is not None and \
+                max(self._m_parts, self.P) * self.nvec <= FOLD_IN_PLACE
+            if fold_a:
+                nat.call("wpc_cg_update_fold", self.n, self.nvec, None if xd else nat.ptr(x),
+                         nat.ptr(self.r), nat.ptr(self.p), nat.ptr(self.ap), self.code, nparts,
+                         nat.ptr(part), st, pr, s)
             else:
                 nat.call("wpc_cg_alpha", self.nvec, nparts, nat.ptr(part), st, s)
                 nat.call("wpc_cg_update", self.n, self.nvec, nat.ptr(x), nat.ptr(self.r),
                          nat.ptr(self.p), nat.ptr(self.ap), self.code, st, pr, s)
-            fold_a = nparts * self.nvec <= FOLD_IN_PLACE
             part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                               dot_w=self.r)
-            if fold_a and max(nparts, self.P) * self.nvec <= FOLD_IN_PLACE:
+            self._m_parts = nparts
+            if xd or (fold_a and max(nparts, self.P) * self.nvec <= FOLD_IN_PLACE):
                 nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
-                         nat.ptr(self.p), self.code, nparts, nat.ptr(part), self.P, pr, st, s)
+                         nat.ptr(self.p), nat.ptr(x) if xd else None, self.code, nparts,
+                         nat.ptr(part), self.P, pr, st, s)
             else:
                 # (after a fused update too: it leaves RZ as the scalar fold expects)
                 nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), self.P, pr, st, s)
