@@ -101,3 +101,66 @@ def test_refined_solve_reaches_true_residual(prec):
     dense = ss.bsr_matrix((val, col, off), shape=(nb * 3, nb * 3)).tocsc()
     xs = sla.spsolve(dense, bh)
     assert np.linalg.norm(host(x) - xs) / np.linalg.norm(xs) < 1e-5
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nvec", [1, 3, 8])
+def test_fused_steps_x_in_direction_bitwise(dtype, nvec):
+    # The fused update with x = NULL leaves x to the direction step (x += ALPHA p
+    # from the p that step replaces): x, r, p, the state and the r.r partials equal,
+    # bit for bit, the pair with x advanced in the update.  Slots: stepping, resting
+    # (RR <= TARGET) and stepping with ALPHA = 0 (p.Ap = inf: p moves, x does not).
+    import warp_preconditioners._native as nat
+    from warp_preconditioners.linear import nat_state_fields
+
+    n, pa, pz = 3001, 37, 41
+    P = nat.lib().wpc_krylov_partials(n)
+    g = torch.Generator(device="cpu").manual_seed(11 + nvec)
+
+    def rand(size, lo=-1.0, hi=1.0):
+        return torch.rand(size, generator=g, dtype=torch.float64) * (hi - lo) + lo
+
+    vecs = {k: rand(n * nvec).to(dtype).to(dev()) for k in ("x", "r", "p", "ap", "z")}
+    part_ap = rand(pa * nvec, lo=0.1, hi=1.0)
+    part_rz = rand(pz * nvec).to(dev())
+    state = torch.zeros(nat_state_fields() * nvec, dtype=torch.float64)
+    kind = [v % 3 for v in range(nvec)]  # 0 steps, 1 rests, 2 steps with ALPHA = 0
+    for v in range(nvec):
+        state[0 * nvec + v] = 1.0 + v                          # RZ
+        state[1 * nvec + v] = 2.0                              # RR
+        state[2 * nvec + v] = 3.0 if kind[v] == 1 else 1.0     # TARGET
+        if kind[v] == 2:
+            part_ap[v] = float("inf")
+    state, part_ap = state.to(dev()), part_ap.to(dev())
+    code = nat.code(dtype)
+    s = nat.stream()
+
+    def run(x_in_update):
+        t = {k: v.clone() for k, v in vecs.items()}
+        st = state.clone()
+        pr = torch.zeros(P * nvec, dtype=torch.float64, device=dev())
+        nat.call("wpc_cg_update_fold", n, nvec, nat.ptr(t["x"]) if x_in_update else None,
+                 nat.ptr(t["r"]), nat.ptr(t["p"]), nat.ptr(t["ap"]), code, pa, nat.ptr(part_ap),
+                 nat.ptr(st), nat.ptr(pr), s)
+        nat.call("wpc_cg_direction_fold", n, nvec, nat.ptr(t["z"]), nat.ptr(t["p"]),
+                 None if x_in_update else nat.ptr(t["x"]), code, pz, nat.ptr(part_rz), P,
+                 nat.ptr(pr), nat.ptr(st), s)
+        torch.cuda.synchronize()
+        return t, st, pr
+
+    a, sa, pra = run(True)
+    b, sb, prb = run(False)
+    for k in ("x", "r", "p"):
+        assert torch.equal(a[k], b[k]), k
+    assert torch.equal(sa, sb) and torch.equal(pra, prb)
+    npt = np.float64 if dtype == torch.float64 else np.float32
+    xs, x0, p0 = (host(t).reshape(n, nvec) for t in (b["x"], vecs["x"], vecs["p"]))
+    ps = host(b["p"]).reshape(n, nvec)
+    for v in range(nvec):
+        if kind[v] == 0:  # x + ALPHA p_old, with ALPHA as recorded
+            alpha = npt(float(sb[3 * nvec + v]))
+            assert alpha != 0
+            np.testing.assert_array_equal(xs[:, v], x0[:, v] + alpha * p0[:, v])
+        else:
+            np.testing.assert_array_equal(xs[:, v], x0[:, v])
+        assert np.array_equal(ps[:, v], p0[:, v]) == (kind[v] == 1), v
