@@ -318,6 +318,8 @@ class Slot:
                 rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
         if not recorded or rc != 0:
             # invalidated from outside: nothing recorded ran; run this chunk directly
+            if rc == 0 and handle.value:
+                nat.lib().wpc_graph_destroy(handle.value)
             for _ in range(steps):
                 self.work.step(self.x_iter)
             return

@@ -284,10 +284,18 @@ class StreamPlan:
             for m in self.CANDIDATES:
                 if nat.lib().wpc_graph_begin(nat.stream()) != 0:
                     break
-                for _ in range(reps):
-                    self._launch(m, x, None, z, 1.0, 0.0, nvec)
+                recorded = True
                 handle = ctypes.c_void_p()
-                if nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle)) != 0:
+                try:
+                    for _ in range(reps):
+                        self._launch(m, x, None, z, 1.0, 0.0, nvec)
+                except nat.NativeError:
+                    recorded = False  # (the capture is still ended below)
+                finally:
+                    rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
+                if rc != 0 or not recorded:
+                    if rc == 0 and handle.value:
+                        nat.lib().wpc_graph_destroy(handle.value)
                     break
                 graphs[m] = handle.value
         if len(graphs) < len(self.CANDIDATES):
