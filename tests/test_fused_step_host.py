@@ -1,0 +1,69 @@
+"""Host logic of the fused CG step sequence (linear.KrylovWork.step), driven by
+stand-in operators with the launches recorded instead of run: where x is advanced
+(the update on a solve's first step, the direction step once the preconditioner's
+partial count is known to fold in place) and the fallbacks for large counts."""
+
+import pytest
+import torch
+
+import warp_preconditioners._native as nat
+from warp_preconditioners.linear import FOLD_IN_PLACE, KrylovWork
+
+
+class _Op:
+    fused = True
+
+    def __init__(self, parts):
+        self.parts = parts
+
+    def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
+        return torch.zeros(self.parts * nvec, dtype=torch.float64), self.parts
+
+
+@pytest.fixture
+def launches(monkeypatch):
+    log = []
+    monkeypatch.setattr(nat, "call", lambda name, *args: log.append((name, args)))
+    monkeypatch.setattr(nat, "stream", lambda: 0)
+    return log
+
+
+def _work(a_parts, m_parts, n=262144, nvec=3):
+    nat.lib()  # the library loads on a host without a GPU (host-only helpers)
+    return KrylovWork("cg", _Op(a_parts), _Op(m_parts), n, nvec, torch.float64, "cpu")
+
+
+def test_x_moves_to_direction_after_first_step(launches):
+    w = _work(1024, 1024)
+    x = torch.zeros(w.n * w.nvec, dtype=torch.float64)
+    for _ in range(3):
+        w.step(x)
+    names = [name for name, _ in launches]
+    assert names == ["wpc_cg_update_fold", "wpc_cg_direction_fold"] * 3
+    (_, u0), (_, d0), (_, u1), (_, d1) = launches[:4]
+    assert u0[2] == x.data_ptr() and d0[4] is None   # first step: x in the update
+    assert u1[2] is None and d1[4] == x.data_ptr()   # then: x in the direction step
+    (_, u2), (_, d2) = launches[4:]
+    assert u2[2] is None and d2[4] == x.data_ptr()
+
+
+def test_large_preconditioner_counts_keep_x_in_update(launches):
+    nvec = 3
+    w = _work(1024, FOLD_IN_PLACE // nvec + 1, nvec=nvec)
+    x = torch.zeros(w.n * nvec, dtype=torch.float64)
+    for _ in range(2):
+        w.step(x)
+    names = [name for name, _ in launches]
+    assert names == ["wpc_cg_update_fold", "wpc_cg_beta", "wpc_cg_direction"] * 2
+    assert all(args[2] == x.data_ptr() for name, args in launches
+               if name == "wpc_cg_update_fold")
+
+
+def test_large_operator_counts_use_scalar_kernels(launches):
+    nvec = 3
+    w = _work(FOLD_IN_PLACE // nvec + 1, 1024, nvec=nvec)
+    x = torch.zeros(w.n * nvec, dtype=torch.float64)
+    for _ in range(2):
+        w.step(x)
+    names = [name for name, _ in launches]
+    assert names == ["wpc_cg_alpha", "wpc_cg_update", "wpc_cg_beta", "wpc_cg_direction"] * 2
