@@ -443,6 +443,28 @@ __device__ __forceinline__ void cg_direction_fold_body(
             }
         }
     }
+    T alpha[NVM];
+    if constexpr (XD) {
+        // x += ALPHA p needs no fold: its stores go out while the folds run
+        bool steps = false;
+#pragma unroll
+        for (int v = 0; v < NVM; ++v) if (v < nv) {
+            alpha[v] = (T)st.get(2, nv, v);
+            steps = steps || alpha[v] != T(0);
+        }
+        if (steps) {
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
+                if (i < end) {
+#pragma unroll
+                    for (int v = 0; v < NVM; ++v) if (v < nv)
+                        x[i * nv + v] = alpha[v] != T(0) ? xv[j][v] + alpha[v] * pv[j][v]
+                                                         : xv[j][v];
+                }
+            }
+        }
+    }
     if constexpr (NV > 0 && NV <= 4) {
         // both folds through one cross-wavefront pass (one barrier pair, not two)
         double acc[8];
@@ -459,14 +481,13 @@ __device__ __forceinline__ void cg_direction_fold_body(
         fold_all<kKrylovThreads>(part_rz, P_rz, nv, s_rz);
         fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
     }
-    T beta[NVM], alpha[NVM];
+    T beta[NVM];
     bool act[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; ++v) if (v < nv) {
         act[v] = st.get(0, nv, v) != 0.0;
         const double b = act[v] ? s_rz[v] / st.get(1, nv, v) : 0.0;
         beta[v] = (T)b;
-        if constexpr (XD) alpha[v] = (T)st.get(2, nv, v);
         if (blk == 0 && threadIdx.x == v) {
             state[WPC_KS_RZ_NEW * nv + v] = s_rz[v];
             state[WPC_KS_RR_NEW * nv + v] = s_rr[v];
@@ -487,12 +508,8 @@ __device__ __forceinline__ void cg_direction_fold_body(
             const long i = begin + threadIdx.x + (long)j * kKrylovThreads;
             if (i < end) {
 #pragma unroll
-                for (int v = 0; v < NVM; ++v) if (v < nv) {
+                for (int v = 0; v < NVM; ++v) if (v < nv)
                     p[i * nv + v] = act[v] ? zv[j][v] + beta[v] * pv[j][v] : pv[j][v];
-                    if constexpr (XD)
-                        x[i * nv + v] = alpha[v] != T(0) ? xv[j][v] + alpha[v] * pv[j][v]
-                                                         : xv[j][v];
-                }
             }
         }
     }
