@@ -477,24 +477,38 @@ __device__ __forceinline__ void sell_body(
         else
             sell_accum<int, TV, TX, NV>(acc, base, len, row, (const int*)col, val, x, nv);
     }
+    // Every lane computes (a lane past n on row n-1's data, with no entries) and
+    // only the stores are predicated: a branch around the loads would leave them
+    // pending at its join, where the compiler then waits for every outstanding
+    // memory operation (these stores included) before the reduction below.
     double dot[NVM];
+    TX out[NVM];
 #pragma unroll
-    for (int v = 0; v < NVM; ++v) dot[v] = 0.0;
+    for (int v = 0; v < NVM; ++v) {
+        dot[v] = 0.0;
+        if (v >= nv) break;
+        const long k = (long)row * nv + v;
+        TA r = TA(0);
+        if (use_a) {
+            r = acc[v];
+            if (HS) r = sv * r;
+            if (HD) r += dv * uv[v];
+            r *= alpha;
+        }
+        if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
+        St<TX>::template st<TA>(out, v, r);
+        // (the dot takes z as stored; the empty asm keeps the product, and the wait
+        // for w, ahead of the stores instead of sunk past them)
+        if (DOT) {
+            if (live) dot[v] = (double)wv[v] * (double)St<TX>::template ld<TA>(out, v);
+            asm volatile("" : "+v"(dot[v]));
+        }
+    }
     if (live) {
 #pragma unroll
         for (int v = 0; v < NVM; ++v) {
             if (v >= nv) break;
-            const long k = (long)row * nv + v;
-            TA r = TA(0);
-            if (use_a) {
-                r = acc[v];
-                if (HS) r = sv * r;
-                if (HD) r += dv * uv[v];
-                r *= alpha;
-            }
-            if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
-            St<TX>::template st<TA>(z, k, r);
-            if (DOT) dot[v] = (double)wv[v] * (double)St<TX>::template ld<TA>(z, k);
+            z[(long)row * nv + v] = out[v];
         }
     }
     if (DOT) {
