@@ -291,24 +291,33 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] += quad_xor<1>(acc[v]);
     }
+    // As in the sliced mapping: every lane computes (on row rc) and only the stores
+    // are predicated, so no pending load reaches a branch join ahead of the
+    // reduction (where the compiler would also wait for the stores).
+    const bool owner = live && sub == 0;
     double dot[NV];
+    TX out[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) dot[v] = 0.0;
-    if (live && sub == 0) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const long k = row * NV + v;
-            TA r = TA(0);
-            if (use_a) {
-                r = acc[v];
-                if (HS) r = sv * r;
-                if (HD) r += dv * uv[v];
-                r *= alpha;
-            }
-            if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
-            St<TX>::template st<TA>(z, k, r);
-            if (DOT) dot[v] = (double)wv[v] * (double)St<TX>::template ld<TA>(z, k);
+    for (int v = 0; v < NV; ++v) {
+        const long k = rc * NV + v;
+        TA r = TA(0);
+        if (use_a) {
+            r = acc[v];
+            if (HS) r = sv * r;
+            if (HD) r += dv * uv[v];
+            r *= alpha;
         }
+        if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
+        St<TX>::template st<TA>(out, v, r);
+        dot[v] = 0.0;
+        if (DOT) {
+            if (owner) dot[v] = (double)wv[v] * (double)St<TX>::template ld<TA>(out, v);
+            asm volatile("" : "+v"(dot[v]));
+        }
+    }
+    if (owner) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) z[row * NV + v] = out[v];
     }
     if (DOT) {
         block_sum<NV>(dot, red);
