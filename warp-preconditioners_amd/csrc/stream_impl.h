@@ -208,8 +208,13 @@ __global__ __launch_bounds__(kStreamRows) void stream_kernel(
                 r *= alpha;
             }
             if (beta != TA(0)) r += beta * St<TX>::template ld<TA>(y, k);
-            St<TX>::template st<TA>(z, k, r);
-            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) * (double)St<TX>::template ld<TA>(z, k);
+            // the dot from the value as stored, and w read, before the store: a load
+            // after it would make the wait for w a wait for the store as well
+            TX out[1];
+            St<TX>::template st<TA>(out, 0, r);
+            if (DOT) dot[v] = (double)St<TX>::template ld<TA>(w, k) *
+                              (double)St<TX>::template ld<TA>(out, 0);
+            z[k] = out[0];
         }
     }
     if (DOT) {
