@@ -105,9 +105,9 @@ int wpc_csr_stream(int nblocks, const int* blocks, int n, int lanes, const int* 
 
 /* Sliced-ELL copy of a scalar CSR matrix for the sliced product mapping: rows in
  * slices of 64, a slice's entries column-major (entry k of the row in position r
- * of slice s at (slice_ptr[s] + k) * 64 + r), rows zero-padded to the slice's
- * longest row on a column they already hold.  Position i holds row rows[i] when
- * `rows` (a permutation of 0..n-1) is given, row i when it is NULL.  Build:
+ * of slice s at (slice_ptr[s] + k) * 64 + r), rows padded to the slice's longest
+ * row with column -1 and value 0.  Position i holds row rows[i] when `rows` (a
+ * permutation of 0..n-1) is given, row i when it is NULL.  Build:
  * wpc_sell_lengths -> slice_ptr[0] = 0, slice_ptr[1..] = inclusive scan of
  * slice_len (wpc_scan_inclusive) -> allocate 64 * slice_ptr[nslices] entries ->
  * wpc_sell_fill -> wpc_sell_values (rerun after the CSR values change).  No
@@ -120,23 +120,29 @@ int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* rows
                   void* stream);
 int wpc_sell_values(long count, const int* sell_map, const void* values, int elem_bytes,
                     void* sell_values, void* stream);
-/* int16 copy of the sliced columns as differences col - row (the row of each
- * position, through `rows`); padding repeats the row's last difference, an empty
- * row points at itself.  overflow[0] (zeroed by the caller) counts entries whose
- * difference is outside [-32768, 32767] (and empty rows >= ncol), in which case
- * the int32 columns must be used.  Halves the index stream of banded
- * (mesh-ordered) matrices. */
-int wpc_sell_col16(int n, int ncol, const int* offsets, const int* rows, const int* slice_ptr,
-                   const int* slice_len, const int* sell_columns, short* sell_col16, int* overflow,
-                   void* stream);
+/* 16-bit copy of the sliced columns (run after wpc_sell_fill): slice_base[s]
+ * (int32, one per slice) = the smallest column of slice s; each entry stores
+ * column - slice_base[s] as an unsigned 16-bit offset (in the bits of
+ * `sell_col16`), padding 0xFFFF.  overflow[0] (zeroed by the caller) counts
+ * slices whose columns span more than 0xFFFE, in which case the int32 columns
+ * must be used.  Halves the index stream of banded (mesh-ordered) matrices. */
+int wpc_sell_col16(int n, const int* offsets, const int* rows, const int* slice_ptr,
+                   const int* slice_len, const int* sell_columns, short* sell_col16,
+                   int* slice_base, int* overflow, void* stream);
 
 /* The product of wpc_csr_stream on the sliced copy: one thread per row, storage
  * order serial sums (bit-identical to the streamed mapping), same epilogue and
  * fused inner product; rows as given to wpc_sell_fill; sell_columns are the int32
- * columns (col16 = 0) or the int16 differences of wpc_sell_col16 (col16 = 1);
- * partials: ceil(n / 256) workgroups (workgroup b covers positions 256b..+255). */
-int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* rows,
-                    const void* sell_columns, int col16, const void* sell_values, int val_dtype,
+ * columns (col16 = 0; then ncol * nvec * sizeof(vector) must not exceed 2^31) or
+ * the 16-bit offsets of wpc_sell_col16 (col16 = 1).  slice_base (one int32 per
+ * slice) is always read: the bases of wpc_sell_col16, or zeros with int32
+ * columns.
+ * Padding adds exactly +0 whatever x holds (its gathers fall outside the buffer
+ * descriptor's range and read 0).  partials: ceil(n / 256) workgroups (workgroup b
+ * covers positions 256b..+255). */
+int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* slice_base,
+                    const int* rows, const void* sell_columns, int col16,
+                    const void* sell_values, int val_dtype,
                     const void* s, int s_dtype, const void* d, const void* u, const void* x,
                     const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
                     const void* w, double* partials, double* out, void* stream);
@@ -465,7 +471,8 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int 
  * launch computes, bit for bit.
  *
  * wpc_sell_stream_grouped: per task {slice_ptr, slice_len, rows, sell_columns,
- *   sell_values, s, d, u, x, z, w, partials, n, block0, col16, nvec} -- the wpc_sell_stream
+ *   sell_values, s, d, u, x, z, w, partials, n, block0, slice_base,
+ *   nvec | 256 * col16} -- the wpc_sell_stream
  *   product with beta = 0 on each system (ceil(n/256) workgroups); s_dtype 0 means
  *   no row scale, has_d selects the d .* u term, dot the w . z partials.
  * wpc_cg_fold_grouped: step 1 = wpc_cg_update_fold with x = NULL, step 2 =

@@ -67,3 +67,64 @@ def test_large_operator_counts_use_scalar_kernels(launches):
         w.step(x)
     names = [name for name, _ in launches]
     assert names == ["wpc_cg_alpha", "wpc_cg_update", "wpc_cg_beta", "wpc_cg_direction"] * 2
+
+
+def test_partials_sized_for_the_launched_mapping():
+    # A plan fixed to 32 (or 64) lanes per row launches ceil(32 n / 256) workgroups,
+    # twice those of 16 lanes: the fused-dot partials must hold one per workgroup
+    # and vector of that launch.
+    import numpy as np
+    import scipy.sparse as ss
+
+    from warp_preconditioners.sparse import StreamPlan, from_scipy
+
+    nat.lib()
+    a = ss.random(5000, 5000, density=0.002, random_state=np.random.default_rng(3), format="csr")
+    A = from_scipy(a + ss.eye(5000), "cpu")
+    for fixed in (None, 16, 32, 64):
+        plan = StreamPlan(A, lanes=fixed)
+        for nvec in (1, 3, 8):
+            need = plan.partials_size(nvec, fixed)
+            launched = plan.workgroups(plan.SLICED if fixed is None else fixed)
+            assert need >= launched * nvec
+            assert need >= max(plan.workgroups(m) for m in plan.CANDIDATES) * nvec
+    plan = StreamPlan(A, lanes=32)
+    assert plan.partials_size(1, 32) == plan.workgroups(32) == -(-5000 * 32 // 256)
+
+
+def test_graph_replay_keyed_on_scratch_generation(monkeypatch):
+    # KrylovWork replays a captured chunk only while x and every operator scratch
+    # buffer are the ones it captured; a reallocation (generation bump) recaptures.
+    import warp_preconditioners.linear as lin
+
+    captured = []
+
+    class _Graph:
+        def replay(self):
+            captured.append("replay")
+
+    class _Ctx:
+        def __init__(self, g):
+            captured.append("capture")
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *exc):
+            return False
+
+    monkeypatch.setattr(lin.torch.cuda, "CUDAGraph", _Graph)
+    monkeypatch.setattr(lin.torch.cuda, "graph", _Ctx)
+    w = _work(64, 64)
+    monkeypatch.setattr(w, "step", lambda x: None)
+    x = torch.zeros(8, dtype=torch.float64)
+    w.run(x, 25, True)
+    w.run(x, 25, True)
+    assert captured == ["capture", "replay", "replay"]
+    nat.bump_generation()  # e.g. the preconditioner's temporary grew for more RHS
+    w.run(x, 25, True)
+    assert captured == ["capture", "replay", "replay", "capture", "replay"]
+    w.run(x, 25, True)
+    assert captured[-1] == "replay" and captured.count("capture") == 2
+    w.run(torch.zeros(8, dtype=torch.float64), 25, True)  # another x: recapture
+    assert captured.count("capture") == 3

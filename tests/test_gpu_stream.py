@@ -213,11 +213,14 @@ def test_sliced_copy_follows_refit():
     np.testing.assert_array_equal(host(z), host(zr))
 
 
-@pytest.mark.parametrize("far", [False, True])
+@pytest.mark.parametrize("far", [False, "wide", True])
 def test_sliced_int16_columns_and_fallback(oracle, far):
-    # Column differences within int16 are stored as such (a banded matrix); one
-    # entry 69,999 columns from its row keeps the int32 columns.  Both give the
-    # oracle's product bit for bit.
+    # Columns are stored as 16-bit offsets from each slice's smallest column when a
+    # slice spans at most 65,534 columns (a banded matrix; "wide": every row also
+    # reaches 33,000 columns back, beyond a signed 16-bit difference from the row,
+    # as the second-neighbour planes of a 128^3 mesh do); one entry 69,999 columns
+    # from its row keeps the int32 columns.  All give the oracle's product bit for
+    # bit.
     from warp_preconditioners.sparse import StreamPlan
 
     n = 70000
@@ -225,9 +228,13 @@ def test_sliced_int16_columns_and_fallback(oracle, far):
     off = rng.integers(-300, 300, size=(n, 5))
     rows = np.repeat(np.arange(n), 5)
     cols = np.clip(rows + off.ravel(), 0, n - 1)
+    if far == "wide":
+        rows = np.concatenate([rows, np.arange(33000, n)])
+        cols = np.concatenate([cols, np.arange(33000, n) - 33000])
     a = ss.csr_matrix((rng.normal(size=rows.size), (rows, cols)), shape=(n, n))
+    a = a.tolil()
     a[17, :] = 0  # an empty row
-    if far:
+    if far is True:
         a[3, n - 1] = 2.5
     a = a.tocsr()
     a.eliminate_zeros()
@@ -236,7 +243,7 @@ def test_sliced_int16_columns_and_fallback(oracle, far):
     x = rng.normal(size=(n, 2))
     z = torch.empty(n * 2, dtype=torch.float64, device=dev())
     plan.run(torch.tensor(x.ravel(), device=dev()), None, z, 1.0, 0.0, 2)
-    assert plan._sell[6] == (not far)
+    assert plan._sell[6] == (far is not True)
     ref = np.stack([oracle.spmv(oracle_csr(oracle, a), np.ascontiguousarray(x[:, v]))
                     for v in range(2)], axis=1)
     np.testing.assert_array_equal(host(z).reshape(n, 2), ref)
@@ -271,7 +278,7 @@ def test_sliced_every_slice_length_bitwise(oracle, monkeypatch, col16):
     z = torch.empty(n, dtype=torch.float64, device=dev())
     plan.run(torch.tensor(x, device=dev()), torch.tensor(y, device=dev()), z, 0.7, -0.3)
     assert bool(plan._sell[6]) == col16
-    lens = host(plan._sliced()[1])
+    lens = np.abs(host(plan._sliced()[1]))
     assert set(range(25)) <= set(lens.tolist())
     np.testing.assert_array_equal(host(z), oracle.spmv(oracle_csr(oracle, a, np.float64), x, y, 0.7, -0.3))
     k = 3
@@ -321,3 +328,114 @@ def test_lanes_mapping_batched_epilogue(lanes, nvec):
         outs.append((host(Z), host(dots)))
     np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=1e-10)
+
+
+@pytest.mark.parametrize("col16", [True, False])
+def test_sliced_padding_never_multiplies_inf_or_nan(oracle, monkeypatch, col16):
+    # Padding entries of the sliced copy add nothing, not 0 * x: their gathers fall
+    # outside the buffer descriptor's range and read 0.  An empty row gives
+    # beta * y although x holds NaN in the columns next to it, and a row shorter
+    # than its slice whose last column holds inf gives inf as the serial product
+    # does (0 * inf would give NaN).  alpha != 0.
+    from warp_preconditioners.sparse import StreamPlan
+
+    monkeypatch.setattr(StreamPlan, "COL16", col16)
+    monkeypatch.setattr(StreamPlan, "SORT_GAIN", 1e9)
+    rng = np.random.default_rng(31)
+    n = 256
+    a = ss.lil_matrix((n, n))
+    for r in range(n):
+        for c in sorted(set(rng.integers(max(0, r - 20), min(n, r + 20), size=6).tolist())):
+            a[r, c] = rng.normal()
+    a[5, :] = 0                   # empty row
+    a[70, :] = 0
+    a[70, 71] = 1.5               # one entry; its slice's longest row has more
+    a[130, :] = 0
+    a[130, 0] = 2.0               # a row whose only column is 0
+    a = a.tocsr()
+    a.eliminate_zeros()
+    A = upload(a)
+    plan = StreamPlan(A, lanes=StreamPlan.SLICED)
+    x = rng.normal(size=n)
+    x[0] = np.nan
+    x[5] = np.nan
+    x[71] = np.inf
+    y = rng.normal(size=n)
+    z = torch.empty(n, dtype=torch.float64, device=dev())
+    plan.run(torch.tensor(x, device=dev()), torch.tensor(y, device=dev()), z, 0.8, 0.5)
+    assert bool(plan._sell[6]) == col16
+    got = host(z)
+    ref = oracle.spmv(oracle_csr(oracle, a), x, y, 0.8, 0.5)
+    assert got[5] == 0.5 * y[5]
+    assert got[70] == np.inf
+    np.testing.assert_array_equal(got, ref)   # NaN where the serial product has NaN
+    cols = host(plan._sliced()[2])
+    pad = (cols == -1) if not col16 else (cols.view(np.uint16) == 0xFFFF)
+    assert pad.sum() > 0                      # padding is stored with the marker
+
+
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_fused_cg_many_lanes_partials(oracle, lanes):
+    # Fused CG with the operator and both FSAI passes fixed to `lanes` lanes per
+    # row: the partials of those launches (2x / 4x the workgroups of 16 lanes) fit
+    # the plans' buffers, and the iterates follow the oracle's PCG.
+    from warp_preconditioners import FSAI, SparseOperator, linear
+    from warp_preconditioners.sparse import StreamPlan
+
+    size = 24
+    d = ss.diags([-np.ones(size - 1), 2 * np.ones(size), -np.ones(size - 1)], [-1, 0, 1])
+    k = ss.kronsum(d, d).tocsr()
+    q = (ss.eye(size * size) / (size + 1) ** 2 + k @ k * (size + 1) ** 2).tocsr()
+    b = np.random.default_rng(13).normal(size=size * size)
+    Q = upload(q)
+    if lanes == 32:
+        op = SparseOperator(Q, row_lanes=32)
+        P = FSAI(Q, max_row_size=8, apply_lanes=32)
+    else:
+        from warp_preconditioners.linear import _MatrixOperator
+
+        op = _MatrixOperator(Q, lanes=64)
+        P = FSAI(Q, max_row_size=8)
+        P._plans = (StreamPlan(P.G, 64), StreamPlan(P.GT, 64))
+    for nvec in (1, 3):
+        B = torch.tensor(np.tile(b[:, None], (1, nvec)), device=dev()).contiguous()
+        X = torch.zeros_like(B)
+        out = linear.solve_batched("cg", op, B, X, M=P, tol=1e-10, maxiter=5000, check_every=1)
+        assert out["work"].use_fused
+        plan = op._stream() if lanes == 32 else op._plan
+        assert plan._partials.numel() >= plan.workgroups(lanes) * nvec
+        for p in P.plans():
+            assert p._partials is None or p._partials.numel() >= p.workgroups(lanes) * nvec
+        Po = oracle.fsai(oracle_csr(oracle, q), max_row_size=8)
+        xo, ito, _ = oracle.krylov("cg", b, A=oracle_csr(oracle, q), P=Po, tol=1e-10,
+                                   maxiter=5000, check_every=1)
+        for v in range(nvec):
+            assert abs(out["iterations"][v] - ito) <= max(2, 0.02 * ito)
+            np.testing.assert_allclose(host(X[:, v]), xo, rtol=1e-8, atol=1e-10 * np.abs(xo).max())
+
+
+def test_graph_replay_after_scratch_grew():
+    # A KrylovWork's captured chunk holds the operators' scratch addresses; after a
+    # solve with more right-hand sides grew that scratch, reusing the work recaptures
+    # (a stale replay would write into freed blocks that other tensors now own).
+    from warp_preconditioners import FSAI, linear
+
+    rng = np.random.default_rng(40)
+    a = random_spd(rng, 3000, 0.003)
+    A = upload(a)
+    P = FSAI(A, max_row_size=8)
+    b = torch.tensor(rng.normal(size=(3000, 1)), device=dev())
+    x1 = torch.zeros_like(b)
+    out1 = linear.solve_batched("cg", A, b, x1, M=P, tol=1e-10, check_every=5)
+    first = host(x1).copy()
+    B8 = torch.tensor(rng.normal(size=(3000, 8)), device=dev())
+    linear.solve_batched("cg", A, B8, torch.zeros_like(B8), M=P, tol=1e-6, check_every=5)
+    torch.cuda.synchronize()
+    sentinels = [torch.full((3000 * k,), 7.0, dtype=torch.float64, device=dev()) for k in (1, 2, 4)]
+    x1.zero_()
+    out2 = linear.solve_batched("cg", A, b, x1, M=P, tol=1e-10, check_every=5, work=out1["work"])
+    torch.cuda.synchronize()
+    assert out2["iterations"] == out1["iterations"]
+    np.testing.assert_array_equal(host(x1), first)
+    for s in sentinels:
+        assert bool((s == 7.0).all())

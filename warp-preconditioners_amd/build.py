@@ -56,7 +56,8 @@ def compile_one(src: Path) -> Path:
 def build(jobs: int | None = None, verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     LIB.parent.mkdir(parents=True, exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
+    # the product kernels (many template instantiations each) start first
+    sources = sorted(CSRC.glob("*.hip"), key=lambda p: (not p.stem.startswith("stream_"), p.stem))
     jobs = jobs or min(len(sources), os.cpu_count() or 4, 8)
     with ThreadPoolExecutor(jobs) as pool:
         objects = list(pool.map(compile_one, sources))

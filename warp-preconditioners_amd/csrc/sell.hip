@@ -7,6 +7,8 @@
 // position i of the slices: ordering rows by length inside windows of a few
 // hundred rows keeps the padding small when row lengths vary.
 
+#include <climits>
+
 #include "wpc_common.h"
 
 namespace wpc {
@@ -27,8 +29,10 @@ __global__ __launch_bounds__(256) void sell_lengths_kernel(int n, int nslices,
     if ((threadIdx.x & (kWave - 1)) == 0 && s < nslices) slice_len[s] = len;
 }
 
-// Column-major slice entries; padding repeats the row's last column (or column 0
-// for an empty row) with map -1.
+// Column-major slice entries; a padding entry stores column -1 and map -1.  The
+// products gather through a buffer descriptor whose range a padding entry's
+// offset lies outside of: such a load returns 0 without a memory access, and the
+// padding value is 0, so padding adds exactly +0 whatever x holds (inf, NaN).
 __global__ __launch_bounds__(256) void sell_fill_kernel(int n, int nslices, const int* __restrict__ off,
                                                         const int* __restrict__ col,
                                                         const int* __restrict__ rows,
@@ -44,50 +48,56 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(int n, int nslices, cons
     const long row = idx < n ? (rows != nullptr ? rows[idx] : idx) : -1;
     const int a = row >= 0 ? off[row] : 0;
     const int b = row >= 0 ? off[row + 1] : 0;
-    const int pad = b > a ? col[b - 1] : 0;
     for (int k = 0; k < len; ++k) {
         const long p = base + (long)k * kSlice;
         const bool real = a + k < b;
-        scol[p] = real ? col[a + k] : pad;
+        scol[p] = real ? col[a + k] : -1;
         smap[p] = real ? a + k : -1;
     }
 }
 
-// int16 copy of the columns as differences col - row; overflow[0] counts entries
-// whose difference does not fit (the int32 copy must then be used).  Padding
-// repeats the row's last difference; an empty row pads with its own index (a
-// valid column of a square matrix, row < ncol).
-__global__ __launch_bounds__(256) void sell_col16_kernel(int n, int ncol, int nslices,
+// 16-bit copy of the columns: slice_base[s] = the smallest column of slice s, and
+// each entry stores col - slice_base[s] as an unsigned 16-bit offset; 0xFFFF marks
+// padding (the products' descriptor spans offsets 0..0xFFFE).  overflow[0] counts
+// slices whose columns span more than 0xFFFE (the int32 columns must then be
+// used).  One wavefront per slice.
+constexpr int kPad16 = 0xFFFF;
+
+__global__ __launch_bounds__(256) void sell_col16_kernel(int n, int nslices,
                                                          const int* __restrict__ off,
                                                          const int* __restrict__ rows,
                                                          const int* __restrict__ slice_ptr,
                                                          const int* __restrict__ slice_len,
                                                          const int* __restrict__ scol,
-                                                         short* __restrict__ out, int* overflow) {
+                                                         unsigned short* __restrict__ out,
+                                                         int* __restrict__ slice_base,
+                                                         int* overflow) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)nslices * kSlice) return;
+    if (idx >= (long)nslices * kSlice) return;  // whole wavefronts: kSlice == kWave
     const int s = (int)(idx / kSlice);
     const long base = (long)slice_ptr[s] * kSlice + (idx % kSlice);
     const int len = slice_len[s];
     const long row = idx < n ? (rows != nullptr ? rows[idx] : idx) : -1;
     const int own = row >= 0 ? off[row + 1] - off[row] : 0;
-    int bad = 0;
-    long last = 0;
-    if (own == 0 && row >= ncol) bad = 1;  // no column of its own to point at
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int k = 0; k < own && k < len; ++k) {
+        const int c = scol[base + (long)k * kSlice];
+        lo = min(lo, c);
+        hi = max(hi, c);
+    }
+    for (int m = kWave / 2; m > 0; m >>= 1) {
+        lo = min(lo, __shfl_xor(lo, m, kWave));
+        hi = max(hi, __shfl_xor(hi, m, kWave));
+    }
+    const int b0 = lo == INT_MAX ? 0 : lo;
     for (int k = 0; k < len; ++k) {
         const long p = base + (long)k * kSlice;
-        long diff = last;
-        if (k < own) {
-            diff = (long)scol[p] - (row >= 0 ? row : 0);
-            if (diff < -32768 || diff > 32767) {
-                ++bad;
-                diff = 0;
-            }
-            last = diff;
-        }
-        out[p] = (short)diff;
+        out[p] = (unsigned short)(k < own ? scol[p] - b0 : kPad16);
     }
-    if (bad) atomicAdd(overflow, bad);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        slice_base[s] = b0;
+        if (lo != INT_MAX && (long)hi - lo > kPad16 - 1) atomicAdd(overflow, 1);
+    }
 }
 
 template <class E>
@@ -128,14 +138,14 @@ int wpc_sell_fill(int n, const int* offsets, const int* columns, const int* rows
     return launch_status();
 }
 
-int wpc_sell_col16(int n, int ncol, const int* offsets, const int* rows, const int* slice_ptr,
-                   const int* slice_len, const int* sell_columns, short* sell_col16, int* overflow,
-                   void* stream) {
+int wpc_sell_col16(int n, const int* offsets, const int* rows, const int* slice_ptr,
+                   const int* slice_len, const int* sell_columns, short* sell_col16,
+                   int* slice_base, int* overflow, void* stream) {
     const int ns = wpc_sell_slices(n);
     if (ns == 0) return WPC_OK;
     hipLaunchKernelGGL(sell_col16_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
-                       as_stream(stream), n, ncol, ns, offsets, rows, slice_ptr, slice_len,
-                       sell_columns, sell_col16, overflow);
+                       as_stream(stream), n, ns, offsets, rows, slice_ptr, slice_len,
+                       sell_columns, (unsigned short*)sell_col16, slice_base, overflow);
     return launch_status();
 }
 

@@ -47,20 +47,23 @@ int wpc_csr_stream(int nblocks, const int* blocks, int n, int lanes, const int* 
         return WPC_EARG;
     }
     if (s != nullptr && s_dtype != WPC_F32 && s_dtype != vec_dtype) return WPC_EDTYPE;
-    const StreamArgs a{nblocks, blocks, nullptr, nullptr, false, n, lg, offsets, columns, values, val_dtype, s,
+    const StreamArgs a{nblocks, blocks, nullptr, nullptr, nullptr, false, n, lg, offsets, columns,
+                       values, val_dtype, s,
                        s != nullptr && s_dtype == WPC_F32, d, u, x, y, z, vec_dtype, alpha, beta, w,
                        partials, out, as_stream(stream)};
     return stream_dispatch(nvec, a);
 }
 
-int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* rows,
-                    const void* sell_columns, int col16, const void* sell_values, int val_dtype,
+int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int* slice_base,
+                    const int* rows, const void* sell_columns, int col16,
+                    const void* sell_values, int val_dtype,
                     const void* s, int s_dtype, const void* d, const void* u, const void* x,
                     const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
                     const void* w, double* partials, double* out, void* stream) {
     if (n > 0 && (slice_ptr == nullptr || slice_len == nullptr)) return WPC_EARG;
+    if (n > 0 && slice_base == nullptr) return WPC_EARG;
     if (s != nullptr && s_dtype != WPC_F32 && s_dtype != vec_dtype) return WPC_EDTYPE;
-    const StreamArgs a{0, slice_ptr, slice_len, rows, col16 != 0, n, -2, nullptr,
+    const StreamArgs a{0, slice_ptr, slice_len, slice_base, rows, col16 != 0, n, -2, nullptr,
                        (const int*)sell_columns, sell_values,
                        val_dtype, s, s != nullptr && s_dtype == WPC_F32, d, u, x, y, z, vec_dtype,
                        alpha, beta, w, partials, out, as_stream(stream)};

@@ -30,8 +30,9 @@ struct StreamArgs {
     int nb;
     const int* blocks;  // streamed mapping: row partition; sliced mapping: slice_ptr
     const int* slice_len;  // sliced mapping only
+    const int* slice_base; // sliced mapping, 16-bit columns: smallest column per slice
     const int* rows;       // sliced mapping: row order of the slices (NULL: natural)
-    bool col16;            // sliced mapping: int16 column differences
+    bool col16;            // sliced mapping: 16-bit column offsets from slice_base
     int n;
     int lg;  // log2(lanes per row), -1 for the streamed mapping, -2 for the sliced one
     const int* off;
@@ -81,8 +82,9 @@ struct SellTask {
     double* partials;
     long n;
     long block0;
-    long col16;  // columns stored as int16 differences from the row
-    long nv;     // interleaved vectors of this system (1..8)
+    const int* slice_base;
+    long nv;  // interleaved vectors of this system (1..8) | 256 if the columns are
+              // 16-bit offsets from slice_base (else int32)
 };
 static_assert(sizeof(SellTask) == 16 * 8, "SellTask is 16 64-bit words");
 
@@ -333,21 +335,40 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 // Sliced-ELL mapping of the same product: rows in slices of 64 (one wavefront);
 // a slice's entries are stored column-major -- entry k of the row in position r of
 // the slice sits at (slice_ptr[s] + k) * 64 + r -- and every row is padded to the
-// slice's longest row with zero values on a column the row already holds.
-// Position i holds row rows[i] (rows == NULL: row i).  One thread per row reads
-// fully coalesced index/value streams, issues its gathers a group of entries at a
-// time, and sums in storage order, so results equal the streamed mapping's bit for bit
-// (padding adds exact zeros).  Same epilogue and fused inner product.
-// Column index of entry p of the row `row`: stored as int32, or as the int16
-// difference col - row when every entry of the matrix is within 32767 of its row
-// (banded meshes; 2 bytes per entry less to stream).
-template <class TI> __device__ __forceinline__ long sell_col(const TI* col, long p, int row);
-template <> __device__ __forceinline__ long sell_col<int>(const int* col, long p, int) {
-    return col[p];
+// slice's longest row with value 0.  Position i holds row rows[i] (rows == NULL:
+// row i).  One thread per row reads fully coalesced index/value streams, issues
+// its gathers a group of entries at a time, and sums in storage order, so results
+// equal the streamed mapping's bit for bit.  Same epilogue and fused inner product.
+//
+// Gathers go through a buffer descriptor over x (32-bit byte offsets, no 64-bit
+// address arithmetic).  Columns are int32 (descriptor over all of x, padding -1:
+// offset >= 2^31) or unsigned 16-bit offsets from the slice's smallest column
+// slice_base[s] (descriptor over 0xFFFF columns from there, padding 0xFFFF).  A
+// padding offset lies outside the descriptor's range, so its load returns 0 with
+// no memory access and padding adds exactly +0, whatever x holds.
+template <class TX> __device__ __forceinline__ TX buffer_ld(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ double buffer_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
-template <> __device__ __forceinline__ long sell_col<short>(const short* col, long p, int row) {
-    return (long)row + col[p];
+template <> __device__ __forceinline__ float buffer_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
+
+// Byte offset of an entry's x row inside the descriptor (row stride `stride`).
+template <class TI> __device__ __forceinline__ unsigned sell_off(const TI* col, long p, unsigned stride);
+template <> __device__ __forceinline__ unsigned sell_off<int>(const int* col, long p, unsigned stride) {
+    return (unsigned)col[p] * stride;
+}
+template <> __device__ __forceinline__ unsigned sell_off<unsigned short>(const unsigned short* col,
+                                                                         long p, unsigned stride) {
+    return (unsigned)col[p] * stride;
+}
+
+// Word 3 of a gfx9-family buffer descriptor (32-bit data format, no swizzle).
+constexpr int kBufferWord3 = 0x00020000;
+// Descriptor range of the int32 columns: a real column's offset is below 2^31
+// (the host keeps ncol * nvec * sizeof(TX) <= 2^31), padding's (-1) is not.
+constexpr unsigned kRange32 = 0x80000000u;
 
 // NV > 0: NV interleaved vectors; NV == 0: nv_rt (<= 8) vectors, the count a
 // batched launch reads per system.  Both perform the same operations per vector.
@@ -359,25 +380,25 @@ template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
 // one masked group of 12 (where 12 entries' gathers fit in registers), then 8 a
 // group of 8, 6-7 a masked group of 8, 3 a masked group of 4, otherwise groups of
 // 4, 2 and 1 (with many vectors, groups of 4 and the remainder 3 masked, 2 or 1:
-// kSellMain).  Column indices are held as 32-bit values (a column fits an int).  A masked
-// group's entries past the slice re-read its last entry (a cache hit; nothing past
-// the slice is read) and their products are discarded by a select, so the sums are
-// those of storage order.  Masking only where it wastes at most two gathers keeps
-// the gathers (which bound these passes) close to the stored entries.
+// kSellMain).  A masked group's entries past the slice re-read its last entry (a
+// cache hit; nothing past the slice is read) and their products are discarded by
+// a select, so the sums are those of storage order.  Masking only where it wastes
+// at most two gathers keeps the gathers (which bound these passes) close to the
+// stored entries.
 template <class TI, class TV, class TX, int NV, int G, bool FULL = true>
-__device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long base, int k, int row,
+__device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long base, int k,
+                                           __amdgpu_buffer_rsrc_t xr, unsigned stride,
                                            const TI* __restrict__ col,
-                                           const TV* __restrict__ val, const TX* __restrict__ x,
-                                           int nv, int m = G) {
+                                           const TV* __restrict__ val, int nv, int m = G) {
     using TA = typename AccOf<TX>::type;
     constexpr int NVM = kNvMax<NV>;
-    int c[G];
+    unsigned o[G];
     TA a[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
         const int e = FULL || j < m ? j : m - 1;
         const long p = base + (long)(k + e) * 64;
-        c[j] = (int)sell_col<TI>(col, p, row);
+        o[j] = sell_off<TI>(col, p, stride);
         a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
     }
 #pragma unroll
@@ -385,7 +406,7 @@ __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long b
 #pragma unroll
         for (int v = 0; v < NVM; ++v) {
             if (v < nv) {
-                const TA t = acc[v] + a[j] * St<TX>::template ld<TA>(x, (long)c[j] * nv + v);
+                const TA t = acc[v] + a[j] * (TA)buffer_ld<TX>(xr, o[j] + v * (unsigned)sizeof(TX));
                 acc[v] = FULL || j < m ? t : acc[v];
             }
         }
@@ -400,61 +421,65 @@ constexpr int kSellMain = kNvMax<NV> * sizeof(typename AccOf<TX>::type) <= 32 ? 
 
 template <class TI, class TV, class TX, int NV>
 __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long base, int len,
-                                           int row, const TI* __restrict__ col,
-                                           const TV* __restrict__ val, const TX* __restrict__ x,
-                                           int nv_rt) {
+                                           __amdgpu_buffer_rsrc_t xr, unsigned stride,
+                                           const TI* __restrict__ col,
+                                           const TV* __restrict__ val, int nv_rt) {
     const int nv = NV > 0 ? NV : nv_rt;
     int k = 0;
     if constexpr (kSellMain<TX, NV> == 4) {
         for (; k + 4 <= len; k += 4)
-            sell_group<TI, TV, TX, NV, 4>(acc, base, k, row, col, val, x, nv);
+            sell_group<TI, TV, TX, NV, 4>(acc, base, k, xr, stride, col, val, nv);
         const int r = len - k;
         if (r == 3)
-            sell_group<TI, TV, TX, NV, 4, false>(acc, base, k, row, col, val, x, nv, 3);
+            sell_group<TI, TV, TX, NV, 4, false>(acc, base, k, xr, stride, col, val, nv, 3);
         else if (r == 2)
-            sell_group<TI, TV, TX, NV, 2>(acc, base, k, row, col, val, x, nv);
+            sell_group<TI, TV, TX, NV, 2>(acc, base, k, xr, stride, col, val, nv);
         else if (r == 1)
-            sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
+            sell_group<TI, TV, TX, NV, 1>(acc, base, k, xr, stride, col, val, nv);
         return;
     }
     // 12 entries' gathers: at most 72 registers (3 double or 6 float vectors)
     constexpr bool kTwelve = kNvMax<NV> * sizeof(typename AccOf<TX>::type) <= 24;
     for (; len - k > (kTwelve ? 12 : 7); k += 8)
-        sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
+        sell_group<TI, TV, TX, NV, 8>(acc, base, k, xr, stride, col, val, nv);
     int r = len - k;
     if (kTwelve && r > 8) {
-        sell_group<TI, TV, TX, NV, 12, false>(acc, base, k, row, col, val, x, nv, r);
+        sell_group<TI, TV, TX, NV, 12, false>(acc, base, k, xr, stride, col, val, nv, r);
         return;
     }
     if (r == 8) {
-        sell_group<TI, TV, TX, NV, 8>(acc, base, k, row, col, val, x, nv);
+        sell_group<TI, TV, TX, NV, 8>(acc, base, k, xr, stride, col, val, nv);
         return;
     }
     if (r >= 6) {
-        sell_group<TI, TV, TX, NV, 8, false>(acc, base, k, row, col, val, x, nv, r);
+        sell_group<TI, TV, TX, NV, 8, false>(acc, base, k, xr, stride, col, val, nv, r);
         return;
     }
     if (r >= 4) {
-        sell_group<TI, TV, TX, NV, 4>(acc, base, k, row, col, val, x, nv);
+        sell_group<TI, TV, TX, NV, 4>(acc, base, k, xr, stride, col, val, nv);
         k += 4;
         r -= 4;
     }
     if (r == 3) {
-        sell_group<TI, TV, TX, NV, 4, false>(acc, base, k, row, col, val, x, nv, 3);
+        sell_group<TI, TV, TX, NV, 4, false>(acc, base, k, xr, stride, col, val, nv, 3);
         return;
     }
     if (r >= 2) {
-        sell_group<TI, TV, TX, NV, 2>(acc, base, k, row, col, val, x, nv);
+        sell_group<TI, TV, TX, NV, 2>(acc, base, k, xr, stride, col, val, nv);
         k += 2;
         r -= 2;
     }
-    if (r == 1) sell_group<TI, TV, TX, NV, 1>(acc, base, k, row, col, val, x, nv);
+    if (r == 1) sell_group<TI, TV, TX, NV, 1>(acc, base, k, xr, stride, col, val, nv);
 }
 
+// blk must be uniform across the workgroup (the slice header is read with scalar
+// loads, and the gather descriptor is built from it).  col16: the columns are
+// 16-bit offsets from slice_base[s]; else int32 and slice_base holds zeros.
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __device__ __forceinline__ void sell_body(
     int blk, int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
-    const int* __restrict__ rows, const void* __restrict__ col, bool col16,
+    const int* __restrict__ slice_base, const int* __restrict__ rows,
+    const void* __restrict__ col, bool col16,
     const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
     const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
     typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
@@ -464,14 +489,26 @@ __device__ __forceinline__ void sell_body(
     const int nv = NV > 0 ? NV : nv_rt;
     const int idx = blk * kStreamRows + threadIdx.x;
     const bool live = idx < n;
-    // Loads without branches (lanes past n read row n-1's data and discard it), so
-    // the slice header and the epilogue operands are in flight together.
+    // The wavefront's slice (uniform: a slice is one wavefront; a wavefront past
+    // the last row works on the last slice and stores nothing).  Its three header
+    // words are scalar loads issued together, with no condition between them and
+    // the kernel arguments.  Lanes past n read row n-1's epilogue operands and
+    // discard them; the padding positions of the last slice they gather from read 0.
+    const int wave0 = __builtin_amdgcn_readfirstlane(idx & ~63);
+    const int slice = min(wave0, n - 1) >> 6;
+    const int sp = slice_ptr[slice];
+    const int sl = slice_len[slice];
+    const int sb = slice_base[slice];
     const int idc = live ? idx : n - 1;
     const int row = rows != nullptr ? rows[idc] : idc;
     const bool use_a = alpha != TA(0);
-    const int slice = idc >> 6;
-    const long base = (long)slice_ptr[slice] * 64 + (idx & 63);
-    const int len = live && use_a ? slice_len[slice] : 0;
+    const long base = (long)sp * 64 + (idx & 63);
+    const int len = use_a && wave0 < n ? sl : 0;
+    const unsigned stride = (unsigned)(nv * sizeof(TX));
+    // (slice_base holds zeros with int32 columns, so the base needs no branch)
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(x + (long)sb * nv), (short)0, (int)(col16 ? 0xFFFFu * stride : kRange32),
+        kBufferWord3);
     TA sv = TA(1), dv = TA(0), uv[NVM], wv[NVM];
     if (HS) sv = (TA)s[row];
     if (HD) dv = St<TX>::template ld<TA>(d, row);
@@ -485,16 +522,16 @@ __device__ __forceinline__ void sell_body(
     TA acc[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; ++v) acc[v] = TA(0);
-    if (live && use_a) {
-        if (col16)
-            sell_accum<short, TV, TX, NV>(acc, base, len, row, (const short*)col, val, x, nv);
-        else
-            sell_accum<int, TV, TX, NV>(acc, base, len, row, (const int*)col, val, x, nv);
-    }
-    // Every lane computes (a lane past n on row n-1's data, with no entries) and
-    // only the stores are predicated: a branch around the loads would leave them
-    // pending at its join, where the compiler then waits for every outstanding
-    // memory operation (these stores included) before the reduction below.
+    // (len is uniform: the entry loop's control is scalar)
+    if (col16)
+        sell_accum<unsigned short, TV, TX, NV>(acc, base, len, xr, stride,
+                                               (const unsigned short*)col, val, nv);
+    else
+        sell_accum<int, TV, TX, NV>(acc, base, len, xr, stride, (const int*)col, val, nv);
+    // Every lane computes (a lane past n on row n-1's operands) and only the stores
+    // are predicated: a branch around the loads would leave them pending at its
+    // join, where the compiler then waits for every outstanding memory operation
+    // (these stores included) before the reduction below.
     double dot[NVM];
     TX out[NVM];
 #pragma unroll
@@ -539,14 +576,15 @@ __device__ __forceinline__ void sell_body(
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
-    const int* __restrict__ rows, const void* __restrict__ col, int col16,
+    const int* __restrict__ slice_base, const int* __restrict__ rows,
+    const void* __restrict__ col, int col16,
     const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
     const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
     typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
     double* __restrict__ partials) {
     __shared__ double red[kStreamRows];
-    sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, rows, col,
-                                           col16 != 0, val, s, d, u, x, y, z, alpha, beta, w,
+    sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, slice_base, rows,
+                                           col, col16 != 0, val, s, d, u, x, y, z, alpha, beta, w,
                                            partials, red);
 }
 
@@ -572,14 +610,15 @@ __global__ __launch_bounds__(kStreamRows) void grouped_sell_kernel(
         s_task = lo;
     }
     __syncthreads();
-    const SellTask& t = tasks[s_task];
+    // (uniform: the task's fields are read with scalar loads)
+    const SellTask& t = tasks[__builtin_amdgcn_readfirstlane(s_task)];
     if (t.n == 0) return;  // capacity padding of a captured launch
     const int blk = g - (int)t.block0;
     sell_body<TV, TX, TS, NV, HS, HD, DOT>(
-        blk, (int)t.n, t.slice_ptr, t.slice_len, t.rows, t.col, t.col16 != 0, (const TV*)t.val,
-        (const TS*)t.s,
+        blk, (int)t.n, t.slice_ptr, t.slice_len, t.slice_base, t.rows, t.col, (t.nv >> 8) & 1,
+        (const TV*)t.val, (const TS*)t.s,
         (const TX*)t.d, (const TX*)t.u, (const TX*)t.x, nullptr, (TX*)t.z, alpha, TA(0),
-        (const TX*)t.w, t.partials, red, (int)t.nv);
+        (const TX*)t.w, t.partials, red, (int)(t.nv & 0xFF));
 }
 
 template <class TV, class TX, class TS, int NV>
@@ -601,22 +640,20 @@ int grouped_launch(const GroupedArgs& a) {
     return launch_status();
 }
 
-template <int NV>
+// One vector format per translation unit (stream_grouped.hip: float64 vectors,
+// stream_grouped32.hip: float32), so the two build in parallel.
+template <int NV, class TX>
 int grouped_for_nv(const GroupedArgs& a) {
-#define GT(TV, TX)                                                                              \
+#define GT(TV)                                                                                  \
     do {                                                                                        \
         if (a.hs && a.s32) return grouped_launch<TV, TX, float, NV>(a);                         \
         return grouped_launch<TV, TX, TX, NV>(a);                                               \
     } while (0)
-    if (a.vec_dtype == WPC_F64) {
-        if (a.val_dtype == WPC_F64) GT(double, double);
-        if (a.val_dtype == WPC_F32) GT(float, double);
-        if (a.val_dtype == WPC_F16) GT(f16s, double);
-        if (a.val_dtype == WPC_BF16) GT(bf16s, double);
-    } else if (a.vec_dtype == WPC_F32) {
-        if (a.val_dtype == WPC_F32) GT(float, float);
-        if (a.val_dtype == WPC_F16) GT(f16s, float);
-        if (a.val_dtype == WPC_BF16) GT(bf16s, float);
+    if (a.val_dtype == WPC_F32) GT(float);
+    if (a.val_dtype == WPC_F16) GT(f16s);
+    if (a.val_dtype == WPC_BF16) GT(bf16s);
+    if constexpr (sizeof(TX) == 8) {
+        if (a.val_dtype == WPC_F64) GT(double);
     }
 #undef GT
     return WPC_EDTYPE;
@@ -644,8 +681,8 @@ __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
 }
 
 template <class TV, class TX, class TS, int NV>
-int stream_launch(int nb, const int* blocks, const int* slice_len, const int* rows, bool col16,
-                  int n, int lg, const int* off,
+int stream_launch(int nb, const int* blocks, const int* slice_len, const int* slice_base,
+                  const int* rows, bool col16, int n, int lg, const int* off,
                   const int* col, const void* val, const void* s, const void* d, const void* u,
                   const void* x, const void* y, void* z, double alpha, double beta, const void* w,
                   double* partials, double* out, hipStream_t st) {
@@ -657,8 +694,8 @@ int stream_launch(int nb, const int* blocks, const int* slice_len, const int* ro
     do {                                                                                        \
         if (lg == -2)                                                                           \
             hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(grid),           \
-                               dim3(kStreamRows), 0, st, n, blocks, slice_len, rows,            \
-                               (const void*)col, (int)col16,                                    \
+                               dim3(kStreamRows), 0, st, n, blocks, slice_len, slice_base,      \
+                               rows, (const void*)col, (int)col16,                              \
                                (const TV*)val, (const TS*)s, (const TX*)d, (const TX*)u,        \
                                (const TX*)x, (const TX*)y, (TX*)z, (TA)alpha, (TA)beta,         \
                                (const TX*)w, partials);                                         \
@@ -697,14 +734,16 @@ int stream_for_nv(const StreamArgs& a) {
 #define ST(TV, TX)                                                                              \
     do {                                                                                        \
         if (a.s32)                                                                              \
-            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.rows,       \
+            return stream_launch<TV, TX, float, NV>(a.nb, a.blocks, a.slice_len, a.slice_base, \
+                                                     a.rows,                                     \
                                                      a.col16, a.n,                               \
                                                      a.lg,                                       \
                                                      a.off, a.col,                               \
                                                      a.val, a.s, a.d, a.u, a.x, a.y, a.z,        \
                                                      a.alpha, a.beta, a.w, a.partials, a.out,    \
                                                      a.st);                                      \
-        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.rows, a.col16,     \
+        return stream_launch<TV, TX, TX, NV>(a.nb, a.blocks, a.slice_len, a.slice_base, a.rows,\
+                                             a.col16,                                            \
                                              a.n, a.lg,                                          \
                                              a.off,                                              \
                                              a.col, a.val,                                       \
@@ -733,8 +772,15 @@ int stream_for_nv(const StreamArgs& a) {
 int WPC_STREAM_CAT(stream_entry_, WPC_STREAM_NV)(const StreamArgs& a) {
     return stream_for_nv<WPC_STREAM_NV>(a);
 }
+#elif defined(WPC_GROUPED_F32)
+int stream_grouped_f32(const GroupedArgs& a) { return grouped_for_nv<0, float>(a); }
 #else
-int stream_grouped_entry(const GroupedArgs& a) { return grouped_for_nv<0>(a); }
+int stream_grouped_f32(const GroupedArgs& a);
+int stream_grouped_entry(const GroupedArgs& a) {
+    if (a.vec_dtype == WPC_F64) return grouped_for_nv<0, double>(a);
+    if (a.vec_dtype == WPC_F32) return stream_grouped_f32(a);
+    return WPC_EDTYPE;
+}
 #endif
 
 }  // namespace wpc

@@ -92,6 +92,22 @@ def ptr(t):
     return None if t is None else t.data_ptr()
 
 
+# Counts reallocations of operator-owned scratch buffers (product partials, the
+# two-pass temporaries).  A captured graph bakes in the addresses of such buffers,
+# so a graph recorded at one generation is replayed only while the generation is
+# unchanged; after a reallocation the old buffer may belong to another tensor.
+_generation = 0
+
+
+def bump_generation() -> None:
+    global _generation
+    _generation += 1
+
+
+def generation() -> int:
+    return _generation
+
+
 def stream():
     return torch.cuda.current_stream().cuda_stream
 

@@ -201,6 +201,7 @@ class FSAI(LinearOperator):
         if t is None or t.numel() < count:
             t = torch.empty(count, dtype=dtype, device=self.device)
             self._tmp[dtype] = t
+            nat.bump_generation()
         return t
 
     # float32/float64 vectors go through wpc_csr_stream: apply_lanes == 1 (the

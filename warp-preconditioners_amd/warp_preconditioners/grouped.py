@@ -97,7 +97,13 @@ def _sliced(plan):
 
 
 def _col16(plan):
-    return int(plan._sell[6])
+    """The task's slice_base word (the per-slice column bases)."""
+    return _p(plan._sell[7])
+
+
+def _nv_word(plan, nv):
+    """The task's vector-count word: nv, plus 256 for 16-bit columns."""
+    return nv | (256 if plan._sell[6] else 0)
 
 
 class GroupedCG:
@@ -137,16 +143,19 @@ class GroupedCG:
             ca, cg, cgt = _col16(plan_a), _col16(pg), _col16(pgt)
             nv = w.nvec
             if self.kind == "factored":
-                k1 = a_sl + [_p(A.inverse_mass), 0, 0, p_, _p(sc.tmp_a), 0, 0, n, b_sell, ca, nv]
+                k1 = a_sl + [_p(A.inverse_mass), 0, 0, p_, _p(sc.tmp_a), 0, 0, n, b_sell, ca,
+                             _nv_word(plan_a, nv)]
                 k2 = a_sl + [0, _p(A._wm) if self.weight else 0, p_ if self.weight else 0,
-                             _p(sc.tmp_a), ap_, p_, _p(sc.part_ap), n, b_sell, ca, nv]
+                             _p(sc.tmp_a), ap_, p_, _p(sc.part_ap), n, b_sell, ca,
+                             _nv_word(plan_a, nv)]
                 ops.append((k1, k2))
             else:
-                ops.append((a_sl + [0, 0, 0, p_, ap_, p_, _p(sc.part_ap), n, b_sell, ca, nv],))
+                ops.append((a_sl + [0, 0, 0, p_, ap_, p_, _p(sc.part_ap), n, b_sell, ca,
+                                    _nv_word(plan_a, nv)],))
             g = _sliced(pg) + [_p(M.row_scale2) if self.scaled else 0, 0, 0, r_,
-                               _p(sc.tmp_m), 0, 0, n, b_sell, cg, nv]
+                               _p(sc.tmp_m), 0, 0, n, b_sell, cg, _nv_word(pg, nv)]
             gt = _sliced(pgt) + [0, 0, 0, _p(sc.tmp_m), z_, r_, _p(sc.part_rz), n,
-                                 b_sell, cgt, nv]
+                                 b_sell, cgt, _nv_word(pgt, nv)]
             sell.append((g, gt))
             pr = _p(w.parts[1])
             st = _p(w.state)

@@ -134,6 +134,7 @@ class KrylovWork:
         self.code = nat.code(dtype)
         self.graph = None
         self.graph_steps = 0
+        self._graph_key = None
         self._m_parts = None  # the preconditioner's partial count (fused steps)
         self.use_fused = (self.method == 0 and M is not None and bool(getattr(A, "fused", False))
                           and bool(getattr(M, "fused", False)) and hasattr(A, "apply_batch")
@@ -244,13 +245,24 @@ class KrylovWork:
             for _ in range(steps):
                 self.step(x)
             return
-        if self.graph is None or self.graph_steps != steps or self._graph_x != x.data_ptr():
+        # The graph bakes in x and the operators' own scratch (product partials,
+        # two-pass temporaries); it is replayed only while none of them moved.
+        key = (steps, x.data_ptr(), nat.generation())
+        if self.graph is None or self._graph_key != key:
             # warm once eagerly (first launches load code objects), then capture
+            self.graph = None
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 for _ in range(steps):
                     self.step(x)
-            self.graph, self.graph_steps, self._graph_x = g, steps, x.data_ptr()
+            self.graph, self.graph_steps = g, steps
+            # (capture itself may size a buffer: key on the generation after it)
+            self._graph_key = (steps, x.data_ptr(), nat.generation())
+            if self._graph_key != key:
+                self.graph = None  # recorded against a buffer that has since moved
+                for _ in range(steps):
+                    self.step(x)
+                return
         self.graph.replay()
 
     def host_state(self):

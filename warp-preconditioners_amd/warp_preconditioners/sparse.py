@@ -196,20 +196,33 @@ class StreamPlan:
                      nat.ptr(rows), nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols),
                      nat.ptr(smap), s)
             col16 = False
+            # per-slice column bases of the 16-bit copy (zeros with int32 columns)
+            base = torch.zeros(max(ns, 1), dtype=torch.int32, device=dev)
             if self.COL16 and total:
                 c16 = torch.empty(total, dtype=torch.int16, device=dev)
                 over = torch.zeros(1, dtype=torch.int32, device=dev)
-                nat.call("wpc_sell_col16", A.nrow, A.ncol, nat.ptr(A.offsets), nat.ptr(rows),
+                nat.call("wpc_sell_col16", A.nrow, nat.ptr(A.offsets), nat.ptr(rows),
                          nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols), nat.ptr(c16),
-                         nat.ptr(over), s)
+                         nat.ptr(base), nat.ptr(over), s)
                 if int(over.item()) == 0:
                     cols, col16 = c16, True
-            self._sell = (slice_ptr, slice_len, cols, smap, total, rows, col16)
+                else:
+                    base.zero_()  # int32 columns: the descriptor starts at x
+            self._sell = (slice_ptr, slice_len, cols, smap, total, rows, col16, base)
         if self._sell_values is None:
             self._sell_values = torch.empty(max(self._sell[4], 1), dtype=A.values.dtype,
                                             device=A.device)
             self.refresh_values()
         return self._sell[0], self._sell[1], self._sell[2], self._sell_values
+
+    # The sliced product gathers int32 columns through a buffer descriptor whose
+    # byte offsets (column * nvec * element size) must stay below 2^31.
+    DESCRIPTOR_RANGE = 1 << 31
+
+    def _sell_fits(self, x, nvec):
+        if self._sell is not None and self._sell[6]:
+            return True  # 16-bit offsets: the range is per slice
+        return self.A.ncol * nvec * x.element_size() <= self.DESCRIPTOR_RANGE
 
     def refresh_values(self):
         """Re-gather the sliced copy's values after A.values changed in place."""
@@ -233,10 +246,22 @@ class StreamPlan:
             return (self.A.nrow + 255) // 256
         return self.nblocks if mode == 0 else (self.A.nrow * mode + 255) // 256
 
+    def partials_size(self, nvec, mode=None):
+        """Doubles the fused inner-product partials need: one per workgroup and
+        vector of the widest mapping this plan can launch (every tuning candidate,
+        the fixed lane count, and `mode`)."""
+        modes = set(self.CANDIDATES)
+        if self.fixed is not None:
+            modes.add(self.fixed)
+        if mode is not None:
+            modes.add(mode)
+        return max(max(self.workgroups(m) for m in modes), 1) * nvec
+
     def partials(self, nvec, mode=None):
-        need = max(self.workgroups(0), self.workgroups(16), 1) * nvec
+        need = self.partials_size(nvec, mode)
         if self._partials is None or self._partials.numel() < need:
             self._partials = torch.zeros(need, dtype=torch.float64, device=self.A.device)
+            nat.bump_generation()
         return self._partials
 
     def mode(self, nvec, dtype):
@@ -336,10 +361,12 @@ class StreamPlan:
     def _launch(self, mode, x, y, z, alpha, beta, nvec, s=None, d=None, u=None, w=None, out=None,
                 values=None, part=None):
         A = self.A
+        if mode == self.SLICED and values is None and not self._sell_fits(x, nvec):
+            mode = 0  # same sums, no descriptor range limit
         if mode == self.SLICED and values is None:
             sp, sl, sc, sv = self._sliced()
-            nat.call("wpc_sell_stream", A.nrow, nat.ptr(sp), nat.ptr(sl), nat.ptr(self._sell[5]),
-                     nat.ptr(sc), int(self._sell[6]), nat.ptr(sv),
+            nat.call("wpc_sell_stream", A.nrow, nat.ptr(sp), nat.ptr(sl), nat.ptr(self._sell[7]),
+                     nat.ptr(self._sell[5]), nat.ptr(sc), int(self._sell[6]), nat.ptr(sv),
                      nat.code(sv.dtype), nat.ptr(s), nat.code(s.dtype) if s is not None else 0,
                      nat.ptr(d), nat.ptr(u), nat.ptr(x), nat.ptr(y), nat.ptr(z), nat.code(x.dtype),
                      nvec, float(alpha), float(beta), nat.ptr(w), nat.ptr(part), nat.ptr(out),
@@ -361,7 +388,7 @@ class StreamPlan:
         With ``w``: per-workgroup partials of w . z are left in the plan's buffer
         (and folded into ``out`` when given); returns (partials, count)."""
         mode = self.mode(nvec, x.dtype)
-        part = self.partials(nvec) if w is not None else None
+        part = self.partials(nvec, mode) if w is not None else None
         self._launch(mode, x, y, z, alpha, beta, nvec, s, d, u, w, out, values, part)
         return part, self.workgroups(mode)
 

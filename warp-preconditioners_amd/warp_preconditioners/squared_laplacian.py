@@ -79,6 +79,7 @@ class SquaredLaplacianOperator(LinearOperator):
         if t is None or t.numel() < count:
             t = torch.empty(count, dtype=dtype, device=self.device)
             self._tmp[dtype] = t
+            nat.bump_generation()
         return t
 
     @property
