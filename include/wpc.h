@@ -74,6 +74,22 @@ int wpc_fsai_apply(int n, const int* g_offsets, const int* g_columns, const void
                    const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
                    int lanes, void* stream);
 
+/* wpc_fsai_apply with its temporary in a library-owned device array instead of
+ * `tmp`: allocates nothing (no hipMalloc, hipFree or allocating library call).
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under graph capture, serving loops, harnesses that forbid it).
+ * n * nvec * sizeof(vector element) must not exceed WPC_NOALLOC_SCRATCH_BYTES
+ * (else WPC_ESCRATCH, nothing launched).  Each GPU holds one such array until the
+ * process ends, shared by every call on that GPU: calls that overlap on one GPU
+ * race, so the caller serialises them.  Same kernels and results as
+ * wpc_fsai_apply. */
+#define WPC_NOALLOC_SCRATCH_BYTES (32u << 20)
+int wpc_fsai_apply_noalloc(int n, const int* g_offsets, const int* g_columns,
+                           const void* g_values, const int* gt_offsets, const int* gt_columns,
+                           const void* gt_values, int factor_dtype, const float* row_scale2,
+                           const void* x, const void* y, void* z, int vec_dtype, int nvec,
+                           double alpha, double beta, int lanes, void* stream);
+
 /* Row partition for wpc_csr_stream (host function, no device work): rows are cut
  * into blocks of at most 256 rows and 2048 stored entries, a longer row gets a
  * block of its own.  host_blocks receives count+1 row boundaries; returns count
