@@ -389,7 +389,7 @@ def cpu_baseline(wl, s, iters_needed, sample_iters=1400):
                                         f"({one:.1f}s, {per_one * 1e3:.2f} ms/iter)"))
 
 
-def block_bench(name):
+def block_bench(name, only=None):
     """Block-system comparisons, one RHS, relative residual 1e-8 (true residual
     recomputed in compensated float64), error against a float64 reference solve:
 
@@ -427,7 +427,10 @@ def block_bench(name):
                          torch.tensor(val, dtype=dtype, device=dev))
 
     A64, A32 = bsr(torch.float64), bsr(torch.float32)
-    op64, op32 = wp.SparseOperator(A64, row_lanes=1), wp.SparseOperator(A32, row_lanes=1)
+    # the BSR matrices themselves are the operators (Warp's bsr_mv in the reference's
+    # block benchmarks, block_challenge.py:82,93): the 3x3 sliced block product or
+    # the scalarised CSR copy, whichever the operator's timing picks
+    op64, op32 = linear.aslinearoperator(A64), linear.aslinearoperator(A32)
     b = torch.tensor(rhs, device=dev)
     n = b.numel()
 
@@ -480,6 +483,8 @@ def block_bench(name):
                    ("fsai8-cg-mixed-bfloat16", "fsai8", "cg", "mixed", torch.bfloat16),
                    ("fsai8-cg-f32", "fsai8", "cg", "f32", None),
                    ("ilu0-cg-f32", "ilu0", "cg", "f32", None)]
+    if only:
+        methods = [m for m in methods if m[0] in only]
     records = []
     for label, prec, method, mode, fdt in methods:
         log(f"running {label}")
@@ -532,8 +537,22 @@ def block_bench(name):
                    error_vs_f64=error(x), converged=bool(out["converged"]))
         log(json.dumps(rec))
         records.append(rec)
+    # the operator's product, both mappings timed by its chooser (graph replays):
+    # algorithmic bytes of the block product nnzb (9 vb + 4) + 4 (nb + 1) + 2 * 3 nb * xb
+    # (values and 4-byte block columns streamed, x gathered once, z written)
+    operator = {}
+    for op in (op64, op32):
+        vb = op.A.values.element_size()
+        for (k, dt), t in op.block_timings.items():
+            xb = torch.empty(0, dtype=dt).element_size()
+            nbytes = len(col) * (9 * vb + 4) + 4 * (nb + 1) + 2 * 3 * nb * k * xb
+            key = f"values {str(op.A.scalar_type)[6:]}, {k} x {str(dt)[6:]}"
+            operator[key] = dict(
+                mode=op.block_mode(k, dt), bsr_us=t["bsr"] * 1e3, csr_us=t["csr"] * 1e3,
+                bytes=nbytes, bsr_GBs=nbytes / (t["bsr"] * 1e-3) / 1e9,
+                bsr_frac=nbytes / (t["bsr"] * 1e-3) / 8e12)
     print(json.dumps({"workload": name, "n": n, "blocks": int(len(col)), "reference": ref,
-                      "methods": records}), flush=True)
+                      "operator": operator, "methods": records}), flush=True)
 
 
 def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8), reps=50):
@@ -716,6 +735,8 @@ def main():
                     help="biharmonic64: the factored-biharmonic 64^3 solve (default); serving: "
                          "the request stream through the continuous-batching engine; kron48 / "
                          "elasticity: the block-coupled and elasticity solver comparisons")
+    ap.add_argument("--methods", default=None,
+                    help="kron48 / elasticity: comma-separated method labels to run (default all)")
     ap.add_argument("--requests", type=int, default=512, help="serving: stream length")
     ap.add_argument("--max-side", type=int, default=96, help="serving: largest mesh side")
     ap.add_argument("--group", action="store_true",
@@ -734,7 +755,7 @@ def main():
         serving_bench(args)
         return
     if args.workload in ("kron48", "elasticity"):
-        block_bench(args.workload)
+        block_bench(args.workload, args.methods.split(",") if args.methods else None)
         return
     if args.workload == "apply":
         apply_bench()

@@ -163,6 +163,29 @@ int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int
                     const void* y, void* z, int vec_dtype, int nvec, double alpha, double beta,
                     const void* w, double* partials, double* out, void* stream);
 
+/* 3x3 BSR product on a sliced block copy: z = alpha * A x + beta * y, vectors
+ * interleaved [scalar row][nvec] (nvec 1..4; scalar row 3i + c of block row i).
+ * The block pattern's sliced copy comes from wpc_sell_lengths / wpc_sell_fill /
+ * wpc_sell_col16 run on the block offsets and columns (nb block rows; int32 block
+ * columns or 16-bit offsets from slice_base); wpc_bsr3_sell_values stores each
+ * slot's nine values component-major inside the slice (value q of the block in
+ * slot p at ((p / 64) * 9 + q) * 64 + p % 64; `slots` = 64 * slice_ptr[nslices],
+ * sell_map from wpc_sell_fill; rerun after the BSR values change).  One thread
+ * per block row sums its blocks in storage order, each block's product
+ * t_r = (v_r0 x_0 + v_r1 x_1) + v_r2 x_2 added to the row's sum, in the vector
+ * type (values F64 or F32 with F64 vectors, F32 with F32).  With w: per-workgroup
+ * partials of w . z (ceil(nb / 256) per vector), folded into out[nvec] when out
+ * is given.  alpha == 0 never reads x, beta == 0 never reads y; x must not alias
+ * z.  Replaces Warp's bsr_mv on 3x3 blocks (sparse_operator.py:104, the CG
+ * operator of benchmarks/block_challenge.py:82,93 and benchmarks/elasticity.py). */
+int wpc_bsr3_sell_values(long slots, const int* sell_map, const void* values, int elem_bytes,
+                         void* sell_values, void* stream);
+int wpc_bsr3_sell_stream(int nb, const int* slice_ptr, const int* slice_len,
+                         const int* slice_base, const void* sell_columns, int col16,
+                         const void* sell_values, int val_dtype, const void* x, const void* y,
+                         void* z, int vec_dtype, int nvec, double alpha, double beta,
+                         const void* w, double* partials, double* out, void* stream);
+
 /* ------------------------------------------------------------ FSAI setup */
 
 /* scale[i] = 1/sqrt(A_ii); non-positive or non-finite diagonals set 0 and add 1

@@ -418,14 +418,17 @@ def block_ilu0_apply(A: CSR, factors, inv, x, y=None, alpha=1.0, beta=0.0, solve
     return out.astype(A.dtype)
 
 
-def bsr_spmv(A: CSR, x, y=None, alpha=1.0, beta=0.0):
+def bsr_spmv(A: CSR, x, y=None, alpha=1.0, beta=0.0, vec_dtype=None):
+    """Warp's bsr_mv on b x b blocks (sparse_operator.py:104): z = alpha A x + beta y,
+    accumulating in the vector type, each block's product added to the row sum."""
     b = A.block
-    x = np.ascontiguousarray(x, A.dtype).reshape(-1)
-    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, A.dtype).reshape(-1)
-    z = np.zeros_like(x)
+    vec_dtype = np.dtype(vec_dtype or A.dtype)
+    x = np.ascontiguousarray(x, vec_dtype).reshape(-1)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, vec_dtype).reshape(-1)
+    z = np.zeros(A.nrow * b, vec_dtype)
     vals = np.ascontiguousarray(A.val.reshape(-1))
-    lib().orc_bsr_spmv(code(A.dtype), A.nrow, b, _p(A.off), _p(A.col), _p(vals), _p(x), _p(y),
-                       _p(z), ctypes.c_double(alpha), ctypes.c_double(beta))
+    lib().orc_bsr_spmv(code(A.dtype), code(vec_dtype), A.nrow, b, _p(A.off), _p(A.col), _p(vals),
+                       _p(x), _p(y), _p(z), ctypes.c_double(alpha), ctypes.c_double(beta))
     return z
 
 

@@ -1101,6 +1101,38 @@ static void ilu_triangular(int nb, int b, const int* off, const int* cols, const
     }
 }
 
+// Warp's bsr_mv on b x b blocks (sparse_operator.py:104, biharmonic.py:103,106):
+// z = alpha A x + beta y, accumulating in the vector type V.  Per block row and
+// row r of the block, each stored block (columns ascending) contributes its block
+// product t_r = sum_c v_rc x_c (c ascending) to the row's sum: acc_r += t_r (the
+// block product, then the accumulation, as bsr_mv adds A_ij x_j).  alpha == 0
+// never reads A or x, beta == 0 never reads y.
+template <class V>
+static void bsr_spmv(int mat_dt, int nb, int b, const int* off, const int* col, const void* val,
+                     const void* x, const void* y, void* z, double alpha_d, double beta_d) {
+    const V alpha = V(alpha_d), beta = V(beta_d);
+    const int bb = b * b;
+    std::vector<V> out((size_t)nb * b);
+    for (int i = 0; i < nb; ++i)
+        for (int r = 0; r < b; ++r) {
+            V acc = V(0);
+            if (alpha != V(0)) {
+                for (int e = off[i]; e < off[i + 1]; ++e) {
+                    V t = V(load_any(mat_dt, val, (long)e * bb + r * b)) *
+                          Store<V>::load(x, (long)col[e] * b);
+                    for (int c = 1; c < b; ++c)
+                        t += V(load_any(mat_dt, val, (long)e * bb + r * b + c)) *
+                             Store<V>::load(x, (long)col[e] * b + c);
+                    acc += t;
+                }
+                acc = alpha * acc;
+            }
+            if (beta != V(0)) acc += beta * Store<V>::load(y, (long)i * b + r);
+            out[(size_t)i * b + r] = acc;
+        }
+    for (long k = 0; k < (long)nb * b; ++k) Store<V>::store(z, k, out[k]);
+}
+
 extern "C" {
 
 int orc_ilu_initialize(int dt, long nnzb, int b, const int* rows, const int* cols,
@@ -1160,24 +1192,10 @@ int orc_block_diagonal_apply(int dt, int nb, int b, const void* inverse, const v
 }
 
 // Warp bsr_mv (sparse_operator.py:104) for b x b blocks: z = alpha A x + beta y.
-int orc_bsr_spmv(int dt, int nb, int b, const int* off, const int* col, const void* val,
-                 const void* x, const void* y, void* z, double alpha, double beta) {
-    int bb = b * b;
-    std::vector<double> out((size_t)nb * b);
-    for (int i = 0; i < nb; ++i)
-        for (int r = 0; r < b; ++r) {
-            double s = 0.0;
-            if (alpha != 0.0) {
-                for (int e = off[i]; e < off[i + 1]; ++e)
-                    for (int c = 0; c < b; ++c)
-                        s += load_any(dt, val, (long)e * bb + r * b + c) *
-                             load_any(dt, x, (long)col[e] * b + c);
-                s *= alpha;
-            }
-            if (beta != 0.0) s += beta * load_any(dt, y, (long)i * b + r);
-            out[(size_t)i * b + r] = s;
-        }
-    for (long k = 0; k < (long)nb * b; ++k) store_any(dt, z, k, out[k]);
+int orc_bsr_spmv(int mat_dt, int vec_dt, int nb, int b, const int* off, const int* col,
+                 const void* val, const void* x, const void* y, void* z, double alpha,
+                 double beta) {
+    DISPATCH_T(vec_dt, (bsr_spmv<T>(mat_dt, nb, b, off, col, val, x, y, z, alpha, beta)));
     return 0;
 }
 

@@ -321,3 +321,37 @@ def test_block_ilu_gmres_on_indefinite_blocks(wp):
     linear.gmres(A, b, x, M=wp.BlockILU0(A, factor_sweeps=20, solve_sweeps=n), tol=1e-11,
                  maxiter=60, restart=15)
     np.testing.assert_allclose(host(x), truth, rtol=1e-8, atol=1e-8)
+
+
+@pytest.mark.parametrize("sweeps", [0, 1, 2, 3])
+@pytest.mark.parametrize("relaxation", [1.0, 0.7])
+def test_block_ilu_few_sweeps_match_oracle(wp, oracle, sweeps, relaxation):
+    # Away from the fixed point (0-3 sweeps, as configuration 4 runs 3; relaxation
+    # 1 and 0.7): the device factors, U-diagonal inverses and apply equal the
+    # oracle's (itself pinned by a numpy restatement of one synchronous sweep).
+    rng = np.random.default_rng(90 + sweeps)
+    n, size = 60, 3
+    blocks = {}
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                blocks[i, j] = rng.normal(size=(size, size)) * 0.2 + np.diag([2.0, 2.5, 3.0])
+            elif abs(i - j) <= 2 or (i * 7 + j * 3) % 11 == 0:
+                blocks[i, j] = rng.normal(size=(size, size)) * 0.35
+    a = np.zeros((size * n, size * n))
+    for (i, j), value in blocks.items():
+        a[size * i:size * (i + 1), size * j:size * (j + 1)] = value
+    A = upload(a, block=size)
+    pre = wp.BlockILU0(A, factor_sweeps=sweeps, solve_sweeps=4, relaxation=relaxation)
+    Ao = oracle_csr(oracle, a, block=size)
+    factors, inv = oracle.block_ilu0(Ao, factor_sweeps=sweeps, relaxation=relaxation)
+    np.testing.assert_allclose(host(pre.factors), factors, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(host(pre.inverse_diagonal), inv, rtol=1e-12, atol=1e-13)
+    other, _ = oracle.block_ilu0(Ao, factor_sweeps=sweeps + 1, relaxation=relaxation)
+    assert np.abs(other - factors).max() > 1e-6  # one sweep more would not pass
+    x = rng.normal(size=(n, size))
+    y = rng.normal(size=(n, size))
+    z = torch.tensor(y, device=dev())
+    pre.matvec(torch.tensor(x, device=dev()), z, z, 0.9, -0.4)
+    ref = oracle.block_ilu0_apply(Ao, factors, inv, x, y, 0.9, -0.4, solve_sweeps=4)
+    np.testing.assert_allclose(host(z), ref, rtol=1e-12, atol=1e-12)

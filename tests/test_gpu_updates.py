@@ -146,3 +146,44 @@ def test_scaled_factor_refit(wp, factor):
     g = dense(full.G)
     rel = 4e-3 if factor == torch.float16 else 2e-2
     assert np.all(np.abs(pre.dense_factor() - g) <= rel * np.abs(g).max(axis=1, keepdims=True))
+
+
+def _probe(n, seed, dtype):
+    from warp_preconditioners import _native as nat
+
+    x = torch.empty(n, dtype=dtype, device=dev())
+    nat.call("wpc_rademacher", n, seed, nat.ptr(x), nat.code(dtype), nat.stream())
+    return host(x).astype(np.float64)
+
+
+@pytest.mark.parametrize("factor_dtype", [None, torch.bfloat16])
+def test_quality_of_inexact_factor_against_dense_estimator(wp, factor_dtype):
+    # quality() on an inexact factor (width 4 on a 300-row SPD matrix): equal to the
+    # dense evaluation of the same estimator, sqrt(sum_p |(G A G^T - I) x_p|^2 /
+    # (probes n)) with the device's own probe vectors x_p (seed, seed + 1, ...), so
+    # a wrong normalisation, sign or transpose fails; the probes are +-1 and differ
+    # per seed; and over many probes the estimate approaches the exact
+    # ||G A G^T - I||_F / sqrt(n) (its expectation).
+    rng = np.random.default_rng(77)
+    n = 300
+    a = random_spd(rng, n, 0.03, (0.1, 10.0))
+    A = upload(a)
+    P = wp.FSAI(A, max_row_size=4, factor_dtype=factor_dtype)
+    g = P.dense_factor()
+    m = g @ a @ g.T - np.eye(n)
+    exact = np.linalg.norm(m) / np.sqrt(n)
+    assert exact > 1e-2  # an inexact factor
+    probes, seed = 5, 23
+    xs = [_probe(n, seed + p, torch.float64) for p in range(probes)]
+    for x in xs:
+        assert set(np.unique(x)) <= {-1.0, 1.0} and abs(x.sum()) < n
+    assert not np.array_equal(xs[0], xs[1])
+    dense_est = np.sqrt(sum(float(np.sum((m @ x) ** 2)) for x in xs) / (probes * n))
+    got = P.quality(probes=probes, seed=seed)
+    np.testing.assert_allclose(got, dense_est, rtol=1e-6 if factor_dtype else 1e-10)
+    # a transposed product (G^T A G) or a missing -I gives a different value
+    wrong_t = g.T @ a @ g - np.eye(n)
+    assert abs(np.sqrt(sum(float(np.sum((wrong_t @ x) ** 2)) for x in xs) / (probes * n))
+               - got) > 1e-3 * got
+    many = P.quality(probes=400, seed=1)
+    assert abs(many - exact) < 0.1 * exact

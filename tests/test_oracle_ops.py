@@ -282,3 +282,133 @@ def test_cg_and_cr_iterates_are_krylov_optimal(oracle, k):
     for method, best in (("cg", best_cg), ("cr", best_cr)):
         x, _, _ = oracle.krylov(method, b, A=A, P=P, tol=0.0, maxiter=k, check_every=1)
         np.testing.assert_allclose(x, best, rtol=1e-10, atol=1e-12 * np.abs(best).max())
+
+
+def _ilu_blocks(n, size, seed, coupling):
+    """Block matrix {(i, j): block} with a diagonal-dominant-ish diagonal, a band and
+    scattered couplings: far from its ILU(0) fixed point after a few sweeps."""
+    rng = np.random.default_rng(seed)
+    blocks = {}
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                d = rng.normal(size=(size, size)) * 0.2
+                blocks[i, j] = d + np.diag(np.linspace(2.0, 3.0, size))
+            elif abs(i - j) <= 2 or (i * 7 + j * 3) % 11 == 0:
+                blocks[i, j] = rng.normal(size=(size, size)) * coupling
+    return blocks
+
+
+def _dense(blocks, n, size):
+    a = np.zeros((size * n, size * n))
+    for (i, j), value in blocks.items():
+        a[size * i:size * (i + 1), size * j:size * (j + 1)] = value
+    return a
+
+
+def ilu_sweeps_numpy(blocks, n, sweeps, relaxation, in_place=False, swap_relaxation=False):
+    """Synchronous block ILU(0), restated from the reference's kernels with dense
+    numpy blocks (block_ilu.py:106-118 initialize, :120-150 sweep; the U-diagonal
+    inverses refreshed after every sweep, :391): every stored block (i, j) is
+    recomputed from the previous sweep's blocks,
+        v = A_ij - sum_{k < min(i, j), (i,k), (k,j) stored} F_ik F_kj,
+        v = v inv(F_jj) if j < i,   F_ij <- (1 - w) F_ij + w v.
+    `in_place` and `swap_relaxation` plant the two mistakes this restatement must
+    tell apart (a Gauss-Seidel sweep reading this sweep's blocks; w and 1 - w
+    exchanged)."""
+    keys = sorted(blocks)
+    inv = {i: np.linalg.inv(blocks[i, i]) for i in range(n)}
+    F = {(i, j): (blocks[i, j] @ inv[j] if j < i else blocks[i, j].copy()) for i, j in keys}
+    for _ in range(sweeps):
+        old = F if in_place else {key: value.copy() for key, value in F.items()}
+        new = F if in_place else {}
+        for i, j in keys:
+            v = blocks[i, j].copy()
+            for k in sorted(kk for (ii, kk) in keys if ii == i):
+                if k >= min(i, j):
+                    break
+                if (k, j) in old:
+                    v -= old[i, k] @ old[k, j]
+            if j < i:
+                v = v @ inv[j]
+            w = 1.0 - relaxation if swap_relaxation else relaxation
+            new[i, j] = (1.0 - w) * old[i, j] + w * v
+        F = new
+        inv = {i: np.linalg.inv(F[i, i]) for i in range(n)}
+    return F
+
+
+@pytest.mark.parametrize("sweeps", [0, 1, 2, 3])
+@pytest.mark.parametrize("relaxation", [1.0, 0.7])
+@pytest.mark.parametrize("size", [2, 3])
+def test_ilu_sweeps_match_numpy_restatement(oracle, sweeps, relaxation, size):
+    # The oracle's factors after 0-3 synchronous sweeps (the configuration-4 beam
+    # runs 3) with relaxation 1 and 0.7 equal the numpy restatement; far from the
+    # fixed point, so one sweep more or less, a Gauss-Seidel sweep and a swapped
+    # relaxation each differ from it by orders of magnitude more than the tolerance.
+    n = 12
+    blocks = _ilu_blocks(n, size, 90 + size, 0.35)
+    A = csr(oracle, _dense(blocks, n, size), block=size)
+    factors, inv = oracle.block_ilu0(A, factor_sweeps=sweeps, relaxation=relaxation)
+    want = ilu_sweeps_numpy(blocks, n, sweeps, relaxation)
+
+    def gap(F):
+        return max(np.abs(factors[e] - F[i, A.col[e]]).max()
+                   for i in range(n) for e in range(A.off[i], A.off[i + 1]))
+
+    assert gap(want) < 1e-13
+    for j in range(n):  # the returned inverses are those of the last U diagonal
+        np.testing.assert_allclose(inv[j], np.linalg.inv(want[j, j]), rtol=1e-12, atol=1e-13)
+    assert gap(ilu_sweeps_numpy(blocks, n, sweeps + 1, relaxation)) > 1e-6
+    if sweeps >= 2:
+        assert gap(ilu_sweeps_numpy(blocks, n, sweeps, relaxation, in_place=True)) > 1e-6
+    if sweeps >= 1 and relaxation != 1.0:
+        assert gap(ilu_sweeps_numpy(blocks, n, sweeps, relaxation, swap_relaxation=True)) > 1e-6
+
+
+@pytest.mark.parametrize("size", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_bsr_spmv_against_scipy(oracle, size, dtype):
+    # The oracle's block product z = alpha A x + beta y (Warp's bsr_mv contract,
+    # sparse_operator.py:104) against scipy's bsr_matrix product: exactly on
+    # small-integer blocks and vectors with power-of-two alpha, beta (every sum is
+    # exact in any order, so indexing mistakes -- a transposed block, a wrong
+    # column or row -- cannot hide in rounding), and within rounding on random
+    # data; with empty block rows; alpha = 0 never reads x, beta = 0 never reads y.
+    rng = np.random.default_rng(100 + size)
+    nb = 37
+    pattern = ss.random(nb, nb, density=0.15, random_state=rng, format="lil")
+    pattern[5, :] = 0
+    pattern[20, :] = 0
+    pattern = pattern.tocsr()
+    pattern.data[:] = 1
+    nnzb = pattern.nnz
+    for kind in ("integer", "random"):
+        if kind == "integer":
+            data = rng.integers(-4, 5, size=(nnzb, size, size)).astype(float)
+            x = rng.integers(-3, 4, size=nb * size).astype(float)
+            y = rng.integers(-3, 4, size=nb * size).astype(float)
+            alpha, beta = 2.0, -0.5
+        else:
+            data = rng.normal(size=(nnzb, size, size))
+            x, y = rng.normal(size=(2, nb * size))
+            alpha, beta = 0.7, -0.3
+        data, x, y = data.astype(dtype), x.astype(dtype), y.astype(dtype)
+        m = ss.bsr_matrix((data, pattern.indices, pattern.indptr), shape=(nb * size, nb * size))
+        A = oracle.CSR(pattern.indptr.astype(np.int32), pattern.indices.astype(np.int32),
+                       np.ascontiguousarray(data), nb, nb, size)
+        want = alpha * (m.astype(np.float64) @ x.astype(np.float64)) + beta * y.astype(np.float64)
+        got = oracle.bsr_spmv(A, x, y, alpha, beta)
+        assert got.dtype == dtype
+        if kind == "integer":
+            np.testing.assert_array_equal(got, want.astype(dtype))
+        else:
+            tol = 1e-5 if dtype == np.float32 else 1e-13
+            np.testing.assert_allclose(got, want, rtol=tol, atol=tol)
+        by = (dtype(beta) * y).reshape(nb, size)  # computed in the vector type
+        assert (got.reshape(nb, size)[[5, 20]] == by[[5, 20]]).all()
+    xn = np.full(nb * size, np.nan, dtype)
+    np.testing.assert_array_equal(oracle.bsr_spmv(A, xn, y, 0.0, 2.0), (2.0 * y).astype(dtype))
+    yn = np.full(nb * size, np.nan, dtype)
+    np.testing.assert_allclose(oracle.bsr_spmv(A, x, yn, 1.0, 0.0), m @ x,
+                               rtol=1e-5 if dtype == np.float32 else 1e-13, atol=1e-5)

@@ -346,13 +346,6 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 // slice_base[s] (descriptor over 0xFFFF columns from there, padding 0xFFFF).  A
 // padding offset lies outside the descriptor's range, so its load returns 0 with
 // no memory access and padding adds exactly +0, whatever x holds.
-template <class TX> __device__ __forceinline__ TX buffer_ld(__amdgpu_buffer_rsrc_t r, unsigned off);
-template <> __device__ __forceinline__ double buffer_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-template <> __device__ __forceinline__ float buffer_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
 
 // Byte offset of an entry's x row inside the descriptor (row stride `stride`).
 template <class TI> __device__ __forceinline__ unsigned sell_off(const TI* col, long p, unsigned stride);
@@ -364,11 +357,6 @@ template <> __device__ __forceinline__ unsigned sell_off<unsigned short>(const u
     return (unsigned)col[p] * stride;
 }
 
-// Word 3 of a gfx9-family buffer descriptor (32-bit data format, no swizzle).
-constexpr int kBufferWord3 = 0x00020000;
-// Descriptor range of the int32 columns: a real column's offset is below 2^31
-// (the host keeps ncol * nvec * sizeof(TX) <= 2^31), padding's (-1) is not.
-constexpr unsigned kRange32 = 0x80000000u;
 
 // NV > 0: NV interleaved vectors; NV == 0: nv_rt (<= 8) vectors, the count a
 // batched launch reads per system.  Both perform the same operations per vector.

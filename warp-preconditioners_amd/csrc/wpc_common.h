@@ -117,6 +117,22 @@ __device__ __forceinline__ float shfl_xor(float v, int m, int w) { return __shfl
 __device__ __forceinline__ double shfl_xor(double v, int m, int w) { return __shfl_xor(v, m, w); }
 
 // ------------------------------------------------------------------ launch plumbing
+// Gathers through a buffer descriptor (sliced products): 32-bit byte offsets from
+// the descriptor's base; an offset at or past its range returns 0 and touches no
+// memory, which is how padding entries contribute nothing.
+template <class TX> __device__ __forceinline__ TX buffer_ld(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ double buffer_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ float buffer_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+// Word 3 of a gfx9-family buffer descriptor (32-bit data format, no swizzle).
+constexpr int kBufferWord3 = 0x00020000;
+// Descriptor range of int32 columns: a real column's offset is below 2^31 (the
+// host keeps ncol * row stride <= 2^31), a padding column's (-1) is not.
+constexpr unsigned kRange32 = 0x80000000u;
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline int launch_status() {

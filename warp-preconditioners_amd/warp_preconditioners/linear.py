@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .sparse import BlockType, BsrMatrix, StreamPlan, _product
+from .sparse import BlockType, BsrMatrix, StreamPlan, _product, time_graph
 
 
 class LinearOperator:
@@ -41,18 +41,37 @@ class LinearOperator:
 
 
 class _MatrixOperator(LinearOperator):
-    """BsrMatrix as an operator: Warp's bsr_mv with aliasing handled."""
+    """BsrMatrix as an operator: Warp's bsr_mv with aliasing handled.
+
+    Scalar CSR runs through a StreamPlan.  3x3 blocks (float32/float64 values) run
+    the sliced block product (BsrPlan) or the scalarised CSR copy through a
+    StreamPlan, whichever is faster for the (vector count, vector dtype): the
+    choice is timed once outside graph capture (the block product is kept unless
+    the scalar one is more than BLOCK_MARGIN faster) and shared by every operator
+    over the same index buffers, as the reference's elasticity benchmark picks
+    between bsr_mv and the cooperative CSR operator (benchmarks/elasticity.py:
+    155-171).  Other blocks use the generic block kernel."""
+
+    BLOCK_MARGIN = 0.03
+    _block_chosen = {}
 
     def __init__(self, A: BsrMatrix, lanes=1):
         self.A = A
         self.lanes = lanes
         self._alias = None
         self._plan = None
+        self._bsr = None
+        self._scalar_plan = None
+        self.block_timings = {}
         super().__init__(A.shape, A.dtype, A.device, self._apply)
 
     @property
+    def block3(self):
+        return self.A.block_shape == (3, 3) and self.A.scalar_type in (torch.float32, torch.float64)
+
+    @property
     def fused(self):
-        return self.A.block_shape == (1, 1)
+        return self.A.block_shape == (1, 1) or self.block3
 
     def _apply(self, x, y, z, alpha, beta):
         xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
@@ -61,23 +80,81 @@ class _MatrixOperator(LinearOperator):
                 self._alias = torch.empty_like(xf)
             self._alias.copy_(xf)
             xf = self._alias
-        if self.A.block_shape == (1, 1) and xf.dtype in (torch.float32, torch.float64):
+        if xf.dtype in (torch.float32, torch.float64) and (
+                self.A.block_shape == (1, 1) or (self.block3 and self._block_vectors(xf, 1))):
             self.apply_batch(xf, yf, zf, alpha, beta, 1)
         else:
             _product(self.A, xf, yf, zf, alpha, beta, 8)
 
+    # ---------------------------------------------------------------- 3x3 blocks
+    def _block_vectors(self, x, nvec):
+        """Vectors the block paths take: float64 (any value type) or float32 with
+        float32 values, up to 8 (4 for the block product) interleaved."""
+        return nvec <= 8 and (x.dtype == torch.float64 or
+                              (x.dtype == torch.float32 and self.A.scalar_type == torch.float32))
+
+    def _bsr_plan(self):
+        if self._bsr is None:
+            from .sparse import BsrPlan
+
+            self._bsr = BsrPlan(self.A)
+        return self._bsr
+
+    def _scalar(self):
+        if self._scalar_plan is None:
+            from .sparse import bsr_copy
+
+            self._scalar_plan = StreamPlan(bsr_copy(self.A, block_shape=(1, 1)),
+                                           None if self.lanes == 1 else self.lanes)
+        return self._scalar_plan
+
+    def block_mode(self, nvec, dtype):
+        """'bsr' (sliced block product) or 'csr' (scalarised copy) for this count."""
+        if nvec > 4:
+            return "csr"
+        A = self.A
+        key = (A.offsets.data_ptr(), A.columns.data_ptr(), A.nrow, A.nnz, A.scalar_type, nvec, dtype)
+        mode = _MatrixOperator._block_chosen.get(key)
+        if mode is not None:
+            return mode
+        if torch.cuda.is_current_stream_capturing():
+            return "bsr"
+        n = A.shape[0]
+        x = torch.ones(n * nvec, dtype=dtype, device=A.device)
+        z = torch.empty_like(x)
+        bsr = self._bsr_plan()
+        if not bsr.fits(x, nvec):
+            mode = "csr"
+        else:
+            scalar = self._scalar()
+            t_bsr = time_graph(lambda: bsr.run(x, None, z, 1.0, 0.0, nvec))
+            t_csr = time_graph(lambda: scalar.run(x, None, z, 1.0, 0.0, nvec))
+            self.block_timings[(nvec, dtype)] = dict(bsr=t_bsr, csr=t_csr)
+            mode = "bsr" if t_bsr <= t_csr * (1 + self.BLOCK_MARGIN) else "csr"
+            if mode == "bsr":
+                self._scalar_plan = None  # the scalar copy is no longer needed
+        _MatrixOperator._block_chosen[key] = mode
+        return mode
+
     def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
-        if self.A.block_shape != (1, 1):
-            raise NotImplementedError("batched products need a scalar matrix")
         xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
         if xf.data_ptr() == zf.data_ptr():
             xf = xf.clone()
         A = self.A
+        w = None if dot_w is None else dot_w.reshape(-1)
+        if self.block3:
+            if not self._block_vectors(xf, nvec):
+                raise NotImplementedError("block products need float64 vectors (or float32 "
+                                          "vectors with float32 values), at most 8")
+            if self.block_mode(nvec, xf.dtype) == "bsr":
+                return self._bsr_plan().run(xf, yf, zf, alpha, beta, nvec, w=w, out=dot_out)
+            return self._scalar().run(xf, yf, zf, alpha, beta, nvec, w=w, out=dot_out)
+        if A.block_shape != (1, 1):
+            raise NotImplementedError("batched products need a scalar or 3x3 block matrix")
         if xf.dtype in (torch.float32, torch.float64):
             if self._plan is None:
                 self._plan = StreamPlan(A, None if self.lanes == 1 else self.lanes)
-            return self._plan.run(xf, yf, zf, alpha, beta, nvec,
-                                  w=None if dot_w is None else dot_w.reshape(-1), out=dot_out)
+            return self._plan.run(xf, yf, zf, alpha, beta, nvec, w=w, out=dot_out)
         nat.call("wpc_csr_spmv", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns), nat.ptr(A.values),
                  nat.code(A.scalar_type), None, nat.ptr(xf), nat.ptr(yf), nat.ptr(zf),
                  nat.code(xf.dtype), nvec, float(alpha), float(beta), self.lanes, nat.stream())

@@ -10,7 +10,8 @@ bounds the relative mass-weighted field error (exactly, up to the factor
 recomputes b - A x with compensated float64 accumulation (wpc_dd_residual), so
 the bound is not limited by the cancellation noise of a plain float64 product.
 Both operator kinds of the reference's factored/assembled paths are covered:
-``SparseOperator`` on an assembled float64 matrix (r = b - A x) and the
+``SparseOperator`` or a BSR matrix operator on an assembled float64 matrix
+(r = b - A x) and the
 identity-free ``SquaredLaplacianOperator`` (r = b - w M x - K diag(1/m) K x,
 squared_laplacian.py:437-474).
 """
@@ -21,6 +22,8 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .linear import _MatrixOperator
+from .sparse import bsr_copy
 from .sparse_operator import SparseOperator
 from .squared_laplacian import SquaredLaplacianOperator
 
@@ -40,8 +43,16 @@ class TrueResidual:
                 raise ValueError("TrueResidual needs a float64 matrix")
             self.matrix, self.inverse, self.alpha = op.scalar, None, 0.0
             self.factored = False
+        elif isinstance(op, _MatrixOperator):
+            # a BSR matrix operator: the residual runs on its scalar CSR form
+            if op.A.scalar_type != torch.float64:
+                raise ValueError("TrueResidual needs a float64 matrix")
+            self.matrix = op.A if op.A.block_shape == (1, 1) else bsr_copy(op.A, block_shape=(1, 1))
+            self.inverse, self.alpha = None, 0.0
+            self.factored = False
         else:
-            raise TypeError("TrueResidual supports SparseOperator and SquaredLaplacianOperator")
+            raise TypeError("TrueResidual supports SparseOperator, BSR matrix operators and "
+                            "SquaredLaplacianOperator")
         if not 1 <= nvec <= 8:
             raise ValueError("nvec must be in [1, 8]")
         n = self.matrix.nrow

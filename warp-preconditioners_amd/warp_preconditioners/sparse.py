@@ -393,6 +393,137 @@ class StreamPlan:
         return part, self.workgroups(mode)
 
 
+def time_graph(launch, reps=10, rounds=3):
+    """Best time (ms) of one `launch()` over `rounds` rounds of `reps` launches
+    replayed from a HIP graph captured on this thread's private stream (direct
+    launches are timed if capture fails); host launch cost does not enter."""
+    work = torch.cuda.Stream()
+    work.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(work):
+        launch()
+        launch()
+    work.synchronize()
+    graph = None
+    with torch.cuda.stream(nat.capture_stream()):
+        if nat.lib().wpc_graph_begin(nat.stream()) == 0:
+            handle = ctypes.c_void_p()
+            ok = True
+            try:
+                for _ in range(reps):
+                    launch()
+            except nat.NativeError:
+                ok = False
+            finally:
+                rc = nat.lib().wpc_graph_end(nat.stream(), ctypes.addressof(handle))
+            if rc == 0 and handle.value:
+                if ok:
+                    graph = handle.value
+                else:
+                    nat.lib().wpc_graph_destroy(handle.value)
+    best = float("inf")
+    try:
+        with torch.cuda.stream(work):
+            for _ in range(rounds):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(work)
+                if graph is not None:
+                    nat.call("wpc_graph_launch", graph, nat.stream())
+                else:
+                    for _ in range(reps):
+                        launch()
+                e1.record(work)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1) / reps)
+    finally:
+        if graph is not None:
+            nat.lib().wpc_graph_destroy(graph)
+    torch.cuda.current_stream().wait_stream(work)
+    return best
+
+
+class BsrPlan:
+    """Sliced copy of a 3x3 BSR matrix and its product (``wpc_bsr3_sell_stream``).
+
+    Block rows in slices of 64 (one wavefront), block columns stored as the scalar
+    sliced copy stores columns (built by the same ``wpc_sell_*`` kernels on the
+    block pattern; 16-bit offsets from per-slice bases when they fit), the nine
+    values of each block component-major inside the slice.  One thread per block
+    row; storage-order sums, bit-identical to the oracle's ``bsr_spmv``.  The
+    fused inner-product partials belong to the plan (one stream at a time).
+    """
+
+    COL16 = True
+
+    def __init__(self, A: "BsrMatrix"):
+        if A.block_shape != (3, 3):
+            raise ValueError("BsrPlan needs 3x3 blocks")
+        if A.scalar_type not in (torch.float32, torch.float64):
+            raise TypeError("BsrPlan supports float32 and float64 values")
+        self.A = A
+        nb = A.nrow
+        dev = A.device
+        ns = nat.lib().wpc_sell_slices(nb)
+        s = nat.stream()
+        self.slice_len = torch.empty(max(ns, 1), dtype=torch.int32, device=dev)
+        self.slice_ptr = torch.zeros(ns + 1, dtype=torch.int32, device=dev)
+        nat.call("wpc_sell_lengths", nb, nat.ptr(A.offsets), None, nat.ptr(self.slice_len), s)
+        if ns:
+            nat.call("wpc_scan_inclusive", ns, nat.ptr(self.slice_len), nat.ptr(self.slice_ptr[1:]), s)
+        self.slots = int(self.slice_ptr[-1].item()) * 64
+        cols = torch.empty(max(self.slots, 1), dtype=torch.int32, device=dev)
+        self.map = torch.empty(max(self.slots, 1), dtype=torch.int32, device=dev)
+        nat.call("wpc_sell_fill", nb, nat.ptr(A.offsets), nat.ptr(A.columns), None,
+                 nat.ptr(self.slice_ptr), nat.ptr(self.slice_len), nat.ptr(cols), nat.ptr(self.map), s)
+        self.slice_base = torch.zeros(max(ns, 1), dtype=torch.int32, device=dev)
+        self.col16 = False
+        if self.COL16 and self.slots:
+            c16 = torch.empty(self.slots, dtype=torch.int16, device=dev)
+            over = torch.zeros(1, dtype=torch.int32, device=dev)
+            nat.call("wpc_sell_col16", nb, nat.ptr(A.offsets), None, nat.ptr(self.slice_ptr),
+                     nat.ptr(self.slice_len), nat.ptr(cols), nat.ptr(c16), nat.ptr(self.slice_base),
+                     nat.ptr(over), s)
+            if int(over.item()) == 0:
+                cols, self.col16 = c16, True
+            else:
+                self.slice_base.zero_()
+        self.columns = cols
+        self.values = torch.empty(max(self.slots, 1) * 9, dtype=A.scalar_type, device=dev)
+        self.refresh_values()
+        self._partials = None
+
+    def refresh_values(self):
+        """Re-gather the sliced values after A.values changed in place."""
+        nat.call("wpc_bsr3_sell_values", self.slots, nat.ptr(self.map), nat.ptr(self.A.values),
+                 self.A.values.element_size(), nat.ptr(self.values), nat.stream())
+
+    def workgroups(self):
+        return (self.A.nrow + 255) // 256
+
+    def partials(self, nvec):
+        need = max(self.workgroups(), 1) * nvec
+        if self._partials is None or self._partials.numel() < need:
+            self._partials = torch.zeros(need, dtype=torch.float64, device=self.A.device)
+            nat.bump_generation()
+        return self._partials
+
+    def fits(self, x, nvec):
+        """The int32 columns' descriptor range (ncol * 3 * nvec * element size <= 2^31)."""
+        return self.col16 or self.A.ncol * 3 * nvec * x.element_size() <= StreamPlan.DESCRIPTOR_RANGE
+
+    def run(self, x, y, z, alpha, beta, nvec=1, w=None, out=None):
+        """z = alpha A x + beta y on nvec interleaved vectors; with w, per-workgroup
+        partials of w . z (folded into `out` when given); returns (partials, count)."""
+        A = self.A
+        part = self.partials(nvec) if w is not None else None
+        nat.call("wpc_bsr3_sell_stream", A.nrow, nat.ptr(self.slice_ptr), nat.ptr(self.slice_len),
+                 nat.ptr(self.slice_base), nat.ptr(self.columns), int(self.col16),
+                 nat.ptr(self.values), nat.code(A.scalar_type), nat.ptr(x), nat.ptr(y), nat.ptr(z),
+                 nat.code(x.dtype), nvec, float(alpha), float(beta), nat.ptr(w), nat.ptr(part),
+                 nat.ptr(out), nat.stream())
+        return part, self.workgroups()
+
+
 def _device(device):
     return torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
 
