@@ -428,7 +428,7 @@ def block_bench(name, only=None):
 
     A64, A32 = bsr(torch.float64), bsr(torch.float32)
     # the BSR matrices themselves are the operators (Warp's bsr_mv in the reference's
-    # block benchmarks, block_challenge.py:82,93): the 3x3 sliced block product or
+    # block benchmarks, block_challenge.py:155,197): the 3x3 sliced block product or
     # the scalarised CSR copy, whichever the operator's timing picks
     op64, op32 = linear.aslinearoperator(A64), linear.aslinearoperator(A32)
     b = torch.tensor(rhs, device=dev)

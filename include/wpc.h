@@ -112,7 +112,7 @@ int wpc_csr_row_blocks(int n, const int* host_offsets, int* host_blocks, int cap
  * into out[v] by a single-workgroup launch when out is non-null.  alpha == 0
  * never reads A, x or u; beta == 0 never reads y; x must not alias z.  Replaces
  * sparse_operator.py:90-104 (_matvec), the two passes of fsai.py:374-381 and of
- * squared_laplacian.py:316-377 (identity free map). */
+ * squared_laplacian.py:37-98 (forward/backward, identity free map). */
 int wpc_csr_stream(int nblocks, const int* blocks, int n, int lanes, const int* offsets,
                    const int* columns, const void* values, int val_dtype, const void* s,
                    int s_dtype, const void* d, const void* u, const void* x, const void* y,
@@ -177,7 +177,7 @@ int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int
  * partials of w . z (ceil(nb / 256) per vector), folded into out[nvec] when out
  * is given.  alpha == 0 never reads x, beta == 0 never reads y; x must not alias
  * z.  Replaces Warp's bsr_mv on 3x3 blocks (sparse_operator.py:104, the CG
- * operator of benchmarks/block_challenge.py:82,93 and benchmarks/elasticity.py). */
+ * operator of benchmarks/block_challenge.py:155,197 and benchmarks/elasticity.py). */
 int wpc_bsr3_sell_values(long slots, const int* sell_map, const void* values, int elem_bytes,
                          void* sell_values, void* stream);
 int wpc_bsr3_sell_stream(int nb, const int* slice_ptr, const int* slice_len,
@@ -310,11 +310,11 @@ int wpc_sqlap_index_free(int n_free, const int* free_indices, int n, int* index,
                          void* stream);
 
 /* inverse[i] = 1/mass[i]; status[2] += non-positive/non-finite masses.
- * Replaces squared_laplacian.py:307-314 (mass_inverse). */
+ * Replaces squared_laplacian.py:27-35 (mass_inverse). */
 int wpc_sqlap_mass_inverse(int n, const void* mass, void* inverse, int dtype, int* status,
                            void* stream);
 
-/* Factored (L M^-1 L + w M)[f,f] product.  Replaces squared_laplacian.py:316-377
+/* Factored (L M^-1 L + w M)[f,f] product.  Replaces squared_laplacian.py:37-98
  * (forward/backward) as driven by _product :437-474.
  *   pass 1: tmp[r] = inverse[r] * sum_{e in row r} L_e * x~[col]   (all rows of L)
  *           x~[col] = x[index[col]] if index[col] >= 0 (boundary = 0), or
@@ -331,7 +331,7 @@ int wpc_sqlap_apply(int n_full, const int* offsets, const int* columns, const vo
                     int boundary, int lanes, void* stream);
 
 /* Block-adaptive FSAI on b x b BSR matrices (b = 2..4, b * max_blocks <= 32).
- * Replaces benchmarks/block_fsai_candidate.py:43-171 (diagonal + build); with
+ * Replaces benchmarks/block_fsai_candidate.py:45-176 (diagonal + build); with
  * refit != 0 the block pattern is read back from `cols` (:76-82, the rows as
  * sorted by wpc_fsai_sort_rows).  Writes scale[n], padded cols/vals[n * b *
  * max_blocks] in pattern order; status[0] counts nonpositive pivots. */
@@ -529,7 +529,7 @@ int wpc_cg_fold_grouped(int step, int ntasks, const void* tasks, int total_block
 /* ------------------------------------------------------------------ certified residuals
  * No single reference routine: these supply the float64 true residual b - A x
  * that the reference's solves check through Warp's cg/cr (`bsr_mv` at
- * sparse_operator.py:104, squared_laplacian.py:437-474), accumulated with the
+ * sparse_operator.py:104, squared_laplacian.py:158-195), accumulated with the
  * compensated Dot2 product so that the residual itself carries no cancellation
  * noise.  Vectors are float64 [n][nvec] (nvec 1..8).
  *   assembled operator (t_hi = t_lo = NULL):  r = b - A x        (values = A)
@@ -561,7 +561,7 @@ int wpc_add_diagonal(int n, const int* offsets, const int* columns, double* valu
 
 /* ------------------------------------------------------------------ GMRES basis
  * The Arnoldi products of restarted GMRES (warp.optim.linear.gmres as called at
- * tests/test_indefinite.py:130 and tests/test_biharmonic.py:312) on a row-major
+ * tests/test_indefinite.py:130,218 and tests/test_biharmonic.py:32) on a row-major
  * basis V (row j at V + j*ld, ld >= n), 1 <= k <= 128 rows:
  *   wpc_gram:          h[j] = V[j] . w (double, deterministic two-stage sum;
  *                      partials: double[wpc_gram_partials(n) * k])

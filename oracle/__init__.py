@@ -7,5 +7,5 @@ the ``warp_preconditioners`` package) never imports it.
 ``oracle.cpu`` wraps ``_build/liboracle.so`` (built from ``wpc_oracle.cpp`` by
 ``oracle/Makefile``) with numpy-level pipelines that mirror the reference's host
 logic (fsai.py:222-344, fsai_reuse.py:157-245, block_ilu.py:225-450,
-squared_laplacian.py:399-497).
+squared_laplacian.py:103-218).
 """

@@ -113,7 +113,7 @@ class CSR:
 
 
 def scalarize(A: CSR) -> CSR:
-    """fsai_pack.py:128-149 -- expand compact b x b BSR keeping stored zeros."""
+    """fsai_pack.py:75-130 -- expand compact b x b BSR keeping stored zeros."""
     if A.block == 1:
         return A
     b = A.block
@@ -302,7 +302,7 @@ def source_map(S: CSR, A: CSR) -> np.ndarray:
 
 # ---------------------------------------------------------------- squared Laplacian
 def sqlap_setup(L: CSR, mass: np.ndarray, free: np.ndarray):
-    """squared_laplacian.py:399-431 (validation and inverse mass)."""
+    """squared_laplacian.py:120-152 (validation and inverse mass)."""
     n = L.nrow
     index = np.zeros(n, np.int32)
     status = np.zeros(3, np.int32)
@@ -319,7 +319,7 @@ def sqlap_setup(L: CSR, mass: np.ndarray, free: np.ndarray):
 
 
 def sqlap_apply(op, x, y=None, alpha=1.0, beta=0.0, mass_weight=0.0, boundary=False):
-    """squared_laplacian.py:437-477."""
+    """squared_laplacian.py:158-218 (product and Dirichlet right-hand side)."""
     L = op["L"]
     dt = np.dtype(L.dtype)
     nf = len(op["free"])

@@ -530,7 +530,7 @@ static void csr_spmv(int mat_dt, int n, const int* off, const int* col, const vo
     for (int row = 0; row < n; ++row) Store<V>::store(z, row, out[row]);
 }
 
-// squared_laplacian.py:28-35 and :317-377 -- factored (L M^-1 L + w M)[f,f] product.
+// squared_laplacian.py:28-98 -- factored (L M^-1 L + w M)[f,f] product.
 template <class T>
 static void sqlap_apply(int n_full, const int* off, const int* cols, const void* vals,
                         const void* inverse, const int* index, int n_free, const int* free,
@@ -761,7 +761,7 @@ static T bsr_entry(const int* off, const int* col, const T* val, int b, int i, i
     return v;
 }
 
-// benchmarks/block_fsai_candidate.py:43-57 -- scale = 1/sqrt(A_ii), status[0]++ if not.
+// benchmarks/block_fsai_candidate.py:45-59 -- scale = 1/sqrt(A_ii), status[0]++ if not.
 template <class T>
 static void block_fsai_diagonal(int n, int b, const int* off, const int* col, const T* val,
                                 T* scale, int* status) {
@@ -776,7 +776,7 @@ static void block_fsai_diagonal(int n, int b, const int* off, const int* col, co
     }
 }
 
-// benchmarks/block_fsai_candidate.py:59-171 -- one block row: greedy block-pattern
+// benchmarks/block_fsai_candidate.py:61-176 -- one block row: greedy block-pattern
 // growth (Cholesky of the equilibrated principal submatrix recomputed per step,
 // b identity right-hand sides, block scores), then G_i = chol(Z_ii)^-1 Z^T D^-1/2
 // written as b padded scalar rows of `width` = b * B entries in pattern order.
@@ -933,7 +933,7 @@ void orc_sqlap_index_free(int n_free, const int* free, int n, int* index, int* s
     }
 }
 
-// squared_laplacian.py:306-314
+// squared_laplacian.py:27-35 (mass_inverse)
 int orc_sqlap_mass_inverse(int dt, int n, const void* mass, void* inverse, int* status) {
     DISPATCH_T(dt, ({
         for (int i = 0; i < n; ++i) {

@@ -144,3 +144,40 @@ def test_block_operator_cg_against_oracle(oracle):
     xo, ito, _ = oracle.krylov("cg", b, A=Ao, P=Po, tol=1e-10, maxiter=5000, check_every=1)
     assert abs(out["iterations"][0] - ito) <= max(2, 0.02 * ito)
     np.testing.assert_allclose(host(X[:, 0]), xo, rtol=1e-7, atol=1e-9 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("size", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_generic_bsr_spmv_against_oracle(oracle, size, dtype):
+    # wpc_bsr_spmv (b x b blocks, 8 lanes per block row and a lane reduction: a
+    # different summation order) against the oracle's product within rounding,
+    # exactly on small-integer data, with empty block rows and the alpha/beta rules.
+    from warp_preconditioners import _native as nat
+    from warp_preconditioners.sparse import BsrMatrix
+
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    rng = np.random.default_rng(200 + size)
+    nb = 300
+    p = _pattern(rng, nb, [0 if i % 37 == 5 else 1 + (i % 11) for i in range(nb)])
+    for kind in ("integer", "random"):
+        if kind == "integer":
+            data = rng.integers(-4, 5, size=(p.nnz, size, size)).astype(npd)
+            x, y = rng.integers(-3, 4, size=(2, nb * size)).astype(npd)
+        else:
+            data = rng.normal(size=(p.nnz, size, size)).astype(npd)
+            x, y = rng.normal(size=(2, nb * size)).astype(npd)
+        A = BsrMatrix(nb, nb, (size, size), torch.tensor(p.indptr.astype(np.int32), device=dev()),
+                      torch.tensor(p.indices.astype(np.int32), device=dev()),
+                      torch.tensor(data, device=dev()))
+        z = torch.empty(nb * size, dtype=dtype, device=dev())
+        nat.call("wpc_bsr_spmv", nb, size, nat.ptr(A.offsets), nat.ptr(A.columns),
+                 nat.ptr(A.values), nat.code(dtype), nat.ptr(torch.tensor(x, device=dev())),
+                 nat.ptr(torch.tensor(y, device=dev())), nat.ptr(z), 2.0, -0.5, nat.stream())
+        Ao = oracle.CSR(p.indptr.astype(np.int32), p.indices.astype(np.int32),
+                        np.ascontiguousarray(data), nb, nb, size)
+        ref = oracle.bsr_spmv(Ao, x, y, 2.0, -0.5)
+        if kind == "integer":
+            np.testing.assert_array_equal(host(z), ref)
+        else:
+            tol = 2e-5 if dtype == torch.float32 else 1e-13
+            np.testing.assert_allclose(host(z), ref, rtol=tol, atol=tol)

@@ -178,7 +178,7 @@ def test_structure_pack_and_transpose(oracle):
 
 
 def test_factor_dtype_conversion(oracle):
-    # tests/test_squared_laplacian.py:152-192: float32 factor of a float64 matrix.
+    # tests/test_squared_laplacian.py:92-109: float32 factor of a float64 matrix.
     with pytest.raises(ValueError, match="positive diagonal"):
         oracle.fsai(csr(oracle, np.eye(3) * 1e100), factor_dtype=np.float32)
     n = 25

@@ -31,7 +31,7 @@ __device__ __forceinline__ T bsr_entry(const int* __restrict__ off, const int* _
     return v;
 }
 
-// block_fsai_candidate.py:43-57
+// block_fsai_candidate.py:45-59
 template <class T>
 __global__ __launch_bounds__(256) void block_diagonal_kernel(int n, int b, const int* __restrict__ off,
                                                              const int* __restrict__ col,
@@ -57,7 +57,7 @@ __device__ __forceinline__ bool better(T s, int c, T bs, int bc) {
     return s > bs || (s == bs && c < bc);
 }
 
-// block_fsai_candidate.py:59-171
+// block_fsai_candidate.py:61-176
 template <class T>
 __global__ __launch_bounds__(kBfWaves * kWave) void block_fsai_kernel(
     int nb, int b, int B, const int* __restrict__ off, const int* __restrict__ col,

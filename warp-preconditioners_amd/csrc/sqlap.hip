@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void mass_inverse_kernel(int n, const T* mass,
     else atomicAdd(status + 2, 1);
 }
 
-// squared_laplacian.py:316-344 (forward).
+// squared_laplacian.py:37-65 (forward).
 template <class T, int NV>
 __global__ __launch_bounds__(256) void forward_kernel(int n_full, const int* __restrict__ off,
                                                       const int* __restrict__ col,
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void forward_kernel(int n_full, const int* __r
     }
 }
 
-// squared_laplacian.py:346-377 (backward).
+// squared_laplacian.py:67-98 (backward).
 template <class T, int NV>
 __global__ __launch_bounds__(256) void backward_kernel(
     int n_free, const int* __restrict__ off, const int* __restrict__ col, const T* __restrict__ val,

@@ -497,7 +497,7 @@ def gmres(A, b, x, tol=None, atol=None, restart=31, maxiter=0, M=None, callback=
     """Restarted right-preconditioned GMRES; returns (iterations, residual, atol).
 
     The signature follows warp.optim.linear.gmres as the reference's tests call it
-    (tests/test_biharmonic.py:312, tests/test_indefinite.py:130).  Operator and
+    (tests/test_biharmonic.py:32, tests/test_indefinite.py:130,218).  Operator and
     preconditioner applies, the Gram products and basis updates (classical
     Gram-Schmidt with one reorthogonalization) and the norms run on the HIP
     kernels with the Krylov basis resident on the device; only the small

@@ -13,7 +13,7 @@ Both operator kinds of the reference's factored/assembled paths are covered:
 ``SparseOperator`` or a BSR matrix operator on an assembled float64 matrix
 (r = b - A x) and the
 identity-free ``SquaredLaplacianOperator`` (r = b - w M x - K diag(1/m) K x,
-squared_laplacian.py:437-474).
+squared_laplacian.py:158-195).
 """
 
 from __future__ import annotations

@@ -597,7 +597,7 @@ def from_scipy(a, device=None, dtype=torch.float64, block_shape=(1, 1)) -> BsrMa
 
 def scalarize(A: BsrMatrix) -> BsrMatrix:
     """Expand square b x b blocks into scalar CSR on the device, keeping stored zeros
-    (fsai_pack.py:128-149)."""
+    (fsai_pack.py:75-130)."""
     b = A.block_shape[0]
     if A.block_shape != (b, b):
         raise ValueError("scalarize requires square blocks")

@@ -3,7 +3,7 @@
 ``SquaredLaplacianOperator(L, mass, free_indices, row_lanes, mass_weight)`` applies
 ``(L diag(1/mass) L + mass_weight diag(mass))[free, free]`` as two sparse passes
 and builds the Dirichlet right-hand side with the same arithmetic
-(squared_laplacian.py:382-497).  All passes are ``wpc_sqlap_apply``.
+(squared_laplacian.py:103-218).  All passes are ``wpc_sqlap_apply``.
 """
 
 from __future__ import annotations
