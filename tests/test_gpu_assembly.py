@@ -73,3 +73,22 @@ def test_device_pipeline_factor_matches_oracle_hash():
     assert h.hexdigest() == meta["biharmonic64_Q_w8"]["pattern_sha256"]
     assert hashlib.sha256(host(P.G.values).tobytes()).hexdigest() == \
         meta["biharmonic64_Q_w8"]["values_sha256"]
+
+
+@pytest.mark.parametrize("side", [16, 128])
+def test_device_spgemm_reproduces_recorded_grid_hashes(side):
+    # The device SpGEMM (K diag(1/m) K + alpha M with K = the five-point Laplacian,
+    # m = 1, alpha = 0.05) builds the reference's grid() matrix byte for byte: the
+    # four matrix_sha256 values of results/fsai-float16-regression.json (float32 by
+    # the device conversion of the float64 values).
+    from test_golden import GRID_SHA256, grid_signature
+    from warp_preconditioners import assembly, workloads
+    from warp_preconditioners.sparse import bsr_copy
+
+    L = upload(workloads.laplacian_2d(side))
+    ones = torch.ones(side * side, dtype=torch.float64, device=dev())
+    Q = assembly.squared_operator(L, ones, 0.05)
+    for dt, name in ((torch.float64, "float64"), (torch.float32, "float32")):
+        Qd = Q if dt == torch.float64 else bsr_copy(Q, scalar_type=dt)
+        got = grid_signature(host(Qd.offsets), host(Qd.columns), host(Qd.values))
+        assert got == GRID_SHA256[(side, name)]
