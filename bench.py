@@ -142,29 +142,15 @@ class Solver:
             self.work = out["work"]
             self.iterations = out["iterations"]
             return
-        # mixed: float32 Krylov on the residual, float64 true residual and update
-        bnorm = torch.linalg.vector_norm(self.B, dim=0)
-        R = self.B.clone()
-        its = [0] * self.k
-        C = torch.zeros(self.B.shape, dtype=torch.float32, device=self.B.device)
-        for outer in range(30):
-            self.op64.apply_batch(self.X, self.B, R, -1.0, 1.0, self.k)
-            rn = torch.linalg.vector_norm(R, dim=0)
-            if bool((rn <= tol * bnorm).all()):
-                break
-            scale = torch.where(rn > 0, rn, torch.ones_like(rn))
-            R32 = (R / scale).float().contiguous()
-            C.zero_()
-            out = linear.solve_batched("cg", self.op32, R32, C, M=self.P, tol=self.inner_tol,
-                                       maxiter=maxiter, check_every=check_every,
-                                       work=self.inner_work)
-            self.inner_work = out["work"]
-            its = [a + b for a, b in zip(its, out["iterations"])]
-            active = (rn > tol * bnorm).double()
-            self.X += C.double() * (scale * active)
-        self.iterations = its
-
-    inner_tol = 1e-3
+        # mixed: one float32 CG sequence (float32 operator copy, vectors and
+        # preconditioner apply) whose residual is replaced by the compensated float64
+        # true residual at intervals (reliable updates); stops on the true residual
+        out = linear.solve_reliable(self.op64, self.B, self.X, self.P, self.op32, tol=tol,
+                                    maxiter=maxiter, check_every=check_every,
+                                    work=self.inner_work)
+        self.inner_work = out["work"]
+        self.iterations = out["iterations"]
+        self.replacements = out["replacements"]
 
 
 def true_residual(s, X=None):
@@ -372,18 +358,17 @@ def profile_kernels(s, reps=10):
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary (a
-    markdown table row `| kernel | fetch KiB | write KiB | bytes |`), or None."""
-    path = ROOT / "profiles" / "factored_biharmonic_64cube_traffic.md"
-    if not path.exists():
-        return None
-    for line in path.read_text().splitlines():
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == kernel:
-            try:
-                return float(cells[3])
-            except ValueError:
-                return None
+    """HBM bytes per launch of `kernel` from the committed counter summaries
+    (profiles/*traffic*.md: a markdown table row `| kernel | fetch KiB | write KiB |
+    bytes | ...`), or None."""
+    for path in sorted((ROOT / "profiles").glob("*traffic*.md")):
+        for line in path.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 4 and cells[0] == kernel:
+                try:
+                    return float(cells[3].replace(",", ""))
+                except ValueError:
+                    return None
     return None
 
 
@@ -515,6 +500,8 @@ def block_bench(name, only=None):
         # preconditioner: CG may break down on it (recorded), GMRES is the fit
         methods = [("ilu0-cg-f64", "ilu0", "cg", "f64", None),
                    ("ilu0-gmres-f64", "ilu0", "gmres", "f64", None),
+                   # the reference's suggested setting (results/indefinite.md:203)
+                   ("ilu0s20-cg-f64", "ilu0s20", "cg", "f64", None),
                    ("fsai8-cg-f64", "fsai8", "cg", "f64", None),
                    ("fsai8-cg-f64-bfloat16", "fsai8", "cg", "f64", torch.bfloat16),
                    ("fsai8-cg-mixed-bfloat16", "fsai8", "cg", "mixed", torch.bfloat16),
@@ -532,6 +519,8 @@ def block_bench(name, only=None):
             P, setup = timed(lambda: ready(wp.BlockJacobi(A, pivot_tolerance=1e-15), vt))
         elif prec == "ilu0":
             P, setup = timed(lambda: ready(wp.BlockILU0(A, factor_sweeps=3), vt))
+        elif prec == "ilu0s20":
+            P, setup = timed(lambda: ready(wp.BlockILU0(A, factor_sweeps=20, solve_sweeps=8), vt))
         elif prec.startswith("blockfsai"):
             # block FSAI: 5 blocks of 3 = 15 scalars per row (block_fsai_candidate.py)
             P, setup = timed(lambda: ready(wp.BlockFSAI(A, max_blocks=int(prec[9:]),
@@ -592,13 +581,14 @@ def block_bench(name, only=None):
                       "operator": operator, "methods": records}), flush=True)
 
 
-def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8), reps=50):
-    """FSAI-apply HBM throughput: z = G^T (G x) with the width-8 factor of the
-    assembled biharmonic Q (device-assembled on the jittered grid, float32 factor,
-    float64 vectors), both passes timed together: `reps` applies recorded into a
-    graph and replayed, HIP events on the replay stream; algorithmic bytes 2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (4-byte
-    column indices, whatever the layout stores: `index_bytes` reports 2 where a pass
-    reads int16 column differences).  Prints one JSON line."""
+def apply_rows(sides, nvecs, reps=50):
+    """FSAI-apply timings: z = G^T (G x) with the width-8 factor of the assembled
+    biharmonic Q (device-assembled on the jittered grid, float32 factor, float64
+    vectors), both passes timed together: `reps` applies recorded into a graph and
+    replayed, HIP events on the replay stream; algorithmic bytes
+    2 nnzG (4 + 4) + 8 (n + 1) + 4 n k 8 per apply (4-byte column indices, whatever
+    the layout stores: `index_bytes` reports 2 where a pass reads 16-bit column
+    offsets).  Returns one row per (side, vector count)."""
     import torch
 
     import warp_preconditioners as wp
@@ -641,14 +631,122 @@ def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8), reps=50):
             ib = [2 if (pl.mode(k, torch.float64) == pl.SLICED and pl._sell is not None
                         and pl._sell[6]) else 4 for pl in (pg, pgt)]
             nbytes = 2 * nnz * (4 + 4) + 8 * (n + 1) + 4 * n * k * 8
-            rows.append(dict(side=side, n=n, nnzG=nnz, nvec=k, us=us, GBs=nbytes / us / 1e3,
-                             frac=nbytes / us / 1e3 / HBM_PEAK_GBS,
+            rows.append(dict(side=side, n=n, nnzG=nnz, nvec=k, us=us, bytes=nbytes,
+                             GBs=nbytes / us / 1e3, frac=nbytes / us / 1e3 / HBM_PEAK_GBS,
                              mapping=[pg.mode(k, torch.float64), pgt.mode(k, torch.float64)],
                              index_bytes=ib))
             log(json.dumps(rows[-1]))
         del P, Q, K, m
+    return rows
+
+
+def apply_bench(sides=(64, 96, 128), nvecs=(1, 3, 8)):
+    """FSAI-apply HBM throughput at three mesh sizes (apply_rows).  Prints one JSON
+    line."""
+    rows = apply_rows(sides, nvecs)
     print(json.dumps({"metric": "FSAI-apply HBM GB/s", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                       "rows": rows}), flush=True)
+
+
+def beam_product_row():
+    """The configuration-4 operator product alone: z = A x on the 40x40x160 P1
+    elasticity beam (3x3 BSR, float64 values and vector), through the matrix
+    operator (the 3x3 sliced block product or the scalarised copy, whichever its
+    timing picks), graph-replayed and timed with HIP events.  Algorithmic bytes
+    nnzb (9 * 8 + 4) + 4 (nb + 1) + 2 * 3 nb * 8."""
+    import torch
+
+    from warp_preconditioners import linear, workloads
+    from warp_preconditioners.sparse import BsrMatrix, time_graph
+
+    dev = torch.device("cuda", 0)
+    v, t, _ = workloads.kuhn_grid(None, 0.15, seed=0, shape=(160, 40, 40))
+    E, _, _, _ = workloads.p1_elasticity(v, t, v[:, 0] == 0.0, young=1.0, poisson=0.45)
+    nb, nnzb = E.shape[0] // 3, E.nnz // 9
+    A = BsrMatrix(nb, nb, (3, 3), torch.tensor(E.indptr.astype(np.int32), device=dev),
+                  torch.tensor(E.indices.astype(np.int32), device=dev),
+                  torch.tensor(np.ascontiguousarray(E.data), device=dev))
+    op = linear.aslinearoperator(A)
+    x = torch.randn(nb * 3, dtype=torch.float64, device=dev)
+    z = torch.empty_like(x)
+    mode = op.block_mode(1, torch.float64)
+    ms = time_graph(lambda: op.apply_batch(x, x, z, 1.0, 0.0, 1), reps=50)
+    nbytes = nnzb * (9 * 8 + 4) + 4 * (nb + 1) + 2 * 3 * nb * 8
+    return dict(matrix="beam40x40x160_elasticity", nb=nb, nnzb=nnzb, mode=mode, us=ms * 1e3,
+                bytes=nbytes, GBs=nbytes / (ms * 1e-3) / 1e9,
+                frac=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+
+
+def large_rooflines():
+    """Roofline entries whose working set exceeds the 256 MB Infinity Cache: the
+    128^3 FSAI apply (1 and 3 right-hand sides) and the elasticity beam's block
+    product."""
+    out = []
+    for r in apply_rows((128,), (1, 3)):
+        out.append({"kernel": f"fsai_apply_128cube_{r['nvec']}rhs", "bound": "hbm",
+                    "achieved": r["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": r["frac"], "bytes": r["bytes"], "avg_us": r["us"],
+                    "traffic": measured_traffic(f"fsai_apply_128cube_{r['nvec']}rhs")})
+    r = beam_product_row()
+    out.append({"kernel": "bsr3_product_beam40x40x160", "bound": "hbm", "achieved": r["GBs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac"], "bytes": r["bytes"],
+                "avg_us": r["us"], "mapping": r["mode"],
+                "traffic": measured_traffic("bsr3_product_beam40x40x160")})
+    return out
+
+
+def setup_bench(widths=(8, 16, 32, 48), reps=3):
+    """FSAI setup and fixed-pattern refit times: the adaptive build at widths 8-48 on
+    the assembled biharmonic Q of the 64^3 mesh (n = 262,144, 65-point rows) and on
+    the configuration-4 elasticity beam (3x3 BSR, n = 763,200), and the numerical
+    refit (FSAI.update on the stored supports: gather, local solves, commit) of
+    each factor after the matrix values change.  Median of `reps` synchronized
+    runs after one warm run.  Prints one JSON line."""
+    import scipy.sparse as ss
+    import torch
+
+    import warp_preconditioners as wp
+    from warp_preconditioners import workloads
+    from warp_preconditioners.sparse import BsrMatrix, from_scipy
+
+    dev = torch.device("cuda", 0)
+    v, t, _ = workloads.kuhn_grid(64, 0.15, seed=0)
+    K, m = workloads.p1_laplacian(v, t)
+    Q = workloads.squared_operator(K, m, 1.0)
+    v, t, _ = workloads.kuhn_grid(None, 0.15, seed=0, shape=(160, 40, 40))
+    E, _, _, _ = workloads.p1_elasticity(v, t, v[:, 0] == 0.0, young=1.0, poisson=0.45)
+    nb = E.shape[0] // 3
+    mats = {
+        "biharmonic64_Q": (from_scipy(Q, dev, torch.float64),
+                           from_scipy(Q + 0.5 * ss.diags(m), dev, torch.float64)),
+        "beam40x40x160_elasticity": tuple(
+            BsrMatrix(nb, nb, (3, 3), torch.tensor(E.indptr.astype(np.int32), device=dev),
+                      torch.tensor(E.indices.astype(np.int32), device=dev),
+                      torch.tensor(np.ascontiguousarray(E.data) * s_, device=dev))
+            for s_ in (1.0, 1.5)),
+    }
+
+    def timed(fn):
+        out = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            out.append(time.perf_counter() - t0)
+        return r, float(np.median(out[1:]))
+
+    rows = []
+    for name, (A, A2) in mats.items():
+        for w in widths:
+            P, setup = timed(lambda: wp.FSAI(A, max_row_size=w, reuse_pattern=True))
+            _, refit = timed(lambda: P.update(A2))
+            rows.append(dict(matrix=name, n=A.shape[0], width=w, nnzG=P.G.nnz,
+                             truncated_rows=int(P.truncated_rows), setup_ms=setup * 1e3,
+                             refit_ms=refit * 1e3))
+            log(json.dumps(rows[-1]))
+            del P
+    print(json.dumps({"workload": "setup", "rows": rows}), flush=True)
 
 
 def serving_bench(args):
@@ -683,7 +781,9 @@ def serving_bench(args):
         makespan = max(r.finished for r in out)
         return eng, out, makespan
 
-    eng, out, makespan = run(ServingConfig(keep_solutions=False, group=args.group), "batched")
+    width = {"laplacian": 8, "biharmonic": args.bih_width}
+    eng, out, makespan = run(ServingConfig(keep_solutions=False, group=args.group, width=width),
+                             "batched")
     lat = np.array([r.latency for r in out])
     rhs = sum(r.nrhs for r in out)
     bounds = [b for r in out for b in r.field_bound]
@@ -764,11 +864,14 @@ def main():
                          "Krylov vectors; serving: the one-at-a-time engine)")
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the roofline entries on working sets above the Infinity Cache")
     ap.add_argument("--mapping", default=None,
                     help="force the row mapping of the G, GT and K products, e.g. --mapping=4,0,-1 "
                          "(0 = streamed, L = L lanes per row, -1 = sliced); default autotunes")
     ap.add_argument("--workload", default="biharmonic64",
-                    choices=["biharmonic64", "serving", "kron48", "elasticity", "apply"],
+                    choices=["biharmonic64", "serving", "kron48", "elasticity", "apply",
+                             "setup"],
                     help="biharmonic64: the factored-biharmonic 64^3 solve (default); serving: "
                          "the request stream through the continuous-batching engine; kron48 / "
                          "elasticity: the block-coupled and elasticity solver comparisons")
@@ -776,8 +879,13 @@ def main():
                     help="kron48 / elasticity: comma-separated method labels to run (default all)")
     ap.add_argument("--requests", type=int, default=512, help="serving: stream length")
     ap.add_argument("--max-side", type=int, default=96, help="serving: largest mesh side")
+    ap.add_argument("--bih-width", type=int, default=8,
+                    help="serving: FSAI width of the factored-biharmonic systems")
     ap.add_argument("--group", action="store_true",
                     help="serving: advance same-shape small solves with one launch per pass")
+    ap.add_argument("--profile-large", action="store_true",
+                    help="time only the roofline entries on working sets above the Infinity "
+                         "Cache (and the calibrating copy); for counter runs under rocprofv3")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="time only the per-iteration kernels (and a 256 MiB copy that "
                          "calibrates HBM byte counters); for counter runs under rocprofv3")
@@ -796,6 +904,20 @@ def main():
         return
     if args.workload == "apply":
         apply_bench()
+        return
+    if args.workload == "setup":
+        setup_bench()
+        return
+    if args.profile_large:
+        # the large-working-set kernels alone, then the calibrating copy
+        rows = large_rooflines()
+        src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline_large": rows, "copy_probe_bytes": COPY_PROBE_BYTES}),
+              flush=True)
         return
     wl = build_workload(args.side)
     s = Solver(wl, args.mode)
@@ -896,6 +1018,12 @@ def main():
             del p
         f64 = record["compare"]["plain-f64"]["solves_per_s"]
         record["compare"]["batched_vs_plain_f64"] = record["value"] / f64
+    if not args.no_large:
+        # kernels on working sets above the Infinity Cache, timed after the solves
+        try:
+            record["roofline_large"] = large_rooflines()
+        except Exception as exc:  # reported, never required
+            record["roofline_large"] = {"error": str(exc)}
     if not args.no_cpu:
         try:
             record["cpu_baseline"] = cpu_baseline(wl, s, max(s.iterations))

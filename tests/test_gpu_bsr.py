@@ -170,9 +170,10 @@ def test_generic_bsr_spmv_against_oracle(oracle, size, dtype):
                       torch.tensor(p.indices.astype(np.int32), device=dev()),
                       torch.tensor(data, device=dev()))
         z = torch.empty(nb * size, dtype=dtype, device=dev())
+        xd, yd = torch.tensor(x, device=dev()), torch.tensor(y, device=dev())  # (kept alive)
         nat.call("wpc_bsr_spmv", nb, size, nat.ptr(A.offsets), nat.ptr(A.columns),
-                 nat.ptr(A.values), nat.code(dtype), nat.ptr(torch.tensor(x, device=dev())),
-                 nat.ptr(torch.tensor(y, device=dev())), nat.ptr(z), 2.0, -0.5, nat.stream())
+                 nat.ptr(A.values), nat.code(dtype), nat.ptr(xd), nat.ptr(yd), nat.ptr(z), 2.0,
+                 -0.5, nat.stream())
         Ao = oracle.CSR(p.indptr.astype(np.int32), p.indices.astype(np.int32),
                         np.ascontiguousarray(data), nb, nb, size)
         ref = oracle.bsr_spmv(Ao, x, y, 2.0, -0.5)

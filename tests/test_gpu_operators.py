@@ -355,3 +355,28 @@ def test_block_ilu_few_sweeps_match_oracle(wp, oracle, sweeps, relaxation):
     pre.matvec(torch.tensor(x, device=dev()), z, z, 0.9, -0.4)
     ref = oracle.block_ilu0_apply(Ao, factors, inv, x, y, 0.9, -0.4, solve_sweeps=4)
     np.testing.assert_allclose(host(z), ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("sweeps,solve_sweeps", [(3, 4), (20, 8)])
+def test_block_ilu_on_reduced_beam_matches_oracle(wp, oracle, sweeps, solve_sweeps):
+    # The reduced elasticity beam of tests/test_oracle_ilu_beam.py (where the oracle
+    # shows CG diverging with 3 sweeps and converging with 20/8): the device factors
+    # and apply equal the oracle's, so the port computes the same preconditioner.
+    from test_oracle_ilu_beam import beam
+    from warp_preconditioners.sparse import BsrMatrix
+
+    K, b = beam()
+    nb = K.shape[0] // 3
+    A = BsrMatrix(nb, nb, (3, 3), torch.tensor(K.indptr.astype(np.int32), device=dev()),
+                  torch.tensor(K.indices.astype(np.int32), device=dev()),
+                  torch.tensor(np.ascontiguousarray(K.data), device=dev()))
+    pre = wp.BlockILU0(A, factor_sweeps=sweeps, solve_sweeps=solve_sweeps)
+    Ao = oracle.CSR(K.indptr.astype(np.int32), K.indices.astype(np.int32),
+                    np.ascontiguousarray(K.data), nb, nb, 3)
+    factors, inv = oracle.block_ilu0(Ao, factor_sweeps=sweeps)
+    scale = np.abs(factors).max()
+    np.testing.assert_allclose(host(pre.factors), factors, rtol=1e-11, atol=1e-13 * scale)
+    z = torch.empty(nb * 3, dtype=torch.float64, device=dev())
+    pre.matvec(torch.tensor(b, device=dev()), z, z, 1.0, 0.0)
+    ref = oracle.block_ilu0_apply(Ao, factors, inv, b, solve_sweeps=solve_sweeps).ravel()
+    np.testing.assert_allclose(host(z), ref, rtol=1e-10, atol=1e-13 * np.abs(ref).max())

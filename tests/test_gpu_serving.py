@@ -357,3 +357,35 @@ def test_serving_budget_evicts_and_still_serves():
     for req, res in zip(reqs, out):
         xs, m = _reference_solution(req)
         assert np.all(_field_error(res.x, xs, m) <= 1e-8)
+
+
+def test_serving_configuration_sides_certified_and_mix_invariant():
+    # Meshes of the configuration-5 range (sides 16-28 here, where host direct solves
+    # stay quick), both kinds, built / cached / refit factors all present: every
+    # result certified to 1e-8 and within it of a host direct solve, and equal to
+    # the same request solved alone (bit for bit for built and cached factors).
+    from warp_preconditioners.serving import MeshSpec, Request, ServingConfig, ServingEngine
+
+    plan = [(16, 11, "laplacian", 3), (20, 12, "biharmonic", 2), (16, 11, "laplacian", 5),
+            (16, 13, "laplacian", 1), (20, 12, "biharmonic", 4), (20, 14, "biharmonic", 3),
+            (28, 15, "biharmonic", 2), (24, 16, "laplacian", 8)]
+    reqs = [Request(i, 0.001 * i, MeshSpec(side, seed), kind, nrhs, 900 + i)
+            for i, (side, seed, kind, nrhs) in enumerate(plan)]
+    eng = ServingEngine(ServingConfig(streams=3, max_active=8, chunk=25))
+    mix = eng.serve(reqs, realtime=True)
+    kinds = {(r.kind, res.factor) for r, res in zip(reqs, mix)}
+    for kind in ("laplacian", "biharmonic"):
+        assert {(kind, "built"), (kind, "cached"), (kind, "refit")} <= kinds, kinds
+    for req, res in zip(reqs, mix):
+        assert res.converged and max(res.field_bound) <= 1e-8, (req, res.field_bound)
+        xs, m = _reference_solution(req)
+        err = _field_error(res.x, xs, m)
+        assert np.all(err <= 1e-8), (req, err)
+        assert np.all(err <= np.array(res.field_bound) * 1.01 + 1e-10)
+        alone = ServingEngine(ServingConfig(streams=1, max_active=1, chunk=25)).serve(
+            [req], realtime=False)[0]
+        if res.factor in ("built", "cached"):
+            np.testing.assert_array_equal(alone.x, res.x)
+            assert alone.iterations == res.iterations
+        else:
+            assert np.all(_field_error(res.x, alone.x, m) <= 2e-8)

@@ -489,6 +489,99 @@ def solve_batched(method, A, B, X, M=None, tol=1.0e-8, atol=0.0, maxiter=10000, 
                 targets=[float(v) ** 0.5 for v in st[2]], chunks=chunks, work=work)
 
 
+def solve_reliable(A, B, X, M, inner, tol=1.0e-8, delta=0.1, maxiter=40000, check_every=25,
+                   use_cuda_graph=True, work=None):
+    """Mixed-precision CG with reliable updates (Sleijpen & van der Vorst, 1996): one
+    float32 Krylov sequence on the operator `inner` (float32 vectors) with
+    preconditioner M, whose recursive residual is replaced by the float64 true
+    residual b - A x at intervals instead of restarting the inner solve.
+
+    B, X: contiguous float64 [n, k] (X is the initial guess and the result).  The
+    float32 iteration runs on the residual scaled by s_v = ||b_v - A x0_v|| (one
+    scale for the whole solve) and accumulates its solution increment in float32
+    Y; every `check_every` steps the recursive ||r|| is read, and when it has
+    fallen below `delta` times its value at the last replacement for every moving
+    right-hand side (or meets the target), X += s Y in float64, Y = 0, and the
+    float32 residual, its preconditioned residual and the r.z / r.r state are
+    replaced from the compensated float64 residual -- the search direction p is
+    kept, so the Krylov sequence continues (no restart).  A right-hand side
+    finishes when its true relative residual ||b - A x|| / ||b|| <= tol.
+
+    Returns dict(iterations, replacements, relative_residual, converged, work).
+    """
+    from .refinement import TrueResidual
+
+    A = aslinearoperator(A)
+    inner = aslinearoperator(inner)
+    M = aslinearoperator(M)
+    n, k = B.shape
+    if B.dtype != torch.float64 or X.dtype != torch.float64 or tuple(X.shape) != (n, k):
+        raise TypeError("solve_reliable takes float64 B and X of one [n, k] shape")
+    if not (B.is_contiguous() and X.is_contiguous()):
+        raise ValueError("B and X must be contiguous")
+    dev = B.device
+    res = TrueResidual(A, torch.ones(n, dtype=torch.float64, device=dev), k)
+    Bf, Xf = B.reshape(-1), X.reshape(-1)
+    bn = res.run(torch.zeros_like(Xf), Bf).cpu().numpy()[:k] ** 0.5
+    res.run(Xf, Bf)
+    scale = res.norms.clone()  # [||r0||^2 ..., ||x0||^2 ...]: the fixed residual scale
+    s0 = scale.cpu().numpy()[:k] ** 0.5
+    if work is None or work.nvec != k or work.n != n or work.dtype != torch.float32:
+        work = KrylovWork("cg", inner, M, n, k, torch.float32, dev)
+    Y = torch.zeros(n * k, dtype=torch.float32, device=dev)
+    R32 = torch.empty(n * k, dtype=torch.float32, device=dev)
+    full = (1 << k) - 1
+    stream = nat.stream()
+    # target on the scaled recursive residual: tol ||b_v|| / s_v
+    target = np.array([tol * bn[v] / s0[v] if s0[v] > 0 else 0.0 for v in range(k)])
+    nat.call("wpc_refine_prepare", n, k, nat.ptr(res.r), nat.ptr(scale), full, nat.ptr(R32),
+             nat.code(torch.float32), stream)
+    work.start(R32, Y, 0.0, 0.0)
+    work._field(2).copy_(torch.tensor(target ** 2, dtype=torch.float64, device=dev))
+    true_rel = np.array([res_v ** 0.5 for res_v in res.norms.cpu().numpy()[:k]]) / np.where(
+        bn > 0, bn, 1.0)
+    last = np.ones(k)  # scaled ||r|| at the last replacement
+    replacements = 0
+    steps = 0
+
+    def replace():
+        """X += s Y; the true residual into the float32 sequence (p kept)."""
+        nonlocal true_rel
+        st = nat.stream()
+        nat.call("wpc_refine_update", n, k, nat.ptr(Xf), nat.ptr(Y), nat.code(torch.float32),
+                 nat.ptr(scale), full, None, st)
+        Y.zero_()
+        norms = res.run(Xf, Bf)
+        nat.call("wpc_refine_prepare", n, k, nat.ptr(res.r), nat.ptr(scale), full,
+                 nat.ptr(work.r), nat.code(torch.float32), st)
+        work._apply_M(work.r, work.z)
+        work._dot(work.r, work.z, work._field(0))
+        work._dot(work.r, work.r, work._field(1))
+        true_rel = norms.cpu().numpy()[:k] ** 0.5 / np.where(bn > 0, bn, 1.0)
+
+    while steps < maxiter:
+        if (true_rel <= tol).all():
+            break
+        chunk = min(check_every, maxiter - steps)
+        work.run(Y, chunk, use_cuda_graph and chunk == check_every and steps > 0)
+        steps += chunk
+        h = work.host_state()
+        rr = h[1].numpy() ** 0.5
+        moving = true_rel > tol
+        met = rr <= target
+        if ((rr <= delta * last) | met | ~moving).all():
+            replace()
+            replacements += 1
+            last = work.host_state()[1].numpy() ** 0.5
+    if replacements == 0 or steps >= maxiter:
+        replace()
+        replacements += 1
+    h = work.host_state()
+    return dict(iterations=[int(v) for v in h[4]], replacements=replacements,
+                relative_residual=[float(v) for v in true_rel],
+                converged=bool((true_rel <= tol).all()), work=work)
+
+
 GMRES_MAX_RESTART = 127  # basis rows the Gram kernel takes (128 with the new vector)
 
 

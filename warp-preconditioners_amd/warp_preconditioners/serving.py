@@ -112,7 +112,7 @@ class ServingConfig:
     budget_bytes: float = 200e9
     streams: int = 4
     max_active: int = 64
-    width: int = 8
+    width: object = 8  # FSAI width: an int, or {kind: width}
     tol: float = 1e-8
     precision: dict = field(default_factory=lambda: {"laplacian": "mixed", "biharmonic": "f64"})
     factor_dtype: dict = field(default_factory=lambda: {"mixed": torch.bfloat16,
@@ -200,7 +200,8 @@ class System:
             self.topology = template.topology
             self.provenance = "refit"
         else:
-            self.P = FSAI(source, max_row_size=cfg.width, factor_dtype=cfg.factor_dtype[self.policy])
+            width = cfg.width[kind] if isinstance(cfg.width, dict) else cfg.width
+            self.P = FSAI(source, max_row_size=width, factor_dtype=cfg.factor_dtype[self.policy])
             self.topology = (source.offsets, source.columns)
             self.provenance = "built"
         self.P.release_source()
