@@ -142,15 +142,33 @@ class Solver:
             self.work = out["work"]
             self.iterations = out["iterations"]
             return
-        # mixed: one float32 CG sequence (float32 operator copy, vectors and
-        # preconditioner apply) whose residual is replaced by the compensated float64
-        # true residual at intervals (reliable updates); stops on the true residual
-        out = linear.solve_reliable(self.op64, self.B, self.X, self.P, self.op32, tol=tol,
-                                    maxiter=maxiter, check_every=check_every,
-                                    work=self.inner_work)
-        self.inner_work = out["work"]
-        self.iterations = out["iterations"]
-        self.replacements = out["replacements"]
+        # mixed: float32 Krylov on the residual, float64 true residual and update.
+        # Each refinement step restarts the inner CG: a float32 copy of the factored
+        # operator is only ~1e-3 accurate on smooth fields (the K M^-1 K x terms
+        # cancel), so one float32 Krylov sequence kept across residual replacements
+        # (linear.solve_reliable) stalls at a true relative residual of 2e-3..5e-3.
+        bnorm = torch.linalg.vector_norm(self.B, dim=0)
+        R = self.B.clone()
+        its = [0] * self.k
+        C = torch.zeros(self.B.shape, dtype=torch.float32, device=self.B.device)
+        for outer in range(30):
+            self.op64.apply_batch(self.X, self.B, R, -1.0, 1.0, self.k)
+            rn = torch.linalg.vector_norm(R, dim=0)
+            if bool((rn <= tol * bnorm).all()):
+                break
+            scale = torch.where(rn > 0, rn, torch.ones_like(rn))
+            R32 = (R / scale).float().contiguous()
+            C.zero_()
+            out = linear.solve_batched("cg", self.op32, R32, C, M=self.P, tol=self.inner_tol,
+                                       maxiter=maxiter, check_every=check_every,
+                                       work=self.inner_work)
+            self.inner_work = out["work"]
+            its = [a + b for a, b in zip(its, out["iterations"])]
+            active = (rn > tol * bnorm).double()
+            self.X += C.double() * (scale * active)
+        self.iterations = its
+
+    inner_tol = 1e-3
 
 
 def true_residual(s, X=None):

@@ -290,22 +290,51 @@ def test_refit_clone_equals_update(factor):
         refit_clone(template, assembly.squared_operator(K3, m3, 1.0))
 
 
+def _ld_matvec(K, x):
+    """K x in extended precision (numpy longdouble: 64-bit significand on x86-64)."""
+    prod = K.data.astype(np.longdouble) * x[K.indices]
+    out = np.zeros(K.shape[0], dtype=np.longdouble)
+    nz = np.diff(K.indptr) > 0
+    out[nz] = np.add.reduceat(prod, K.indptr[:-1][nz])
+    return out
+
+
 def _reference_solution(req):
-    """Host float64 direct solve of one request (the device-assembled K and M, the
-    same right-hand sides)."""
+    """Host direct solve of one request (the device-assembled K and M, the same
+    right-hand sides), refined with residuals of the served operator itself
+    (K + alpha M, or the factored alpha M + K M^-1 K) in extended precision: a
+    plain float64 direct solve of the biharmonic Q is only ~kappa * eps accurate
+    (a few 1e-9 at side 28), coarser than the certificates it must check."""
     from warp_preconditioners import workloads
 
     _, _, Kd, md = _mesh(req.mesh.side, req.mesh.seed)
     K, m = _csr(Kd), host(md)
+    K.sort_indices()
+    a = req.alpha
     if req.kind == "laplacian":
-        A = (K + req.alpha * ss.diags(m)).tocsc()
+        A = (K + a * ss.diags(m)).tocsc()
     else:
-        A = workloads.squared_operator(K, m, req.alpha).tocsc()
+        A = workloads.squared_operator(K, m, a).tocsc()
+    ml = m.astype(np.longdouble)
+
+    def apply_ld(x):
+        x = x.astype(np.longdouble)
+        if req.kind == "laplacian":
+            return _ld_matvec(K, x) + a * ml * x
+        return a * ml * x + _ld_matvec(K, _ld_matvec(K, x) / ml)
+
     gen = torch.Generator(device=dev())
     gen.manual_seed(req.rhs_seed)
     B = host(torch.randn((len(m), req.nrhs), generator=gen, dtype=torch.float64, device=dev()))
     lu = sla.splu(A)
-    return np.column_stack([lu.solve(B[:, c]) for c in range(req.nrhs)]), m
+    out = []
+    for c in range(req.nrhs):
+        b = B[:, c].astype(np.longdouble)
+        x = lu.solve(B[:, c]).astype(np.longdouble)
+        for _ in range(3):
+            x = x + lu.solve((b - apply_ld(x)).astype(np.float64))
+        out.append(x.astype(np.float64))
+    return np.column_stack(out), m
 
 
 def _field_error(x, xs, m):

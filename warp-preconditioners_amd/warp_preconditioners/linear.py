@@ -490,7 +490,7 @@ def solve_batched(method, A, B, X, M=None, tol=1.0e-8, atol=0.0, maxiter=10000, 
 
 
 def solve_reliable(A, B, X, M, inner, tol=1.0e-8, delta=0.1, maxiter=40000, check_every=25,
-                   use_cuda_graph=True, work=None):
+                   use_cuda_graph=True, work=None, replace_every=0):
     """Mixed-precision CG with reliable updates (Sleijpen & van der Vorst, 1996): one
     float32 Krylov sequence on the operator `inner` (float32 vectors) with
     preconditioner M, whose recursive residual is replaced by the float64 true
@@ -501,11 +501,13 @@ def solve_reliable(A, B, X, M, inner, tol=1.0e-8, delta=0.1, maxiter=40000, chec
     scale for the whole solve) and accumulates its solution increment in float32
     Y; every `check_every` steps the recursive ||r|| is read, and when it has
     fallen below `delta` times its value at the last replacement for every moving
-    right-hand side (or meets the target), X += s Y in float64, Y = 0, and the
+    right-hand side (or meets the target), or `replace_every` chunks (> 0) have
+    passed since the last replacement, X += s Y in float64, Y = 0, and the
     float32 residual, its preconditioned residual and the r.z / r.r state are
     replaced from the compensated float64 residual -- the search direction p is
     kept, so the Krylov sequence continues (no restart).  A right-hand side
-    finishes when its true relative residual ||b - A x|| / ||b|| <= tol.
+    finishes when its true relative residual ||b - A x|| / ||b|| <= tol (norms in
+    M^-1 for the factored operator, whose mass M it is; 2-norms otherwise).
 
     Returns dict(iterations, replacements, relative_residual, converged, work).
     """
@@ -520,7 +522,7 @@ def solve_reliable(A, B, X, M, inner, tol=1.0e-8, delta=0.1, maxiter=40000, chec
     if not (B.is_contiguous() and X.is_contiguous()):
         raise ValueError("B and X must be contiguous")
     dev = B.device
-    res = TrueResidual(A, torch.ones(n, dtype=torch.float64, device=dev), k)
+    res = TrueResidual(A, None, k)  # (norms in M^-1 / M for a factored operator)
     Bf, Xf = B.reshape(-1), X.reshape(-1)
     bn = res.run(torch.zeros_like(Xf), Bf).cpu().numpy()[:k] ** 0.5
     res.run(Xf, Bf)
@@ -540,9 +542,10 @@ def solve_reliable(A, B, X, M, inner, tol=1.0e-8, delta=0.1, maxiter=40000, chec
     work._field(2).copy_(torch.tensor(target ** 2, dtype=torch.float64, device=dev))
     true_rel = np.array([res_v ** 0.5 for res_v in res.norms.cpu().numpy()[:k]]) / np.where(
         bn > 0, bn, 1.0)
-    last = np.ones(k)  # scaled ||r|| at the last replacement
+    last = work.host_state()[1].numpy() ** 0.5  # recursive ||r|| at the last replacement
     replacements = 0
     steps = 0
+    since = 0  # chunks since the last replacement
 
     def replace():
         """X += s Y; the true residual into the float32 sequence (p kept)."""
@@ -569,9 +572,11 @@ def solve_reliable(A, B, X, M, inner, tol=1.0e-8, delta=0.1, maxiter=40000, chec
         rr = h[1].numpy() ** 0.5
         moving = true_rel > tol
         met = rr <= target
-        if ((rr <= delta * last) | met | ~moving).all():
+        since += 1
+        if ((rr <= delta * last) | met | ~moving).all() or (replace_every and since >= replace_every):
             replace()
             replacements += 1
+            since = 0
             last = work.host_state()[1].numpy() ** 0.5
     if replacements == 0 or steps >= maxiter:
         replace()
@@ -725,8 +730,7 @@ def solve_refined(method, A, B, X, M=None, inner=None, tol=1.0e-8, inner_tol=1.0
     inner = A if inner is None else inner
     vdt = getattr(inner, "scalar_type", torch.float64)
     dev = B2.device
-    ones = torch.ones(n, dtype=torch.float64, device=dev)
-    res = TrueResidual(A, ones, k)
+    res = TrueResidual(A, None, k)
     zero = torch.zeros_like(X2)
     bn = torch.sqrt(res.run(zero.reshape(-1), B2.reshape(-1))[:k]).cpu().numpy()
     norms = res.run(X2.reshape(-1), B2.reshape(-1)).cpu().numpy()

@@ -32,10 +32,17 @@ class TrueResidual:
     """Compensated r = b - A x and the norms ||r||^2_{M^-1}, ||x||^2_M on nvec
     interleaved float64 right-hand sides (all buffers preallocated)."""
 
-    def __init__(self, op, mass: torch.Tensor, nvec: int):
+    def __init__(self, op, mass: torch.Tensor | None, nvec: int):
+        """mass: the M of the norms; the factored operator's own mass when None (its
+        alpha*M term is the operator's, so any other mass is refused), ones (plain
+        2-norms) for an assembled operator when None."""
         if isinstance(op, SquaredLaplacianOperator):
             if not op.identity or op.L.scalar_type != torch.float64:
                 raise ValueError("TrueResidual needs an all-free float64 factored operator")
+            if mass is None:
+                mass = op.mass
+            elif mass.data_ptr() != op.mass.data_ptr() and not torch.equal(mass, op.mass):
+                raise ValueError("a factored operator's residual uses the operator's own mass")
             self.matrix, self.inverse, self.alpha = op.L, op.inverse_mass, op.mass_weight
             self.factored = True
         elif isinstance(op, SparseOperator):
@@ -57,6 +64,8 @@ class TrueResidual:
             raise ValueError("nvec must be in [1, 8]")
         n = self.matrix.nrow
         dev = self.matrix.device
+        if mass is None:
+            mass = torch.ones(n, dtype=torch.float64, device=dev)
         self.n, self.nvec, self.mass = n, nvec, mass
         size = n * nvec
         self.r = torch.empty(size, dtype=torch.float64, device=dev)
