@@ -418,3 +418,17 @@ def test_serving_configuration_sides_certified_and_mix_invariant():
             assert alone.iterations == res.iterations
         else:
             assert np.all(_field_error(res.x, alone.x, m) <= 2e-8)
+
+
+def test_serving_deadline_stops_and_reports_unfinished():
+    # A stream cut by the wall-clock guard returns None for the requests it did not
+    # finish and flags the engine as timed out (the bench then reports no rate).
+    from warp_preconditioners.serving import MeshSpec, Request, ServingConfig, ServingEngine
+
+    reqs = [Request(i, 0.0, MeshSpec(10, 300 + i), "laplacian", 2, 40 + i) for i in range(3)]
+    eng = ServingEngine(ServingConfig(streams=2, max_active=4, chunk=10))
+    out = eng.serve(reqs, realtime=False, deadline=0.0)
+    assert eng.timed_out and out == [None, None, None]
+    eng = ServingEngine(ServingConfig(streams=2, max_active=4, chunk=10))
+    out = eng.serve(reqs, realtime=False, deadline=600.0)
+    assert not eng.timed_out and all(r is not None and r.converged for r in out)

@@ -737,14 +737,17 @@ class ServingEngine:
         cached = (req.kind, req.mesh, req.alpha) in self.systems
         return estimated_seconds(req, cached) - self.AGING * max(0.0, now - req.arrival)
 
-    def serve(self, requests, realtime=True, on_result=None):
+    def serve(self, requests, realtime=True, on_result=None, deadline=None):
         """Run the request list to completion; returns Results in request order.
         With ``realtime`` each request becomes visible at its arrival time; otherwise
         all are available at once.  Arrived requests are admitted smallest estimated
         work first (aged by waiting time) while slots and budget allow; chunk launches
         go to the jobs with the least estimated remaining work first, up to FILL_CAP
-        of GPU fill, so small requests run side by side and a large one alone."""
+        of GPU fill, so small requests run side by side and a large one alone.
+        With ``deadline`` (seconds on the engine clock) the loop stops there: the
+        unfinished requests' entries are None and ``self.timed_out`` is set."""
         self.t0 = time.perf_counter()
+        self.timed_out = False
         if self.cfg.split_rhs:
             return self._serve_split(requests, realtime, on_result)
         queue = deque(sorted(requests, key=lambda r: r.arrival))
@@ -752,6 +755,9 @@ class ServingEngine:
         try:
             while queue or ready or active or self.building:
                 now = self.clock()
+                if deadline is not None and now > deadline:
+                    self.timed_out = True
+                    break
                 while queue and (not realtime or queue[0].arrival <= now):
                     ready.append(queue.popleft())
                 self._collect_builds(active)
@@ -798,7 +804,9 @@ class ServingEngine:
         finally:
             for fut, _ in self.building.values():
                 fut.cancel()
-        return [results[r.rid] for r in requests]
+        if self.timed_out:
+            torch.cuda.synchronize(self.device)  # in-flight chunks end before teardown
+        return [results.get(r.rid) for r in requests]
 
     def _launch_batch(self, key, jobs):
         """One chunk of CG steps for every job in `jobs` (same group key) with one
