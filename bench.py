@@ -279,14 +279,31 @@ def kernel_times(s, reps=30):
     kernels = {
         "fsai_G_pass": lambda: pg.run(X, None, T, 1.0, 0.0, k, s=P.row_scale2),
         "fsai_GT_pass_dot": lambda: pgt.run(T, None, Y, 1.0, 0.0, k, w=X),
-        "sqlap_forward": lambda: pl.run(X, None, T, 1.0, 0.0, k, s=op.inverse_mass),
-        "sqlap_backward_dot": lambda: pl.run(T, None, Y, 1.0, 0.0, k, d=op._wm, u=X, w=X),
     }
-    # the vector steps exactly as the solver issues them (folded in place when the
-    # partial counts allow, linear.KrylovWork.step)
+    # the launches exactly as the solver issues them (linear.KrylovWork.step): the
+    # update folded into the operator's second pass where the operator offers it,
+    # else the separate update step (the folds in place when the partial counts allow)
+    from warp_preconditioners import linear
+
+    fused = (linear.FUSE_OPERATOR_UPDATE and op.cg_update_parts(k, dt) is not None
+             and max(pk, pz) * k <= FOLD_IN_PLACE)
+    prr = pk if fused else npart  # r.r partials the direction step folds
+    if fused:
+        rr = torch.zeros(pk * k, dtype=torch.float64, device="cuda:0")
+        kernels["sqlap_forward_energy"] = lambda: op.cg_update(X, Y, k, state, rr, passes=1)
+        kernels["sqlap_backward_update"] = lambda: op.cg_update(X, Y, k, state, rr, passes=2)
+        kernels["cg_direction_fold"] = lambda: nat.call(
+            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), nat.ptr(X), code, pz,
+            nat.ptr(pgt.partials(k)), pk, nat.ptr(rr), nat.ptr(state), cur())
+    else:
+        kernels["sqlap_forward"] = lambda: pl.run(X, None, T, 1.0, 0.0, k, s=op.inverse_mass)
+        kernels["sqlap_backward_dot"] = lambda: pl.run(T, None, Y, 1.0, 0.0, k, d=op._wm, u=X,
+                                                       w=X)
     fold_a = pk * k <= FOLD_IN_PLACE
     fold_b = fold_a and max(pz, npart) * k <= FOLD_IN_PLACE
-    if fold_a:
+    if fused:
+        pass
+    elif fold_a:
         # (with both steps folded, x advances in the direction step)
         kernels["cg_update_fold"] = lambda: nat.call(
             "wpc_cg_update_fold", n, k, None if fold_b else nat.ptr(X), nat.ptr(Y), nat.ptr(T),
@@ -297,7 +314,9 @@ def kernel_times(s, reps=30):
         kernels["cg_update"] = lambda: nat.call(
             "wpc_cg_update", n, k, nat.ptr(X), nat.ptr(Y), nat.ptr(T), nat.ptr(T), code,
             nat.ptr(state), nat.ptr(parts), cur())
-    if fold_b:
+    if fused:
+        pass
+    elif fold_b:
         kernels["cg_direction_fold"] = lambda: nat.call(
             "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), nat.ptr(X), code, pz,
             nat.ptr(pgt.partials(k)), npart, nat.ptr(parts), nat.ptr(state), cur())
@@ -350,12 +369,18 @@ def kernel_times(s, reps=30):
         "fsai_GT_pass_dot": nnzG * (vb_g + 4) + idx + 3 * n * k * xb,
         "sqlap_forward": nnzK * (vb_k + 4) + idx + 2 * n * k * xb + n * vb_k,
         "sqlap_backward_dot": nnzK * (vb_k + 4) + idx + 3 * n * k * xb + n * vb_k,
+        # p gathered and t written, 1/mass (and w * mass) read
+        "sqlap_forward_energy": nnzK * (vb_k + 4) + idx + 2 * n * k * xb
+        + n * vb_k * (2 if op.mass_weight != 0.0 else 1),
+        # t gathered, p read, r read and written, w * mass read, the p.Ap partials
+        "sqlap_backward_update": nnzK * (vb_k + 4) + idx + 4 * n * k * xb
+        + (n * vb_k if op.mass_weight != 0.0 else 0) + 8 * pk * k,
         "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
         "cg_update_fold": (3 if fold_b else 6) * n * k * xb + 8 * pk * k,
         "cg_beta": 8 * (pz + npart) * k,
         "cg_direction": 3 * n * k * xb,
-        "cg_direction_fold": 5 * n * k * xb + 8 * (pz + npart) * k,
+        "cg_direction_fold": 5 * n * k * xb + 8 * (pz + prr) * k,
     }
     return out, {name: bytes_[name] for name in out}, tuned
 

@@ -499,6 +499,30 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int 
                           int nparts_rz, const double* part_rz, int nparts_rr,
                           const double* part_rr, double* state, void* stream);
 
+/* The factored squared-Laplacian operator's two sliced passes inside fused CG, with
+ * the update step folded into the second pass: replaces, for an operator with the
+ * identity free map, the product pair (squared_laplacian.py:37-65 forward,
+ * :67-98 backward, applied by Warp's cg as A p) followed by wpc_cg_update_fold
+ * with x = NULL.  Pass 1: t = inverse_mass .* (L p) and per-workgroup partials of
+ * the curvature p.Ap = sum_i (L p)_i t_i + W_i p_i^2 (W = weighted_mass = w * mass,
+ * NULL for w = 0; L symmetric).  Pass 2: Ap = L t + W p row by row (not stored),
+ * ALPHA = RZ / (fold of the pass-1 partials) for slots with RR > TARGET (else 0),
+ * r -= ALPHA Ap, per-workgroup partials of r.r; workgroup 0 writes RZ_OLD, DENOM,
+ * ACTIVE, ALPHA as wpc_cg_update_fold does.  The direction step then folds
+ * ceil(n / 256) r.r partials from part_rr and advances x (wpc_cg_direction_fold
+ * with x).  The sliced copy of L (rows in natural or given order) as for
+ * wpc_sell_stream; values, inverse_mass, weighted_mass and vectors all F64 or all
+ * F32; nvec 1..8; part_energy and part_rr hold ceil(n / 256) * nvec doubles each
+ * (folded in place by every workgroup: meant for ceil(n / 256) * nvec <= 4096).
+ * The curvature differs from p . (A p) by rounding only.  passes = 3 runs both;
+ * 1 or 2 runs one of them (for timing each kernel alone). */
+int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
+                        const int* slice_base, const int* rows, const void* sell_columns,
+                        int col16, const void* sell_values, int val_dtype,
+                        const void* inverse_mass, const void* weighted_mass, const void* p,
+                        void* t, void* r, int vec_dtype, int nvec, double* part_energy,
+                        double* part_rr, double* state, int passes, void* stream);
+
 /* ------------------------------------------------------------------ grouped CG steps
  * No reference counterpart: a serving extension that advances the fused CG of
  * several systems (uneven sizes and right-hand-side counts, same vector format

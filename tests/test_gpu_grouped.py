@@ -64,6 +64,7 @@ def test_grouped_steps_equal_own_steps(case, nvecs):
         B = _vec(rng.normal(size=(n, nvec)), dtype)
         for works, xs in ((own, xs_own), (mine, xs_mine)):
             w = KrylovWork("cg", A, P, n, nvec, dtype, dev())
+            w.fuse_update = False  # the grouped kernels keep the separate update step
             X = torch.zeros_like(B)
             w.start(B.reshape(-1), X.reshape(-1), 1e-9 if dtype == torch.float64 else 1e-5, 0.0)
             works.append(w)

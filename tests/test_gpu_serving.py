@@ -361,7 +361,9 @@ def test_serving_results_certified_and_mix_invariant(group):
         err = _field_error(res.x, xs, m)
         assert np.all(err <= 1e-8), (req, err)
         assert np.all(err <= np.array(res.field_bound) * 1.01 + 1e-10)
-        alone = ServingEngine(ServingConfig(streams=1, max_active=1, chunk=10)).serve(
+        # alone under the same configuration (grouping also decides whether the
+        # update step is folded into the operator's pass)
+        alone = ServingEngine(ServingConfig(streams=1, max_active=1, chunk=10, group=group)).serve(
             [req], realtime=False)[0]
         if res.factor in ("built", "cached"):
             np.testing.assert_array_equal(alone.x, res.x)

@@ -257,31 +257,6 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
     }
 }
 
-// State fields F0, F1 (, F2) of all nv vectors, held by the lanes of every
-// wavefront (lane k * nv + v holds field k of vector v) and loaded with ordinary
-// vector loads, so that they travel with a step's other loads (uniform reads
-// would become scalar loads issued only after the rows have arrived).
-template <int F0, int F1, int F2 = -1>
-struct StateLanes {
-    static constexpr int kFields = F2 < 0 ? 2 : 3;
-    double val;
-
-    __device__ __forceinline__ void load(const double* state, int nv) {
-        const int lane = threadIdx.x & (kWave - 1);
-        const int k = lane / nv, v = lane - k * nv;
-        const int f = k == 0 ? F0 : (k == 1 ? F1 : F2);
-        val = k < kFields ? state[f * nv + v] : 0.0;
-    }
-
-    __device__ __forceinline__ double get(int k, int nv, int v) const {
-        const int lane = k * nv + v;
-        const long long bits = __double_as_longlong(val);
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, lane);
-        const long long hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
-        return __longlong_as_double((hi << 32) | lo);
-    }
-};
-
 // The same two steps with the scalar folds folded in, for partial counts small
 // enough that every workgroup can afford to fold them: each workgroup folds in the
 // same fixed order, so all agree bit for bit, and workgroup 0 records the scalars.
@@ -330,6 +305,7 @@ __device__ __forceinline__ void cg_update_fold_body(
             }
         }
     }
+    __builtin_amdgcn_sched_barrier(0);  // (as in the direction step below)
     if constexpr (NV > 0)
         fold_a.finish(part_denom, P, sums);
     else
@@ -443,6 +419,9 @@ __device__ __forceinline__ void cg_direction_fold_body(
             }
         }
     }
+    // every load above goes out before the first wait: the scheduler would start
+    // the folds' adds early (to free registers) and wait for each row in turn
+    __builtin_amdgcn_sched_barrier(0);
     T alpha[NVM];
     if constexpr (XD) {
         // x += ALPHA p needs no fold: its stores go out while the folds run

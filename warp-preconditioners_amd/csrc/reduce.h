@@ -196,28 +196,33 @@ struct FoldRegs {
     static constexpr int K = (kFoldStaged / NV + NT - 1) / NT;
     double val[K][NV];
 
+    // Branch-free loads through a buffer descriptor whose range ends at row P: a
+    // row past P reads 0 without a memory access.  (With the loads under a per-row
+    // condition the compiler moved the first row's add into its branch and waited
+    // there, so the remaining loads, and every load issued after them, went out
+    // one memory round trip late.)
     __device__ __forceinline__ void load(const double* __restrict__ partials, int P) {
+        const __amdgpu_buffer_rsrc_t pr = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)partials, (short)0, P > 0 ? P * NV * (int)sizeof(double) : 0, kBufferWord3);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int b = threadIdx.x + k * NT;
-            if (b < P) {
+            const unsigned row = threadIdx.x + k * NT;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) val[k][v] = partials[(long)b * NV + v];
-            }
+            for (int v = 0; v < NV; ++v)
+                val[k][v] = buffer_ld<double>(pr, (row * NV + v) * (unsigned)sizeof(double));
         }
     }
 
     // Per-thread sums acc[0..NV) of the staged rows (and of any rows beyond the
-    // staged capacity), in fold_all's order.
+    // staged capacity), in fold_all's order.  A row past P was loaded as +0, which
+    // leaves the sum unchanged: it starts at +0 and so is never -0.
     __device__ __forceinline__ void sums(const double* __restrict__ partials, int P, double* acc) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (threadIdx.x + k * NT < P) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v] += val[k][v];
-            }
+            for (int v = 0; v < NV; ++v) acc[v] += val[k][v];
         }
         for (int b = threadIdx.x + K * NT; b < P; b += NT)
 #pragma unroll
@@ -229,6 +234,31 @@ struct FoldRegs {
         double acc[8];
         sums(partials, P, acc);
         fold_reduce<NT>(acc, NV, out_s);
+    }
+};
+
+// State fields F0, F1 (, F2) of all nv vectors, held by the lanes of every
+// wavefront (lane k * nv + v holds field k of vector v) and loaded with ordinary
+// vector loads, so that they travel with a step's other loads (uniform reads
+// would become scalar loads issued only after the rows have arrived).
+template <int F0, int F1, int F2 = -1>
+struct StateLanes {
+    static constexpr int kFields = F2 < 0 ? 2 : 3;
+    double val;
+
+    __device__ __forceinline__ void load(const double* state, int nv) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int k = lane / nv, v = lane - k * nv;
+        const int f = k == 0 ? F0 : (k == 1 ? F1 : F2);
+        val = k < kFields ? state[f * nv + v] : 0.0;
+    }
+
+    __device__ __forceinline__ double get(int k, int nv, int v) const {
+        const int lane = k * nv + v;
+        const long long bits = __double_as_longlong(val);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, lane);
+        const long long hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+        return __longlong_as_double((hi << 32) | lo);
     }
 };
 

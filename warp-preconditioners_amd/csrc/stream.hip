@@ -70,6 +70,36 @@ int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int
     return stream_dispatch(nvec, a);
 }
 
+int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
+                        const int* slice_base, const int* rows, const void* sell_columns,
+                        int col16, const void* sell_values, int val_dtype,
+                        const void* inverse_mass, const void* weighted_mass, const void* p,
+                        void* t, void* r, int vec_dtype, int nvec, double* part_energy,
+                        double* part_rr, double* state, int passes, void* stream) {
+    if (passes < 1 || passes > 3) return WPC_EARG;
+    if (n <= 0) return WPC_OK;
+    if (slice_ptr == nullptr || slice_len == nullptr || slice_base == nullptr ||
+        sell_columns == nullptr || sell_values == nullptr || inverse_mass == nullptr ||
+        p == nullptr || t == nullptr || r == nullptr || part_energy == nullptr ||
+        part_rr == nullptr || state == nullptr)
+        return WPC_EARG;
+    if (p == t || p == r || t == r) return WPC_EARG;
+    const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
+                        sell_values, val_dtype, inverse_mass, weighted_mass, p, t, r, vec_dtype,
+                        part_energy, part_rr, state, passes, as_stream(stream)};
+    switch (nvec) {
+        case 1: return sqlap_cg_entry_1(a);
+        case 2: return sqlap_cg_entry_2(a);
+        case 3: return sqlap_cg_entry_3(a);
+        case 4: return sqlap_cg_entry_4(a);
+        case 5: return sqlap_cg_entry_5(a);
+        case 6: return sqlap_cg_entry_6(a);
+        case 7: return sqlap_cg_entry_7(a);
+        case 8: return sqlap_cg_entry_8(a);
+        default: return WPC_EARG;
+    }
+}
+
 int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int val_dtype,
                             int s_dtype, int has_d, int dot, int vec_dtype, double alpha,
                             void* stream) {

@@ -105,6 +105,40 @@ struct GroupedArgs {
 // Batched products, each task with its own vector count (stream_grouped.hip).
 int stream_grouped_entry(const GroupedArgs& a);
 
+// The factored operator's two sliced passes inside CG with the residual update
+// fused into the second pass (wpc_sqlap_cg_update).
+struct SqlapCgArgs {
+    int n;
+    const int* slice_ptr;
+    const int* slice_len;
+    const int* slice_base;
+    const int* rows;
+    bool col16;
+    const void* col;
+    const void* val;
+    int val_dtype;
+    const void* s;   // 1/mass, vector type
+    const void* wm;  // w * mass or NULL
+    const void* p;
+    void* t;
+    void* r;
+    int vec_dtype;
+    double* part_e;
+    double* part_rr;
+    double* state;
+    int passes;  // bit 0: the first pass, bit 1: the second
+    hipStream_t st;
+};
+
+int sqlap_cg_entry_1(const SqlapCgArgs& a);
+int sqlap_cg_entry_2(const SqlapCgArgs& a);
+int sqlap_cg_entry_3(const SqlapCgArgs& a);
+int sqlap_cg_entry_4(const SqlapCgArgs& a);
+int sqlap_cg_entry_5(const SqlapCgArgs& a);
+int sqlap_cg_entry_6(const SqlapCgArgs& a);
+int sqlap_cg_entry_7(const SqlapCgArgs& a);
+int sqlap_cg_entry_8(const SqlapCgArgs& a);
+
 }  // namespace wpc
 
 #ifdef WPC_STREAM_NV
@@ -576,6 +610,177 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
                                            partials, red);
 }
 
+// The factored operator A = L M^-1 L + W (W = w M, diagonal; L symmetric) inside
+// CG, with the residual update fused into its second pass.  With t = M^-1 L p the
+// curvature is
+//     p.Ap = sum_i (L p)_i t_i + W_i p_i^2,
+// so the first pass leaves partials of it and the second pass, which folds them
+// (fixed order, every workgroup alike, as the update step folds p.Ap), knows
+// ALPHA before its epilogue: it forms Ap_i = (L t)_i + W_i p_i exactly as the plain
+// second pass does and applies r -= ALPHA Ap instead of storing Ap, which no later
+// step reads.  The update launch and Ap's round trip through HBM leave the
+// iteration.  State fields as in the update step: reads RZ, RR, TARGET; workgroup 0
+// writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA; x is left to the direction step.
+
+// Wavefront slice header and gather descriptor of the sliced mapping (as sell_body).
+struct SellHead {
+    int row, len;
+    bool live;
+    long base;
+};
+
+template <class TX, int NV>
+__device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restrict__ slice_ptr,
+                                              const int* __restrict__ slice_len,
+                                              const int* __restrict__ slice_base,
+                                              const int* __restrict__ rows, bool col16,
+                                              const TX* x, __amdgpu_buffer_rsrc_t* xr) {
+    const int idx = blk * kStreamRows + threadIdx.x;
+    const int wave0 = __builtin_amdgcn_readfirstlane(idx & ~63);
+    const int slice = min(wave0, n - 1) >> 6;
+    const int sp = slice_ptr[slice];
+    const int sl = slice_len[slice];
+    const int sb = slice_base[slice];
+    const int idc = idx < n ? idx : n - 1;
+    SellHead h;
+    h.live = idx < n;
+    h.row = rows != nullptr ? rows[idc] : idc;
+    h.base = (long)sp * 64 + (idx & 63);
+    h.len = wave0 < n ? sl : 0;
+    const unsigned stride = (unsigned)(NV * sizeof(TX));
+    *xr = __builtin_amdgcn_make_buffer_rsrc((void*)(x + (long)sb * NV), (short)0,
+                                            (int)(col16 ? 0xFFFFu * stride : kRange32),
+                                            kBufferWord3);
+    return h;
+}
+
+// Pass 1: t = s .* (L p) (s = 1/mass), partials of (L p)_i t_i + W_i p_i^2.
+template <class TV, class TX, int NV, bool HW>
+__global__ __launch_bounds__(kStreamRows) void sell_energy_kernel(
+    int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
+    const int* __restrict__ slice_base, const int* __restrict__ rows,
+    const void* __restrict__ col, int col16, const TV* __restrict__ val,
+    const TX* __restrict__ s, const TX* __restrict__ wm, const TX* __restrict__ p,
+    TX* __restrict__ t, double* __restrict__ partials) {
+    using TA = typename AccOf<TX>::type;
+    __shared__ double red[kStreamRows];
+    const int blk = xcd_block();
+    __amdgpu_buffer_rsrc_t xr;
+    const SellHead h = sell_head<TX, NV>(blk, n, slice_ptr, slice_len, slice_base, rows,
+                                         col16 != 0, p, &xr);
+    const unsigned stride = (unsigned)(NV * sizeof(TX));
+    const TA sv = St<TX>::template ld<TA>(s, h.row);
+    TA wv = TA(0), pv[NV];
+    if (HW) {
+        wv = St<TX>::template ld<TA>(wm, h.row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pv[v] = St<TX>::template ld<TA>(p, (long)h.row * NV + v);
+    }
+    TA acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    if (col16)
+        sell_accum<unsigned short, TV, TX, NV>(acc, h.base, h.len, xr, stride,
+                                               (const unsigned short*)col, val, NV);
+    else
+        sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
+    double e[NV];
+    TX out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        // the plain first pass's arithmetic (alpha = 1, beta = 0)
+        St<TX>::template st<TA>(out, v, sv * acc[v]);
+        double ev = (double)acc[v] * (double)out[v];
+        if (HW) ev += (double)wv * ((double)pv[v] * (double)pv[v]);
+        e[v] = h.live ? ev : 0.0;
+        asm volatile("" : "+v"(e[v]));
+    }
+    if (h.live) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) t[(long)h.row * NV + v] = out[v];
+    }
+    block_sum<NV>(e, red);
+    write_partials<NV>(e, partials, blk);
+}
+
+// Pass 2: Ap = L t + W p (not stored); ALPHA from the P pass-1 partials;
+// r -= ALPHA Ap for stepping slots; partials of r.r.
+template <class TV, class TX, int NV, bool HW>
+__global__ __launch_bounds__(kStreamRows) void sell_cg_update_kernel(
+    int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
+    const int* __restrict__ slice_base, const int* __restrict__ rows,
+    const void* __restrict__ col, int col16, const TV* __restrict__ val,
+    const TX* __restrict__ wm, const TX* __restrict__ p, const TX* __restrict__ t,
+    TX* __restrict__ r, int P, const double* __restrict__ part_e, double* state,
+    double* __restrict__ part_rr) {
+    using TA = typename AccOf<TX>::type;
+    __shared__ double red[kStreamRows];
+    __shared__ double sums[8];
+    const int blk = xcd_block();
+    __amdgpu_buffer_rsrc_t xr;
+    const SellHead h = sell_head<TX, NV>(blk, n, slice_ptr, slice_len, slice_base, rows,
+                                         col16 != 0, t, &xr);
+    const unsigned stride = (unsigned)(NV * sizeof(TX));
+    TA wv = TA(0), pv[NV];
+    if (HW) {
+        wv = St<TX>::template ld<TA>(wm, h.row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pv[v] = St<TX>::template ld<TA>(p, (long)h.row * NV + v);
+    }
+    TA acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    if (col16)
+        sell_accum<unsigned short, TV, TX, NV>(acc, h.base, h.len, xr, stride,
+                                               (const unsigned short*)col, val, NV);
+    else
+        sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
+    // After the gathers (their registers are free again): the state fields, the
+    // partials and the residual rows, all in one round trip.
+    StateLanes<WPC_KS_RZ, WPC_KS_RR, WPC_KS_TARGET> st;
+    st.load(state, NV);
+    FoldRegs<kStreamRows, NV> fold;
+    fold.load(part_e, P);
+    TX rv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) rv[v] = r[(long)h.row * NV + v];
+    fold.finish(part_e, P, sums);
+    TX alpha[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double denom = sums[v];
+        const double rz = st.get(0, NV, v);
+        const bool active = st.get(1, NV, v) > st.get(2, NV, v);
+        const double a = active ? rz / denom : 0.0;
+        alpha[v] = (TX)a;
+        if (blk == 0 && threadIdx.x == v) {
+            state[WPC_KS_RZ_OLD * NV + v] = rz;
+            state[WPC_KS_DENOM * NV + v] = denom;
+            state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
+            state[WPC_KS_ALPHA * NV + v] = a;
+        }
+    }
+    double rr[NV];
+    TX rn[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        // Ap as the plain second pass stores it (alpha = 1, beta = 0), then the
+        // update step's arithmetic
+        TA ap = acc[v];
+        if (HW) ap += wv * pv[v];
+        TX apx[1];
+        St<TX>::template st<TA>(apx, 0, ap);
+        rn[v] = alpha[v] != TX(0) ? rv[v] - alpha[v] * apx[0] : rv[v];
+        rr[v] = h.live ? (double)rn[v] * (double)rn[v] : 0.0;
+    }
+    if (h.live) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) r[(long)h.row * NV + v] = rn[v];
+    }
+    block_sum<NV>(rr, red);
+    write_partials<NV>(rr, part_rr, blk);
+}
+
 // The same product on several systems in one launch: workgroup g (after the XCD
 // renumbering) belongs to the task with the largest block0 <= g and runs that
 // task's local block g - block0, so every system computes exactly what its own
@@ -752,6 +957,37 @@ int stream_for_nv(const StreamArgs& a) {
     return WPC_EDTYPE;
 }
 
+template <class TV, class TX, int NV>
+int sqlap_cg_launch(const SqlapCgArgs& a) {
+    const int grid = (int)blocks_for(a.n, kStreamRows);
+#define SC(HW)                                                                                  \
+    do {                                                                                        \
+        if (a.passes & 1)                                                                       \
+        hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW>), dim3(grid), dim3(kStreamRows), \
+                           0, a.st, a.n, a.slice_ptr, a.slice_len, a.slice_base, a.rows, a.col, \
+                           (int)a.col16, (const TV*)a.val, (const TX*)a.s, (const TX*)a.wm,     \
+                           (const TX*)a.p, (TX*)a.t, a.part_e);                                 \
+        if (a.passes & 2)                                                                       \
+        hipLaunchKernelGGL((sell_cg_update_kernel<TV, TX, NV, HW>), dim3(grid),                 \
+                           dim3(kStreamRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
+                           a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
+                           (const TX*)a.wm, (const TX*)a.p, (const TX*)a.t, (TX*)a.r, grid,     \
+                           a.part_e, a.state, a.part_rr);                                       \
+    } while (0)
+    if (a.wm != nullptr) SC(true);
+    else SC(false);
+#undef SC
+    return launch_status();
+}
+
+template <int NV>
+int sqlap_cg_for_nv(const SqlapCgArgs& a) {
+    if (a.n <= 0) return WPC_OK;
+    if (a.vec_dtype == WPC_F64 && a.val_dtype == WPC_F64) return sqlap_cg_launch<double, double, NV>(a);
+    if (a.vec_dtype == WPC_F32 && a.val_dtype == WPC_F32) return sqlap_cg_launch<float, float, NV>(a);
+    return WPC_EDTYPE;
+}
+
 }  // namespace
 
 #define WPC_STREAM_CAT2(a, b) a##b
@@ -759,6 +995,9 @@ int stream_for_nv(const StreamArgs& a) {
 #if WPC_STREAM_NV > 0
 int WPC_STREAM_CAT(stream_entry_, WPC_STREAM_NV)(const StreamArgs& a) {
     return stream_for_nv<WPC_STREAM_NV>(a);
+}
+int WPC_STREAM_CAT(sqlap_cg_entry_, WPC_STREAM_NV)(const SqlapCgArgs& a) {
+    return sqlap_cg_for_nv<WPC_STREAM_NV>(a);
 }
 #elif defined(WPC_GROUPED_F32)
 int stream_grouped_f32(const GroupedArgs& a) { return grouped_for_nv<0, float>(a); }

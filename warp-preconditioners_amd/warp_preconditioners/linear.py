@@ -188,6 +188,10 @@ def preconditioner(A: BsrMatrix, ptype: str = "diag") -> LinearOperator:
 # Largest nparts * nvec whose scalar fold is repeated by every workgroup of the
 # following vector kernel instead of running as a kernel of its own.
 FOLD_IN_PLACE = 4096
+# Fold CG's update step into the operator's last pass where the operator offers it
+# (SquaredLaplacianOperator.cg_update); False keeps the separate update launch.  Read
+# when a KrylovWork is made (its `fuse_update`).
+FUSE_OPERATOR_UPDATE = True
 
 
 class KrylovWork:
@@ -213,6 +217,11 @@ class KrylovWork:
         self.graph_steps = 0
         self._graph_key = None
         self._m_parts = None  # the preconditioner's partial count (fused steps)
+        self._rr_parts = None  # r.r partials of an operator pass with the update folded in
+        # Fold the update into the operator's last pass where the operator offers it.
+        # Off for works that also advance in grouped launches (grouped.py), whose
+        # batched kernels keep the separate update: both then take the same steps.
+        self.fuse_update = FUSE_OPERATOR_UPDATE
         self.use_fused = (self.method == 0 and M is not None and bool(getattr(A, "fused", False))
                           and bool(getattr(M, "fused", False)) and hasattr(A, "apply_batch")
                           and hasattr(M, "apply_batch"))
@@ -259,6 +268,8 @@ class KrylovWork:
         s = nat.stream()
         pd, pr, pz = (nat.ptr(self.parts[k]) for k in range(3))
         st = nat.ptr(self.state)
+        if self.use_fused and self._fused_update_step(x, s, st):
+            return
         if self.use_fused:
             # A p (+p.Ap partials) | alpha | update (+r.r partials) | M r (+r.z partials)
             # | beta | direction
@@ -316,6 +327,35 @@ class KrylovWork:
                      self.code, pz, s)
             nat.call("wpc_krylov_direction", 1, self.n, self.nvec, nat.ptr(self.z),
                      nat.ptr(self.p), nat.ptr(self.az), nat.ptr(self.ap), self.code, st, pz, pr, s)
+
+    def _fused_update_step(self, x, s, st):
+        """One CG step with the update folded into the operator's last pass
+        (operators with `cg_update`: the factored squared Laplacian) -- four launches
+        on the headline path instead of five: A p with the update (+r.r partials) |
+        M r (+r.z partials) | direction (+x).  Taken once the preconditioner's
+        partial count is known (from the first step) and both folds fit in place;
+        returns False to leave the step to the general path."""
+        if not self.fuse_update or self._m_parts is None:
+            return False
+        parts_of = getattr(self.A, "cg_update_parts", None)
+        if parts_of is None:
+            return False
+        count = parts_of(self.nvec, self.dtype)
+        if count is None or max(count, self._m_parts) * self.nvec > FOLD_IN_PLACE:
+            return False
+        need = count * self.nvec
+        if self._rr_parts is None or self._rr_parts.numel() < need:
+            self._rr_parts = torch.zeros(need, dtype=torch.float64, device=self.r.device)
+            nat.bump_generation()
+        self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts)
+        part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
+                                          dot_w=self.r)
+        self._m_parts = nparts
+        # (exact for any partial count; only slower past FOLD_IN_PLACE)
+        nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
+                 nat.ptr(x), self.code, nparts, nat.ptr(part), count, nat.ptr(self._rr_parts), st,
+                 s)
+        return True
 
     def run(self, x, steps, use_graph):
         if not use_graph or steps <= 0:

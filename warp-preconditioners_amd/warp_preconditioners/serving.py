@@ -274,6 +274,9 @@ class Slot:
             self.c64 = torch.empty(size, dtype=torch.float64, device=dev)
             self.norms_w = torch.empty(2 * nvec, dtype=torch.float64, device=dev)
             self.work = KrylovWork("cg", system.inner, system.P, n, nvec, vdt, dev)
+            # grouped launches keep the separate update step; a work that may join
+            # them takes the same steps alone
+            self.work.fuse_update = not system.group
             self.x_iter = self.C
             self.gkey = grouped.group_key(self.work) if system.group else None
             gs = grouped.scratch(self.work) if self.gkey is not None else None

@@ -220,9 +220,12 @@ class StreamPlan:
     DESCRIPTOR_RANGE = 1 << 31
 
     def _sell_fits(self, x, nvec):
+        return self.sliced_fits(nvec, x.element_size())
+
+    def sliced_fits(self, nvec, elem_bytes):
         if self._sell is not None and self._sell[6]:
             return True  # 16-bit offsets: the range is per slice
-        return self.A.ncol * nvec * x.element_size() <= self.DESCRIPTOR_RANGE
+        return self.A.ncol * nvec * elem_bytes <= self.DESCRIPTOR_RANGE
 
     def refresh_values(self):
         """Re-gather the sliced copy's values after A.values changed in place."""

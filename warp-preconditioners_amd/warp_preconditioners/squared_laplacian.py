@@ -117,6 +117,40 @@ class SquaredLaplacianOperator(LinearOperator):
                  nat.ptr(z), nvec, float(alpha), float(beta), int(boundary), self.row_lanes,
                  nat.stream())
 
+    def cg_update_parts(self, nvec, dtype):
+        """Partial count of `cg_update` for `nvec` vectors of `dtype`, or None when
+        the fused pair does not apply (a free map other than the identity, vectors
+        of another type than L, or a row mapping other than the sliced copy)."""
+        if not self.identity or dtype != self.L.scalar_type:
+            return None
+        if dtype not in (torch.float32, torch.float64) or not 1 <= nvec <= 8:
+            return None
+        plan = self._stream()
+        if plan.mode(nvec, dtype) != StreamPlan.SLICED:
+            return None
+        if not plan.sliced_fits(nvec, torch.finfo(dtype).bits // 8):
+            return None
+        return plan.workgroups(StreamPlan.SLICED)
+
+    def cg_update(self, p, r, nvec, state, part_rr, passes=3):
+        """The operator's two passes on p inside fused CG with the residual update
+        folded into the second (`wpc_sqlap_cg_update`): r -= ALPHA A p with ALPHA =
+        RZ / p.Ap from `state`, A p never stored; leaves `cg_update_parts` r.r
+        partials in `part_rr` and the step's scalars in `state`.  `passes` 1 or 2
+        runs one pass alone (timing)."""
+        plan = self._stream()
+        sp, sl, sc, sv = plan._sliced()
+        count = plan.workgroups(StreamPlan.SLICED)
+        part_e = plan.partials(nvec, StreamPlan.SLICED)
+        tmp = self._scratch(p.dtype, self.L.nrow * nvec)
+        nat.call("wpc_sqlap_cg_update", self.L.nrow, nat.ptr(sp), nat.ptr(sl),
+                 nat.ptr(plan._sell[7]), nat.ptr(plan._sell[5]), nat.ptr(sc), int(plan._sell[6]),
+                 nat.ptr(sv), nat.code(sv.dtype), nat.ptr(self.inverse_mass),
+                 nat.ptr(self._wm) if self.mass_weight != 0.0 else None, nat.ptr(p), nat.ptr(tmp),
+                 nat.ptr(r), nat.code(p.dtype), nvec, nat.ptr(part_e), nat.ptr(part_rr),
+                 nat.ptr(state), int(passes), nat.stream())
+        return count
+
     def _apply(self, x, y, z, alpha, beta):
         self.product(x, y, z, alpha, beta, False)
 
