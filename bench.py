@@ -13,7 +13,11 @@ relative residual (--rtol, default 1e-5, which leaves a 10x margin on this
 workload) and every timed solve's result is kept and its field error checked -- a
 step that misses 1e-8 makes the reported value 0.  The line also reports, per
 right-hand side, the certified field-error bound from a compensated residual and
-the componentwise backward error against the host Q.
+the componentwise backward error against the host Q.  After the headline come
+the roofline entries on working sets above the Infinity Cache, the CPU baseline,
+the two 3x3 block configurations (K (x) C on a 48^3 grid and the elasticity beam:
+the block product against its scalarised copy and a set of preconditioners, each
+result's error against a float64 solve) and the 512-request serving stream.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode MODE]
 
@@ -39,8 +43,8 @@ for _p in (str(ROOT), str(ROOT / "warp-preconditioners_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# The default run (headline, large-working-set rooflines, CPU baseline, serving
-# stream) stays within this many seconds of wall clock: the serving stream's guard
+# The default run (headline, large-working-set rooflines, CPU baseline, block
+# configurations, serving stream) stays within this many seconds of wall clock: the serving stream's guard
 # is what remains of it.
 SERVING_BUDGET_S = 540.0
 T_START = time.perf_counter()
@@ -459,7 +463,24 @@ def cpu_baseline(wl, s, iters_needed, sample_iters=1400):
                                         f"({one:.1f}s, {per_one * 1e3:.2f} ms/iter)"))
 
 
+# The block configurations' methods the default line runs (the rest with
+# --workload kron48 / elasticity); block ILU(0) with 3 sweeps is capped there at
+# 2,000 CG iterations (it diverges: the record shows its residual at the cap).
+DEFAULT_BLOCK_METHODS = {
+    "kron48": ["jacobi-cg-f64", "jacobi-cr-f64", "fsai16-cg-f64", "fsai16-cg-f64-bfloat16",
+               "blockfsai5-cg-f64"],
+    "elasticity": ["fsai8-cg-f64", "fsai8-cg-f64-bfloat16", "fsai8-cg-mixed-bfloat16",
+                   "ilu0-cg-f64"],
+}
+DEFAULT_BLOCK_CAPS = {"ilu0-cg-f64": 2000}
+
+
 def block_bench(name, only=None):
+    """The block-system record (block_record) printed as one JSON line."""
+    print(json.dumps(block_record(name, only)), flush=True)
+
+
+def block_record(name, only=None, caps=None):
     """Block-system comparisons, one RHS, relative residual 1e-8 (true residual
     recomputed in compensated float64), error against a float64 reference solve:
 
@@ -471,7 +492,8 @@ def block_bench(name, only=None):
                   nu=0.45, clamped at x=0, body force (0,0,-1)), 3x3 BSR, n = 763,200;
                   block ILU(0) with 3 factor sweeps against FSAI width 8, CG, float32
                   and mixed precision.
-    Prints one JSON line with a record per method."""
+    `only`: method labels to run; `caps`: {label: iteration cap} (one outer pass).
+    Returns a dict with a record per method."""
     import torch
 
     import warp_preconditioners as wp
@@ -587,9 +609,10 @@ def block_bench(name, only=None):
             rel = float(nr[0]) ** 0.5 / float(torch.linalg.vector_norm(b))
             out = dict(iterations=o[0], outer=1, relative_residual=rel, converged=rel <= 1e-8)
         elif mode == "f64":
-            out, solve = timed(lambda: linear.solve_refined(method, op64, b, x, M=P, tol=1e-8,
-                                                            inner_tol=1e-8, max_outer=3,
-                                                            maxiter=20000))
+            cap = (caps or {}).get(label)
+            out, solve = timed(lambda: linear.solve_refined(
+                method, op64, b, x, M=P, tol=1e-8, inner_tol=1e-8,
+                max_outer=3 if cap is None else 1, maxiter=20000 if cap is None else cap))
         elif mode == "mixed":
             out, solve = timed(lambda: linear.solve_refined(method, op64, b, x, M=P, inner=op32,
                                                             tol=1e-8, inner_tol=1e-4, max_outer=6,
@@ -609,6 +632,8 @@ def block_bench(name, only=None):
         rec = dict(method=label, setup_s=setup, solve_s=solve, iterations=out["iterations"],
                    outer=out["outer"], relative_residual=out["relative_residual"],
                    error_vs_f64=error(x), converged=bool(out["converged"]))
+        if (caps or {}).get(label) is not None:
+            rec["iteration_cap"] = caps[label]
         log(json.dumps(rec))
         records.append(rec)
     # the operator's product, both mappings timed by its chooser (graph replays):
@@ -625,8 +650,8 @@ def block_bench(name, only=None):
                 mode=op.block_mode(k, dt), bsr_us=t["bsr"] * 1e3, csr_us=t["csr"] * 1e3,
                 bytes=nbytes, bsr_GBs=nbytes / (t["bsr"] * 1e-3) / 1e9,
                 bsr_frac=nbytes / (t["bsr"] * 1e-3) / 8e12)
-    print(json.dumps({"workload": name, "n": n, "blocks": int(len(col)), "reference": ref,
-                      "operator": operator, "methods": records}), flush=True)
+    return {"workload": name, "n": n, "blocks": int(len(col)), "reference": ref,
+            "operator": operator, "methods": records}
 
 
 def apply_rows(sides, nvecs, reps=50):
@@ -837,7 +862,7 @@ def serving_record(args, deadline=None):
         makespan = max(r.finished for r in fin) if fin else float("nan")
         return eng, out, makespan
 
-    width = {"laplacian": 8, "biharmonic": args.bih_width}
+    width = {"laplacian": 8, "biharmonic": parse_width(args.bih_width)}
     eng, out, makespan = run(ServingConfig(keep_solutions=False, group=args.group, width=width),
                              "batched")
     if eng.timed_out:
@@ -876,7 +901,8 @@ def serving_record(args, deadline=None):
                                f"jittered tet grids of side U{{16..{args.max_side}}}, Laplacian+mass "
                                "or factored biharmonic, 1-8 N(0,1) RHS, 50% mesh repeats, "
                                "200 GB HBM budget, certified field error <= 1e-8",
-                   "requests": args.requests, "rhs": rhs},
+                   "requests": args.requests, "rhs": rhs,
+                   "biharmonic_fsai_width": str(args.bih_width)},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "requests_per_s": len(out) / makespan,
@@ -901,6 +927,18 @@ def serving_record(args, deadline=None):
             "converged": all(r.converged for r in plain)}
         record["speedup_vs_one_at_a_time"] = record["value"] / record["one_at_a_time"]["value"]
     return record
+
+
+def parse_width(text):
+    """`8` -> 8; `8,64:16` -> [(0, 8), (64, 16)] (width 16 from side 64 up)."""
+    text = str(text)
+    if "," not in text and ":" not in text:
+        return int(text)
+    out = []
+    for part in text.split(","):
+        side, _, w = part.rpartition(":")
+        out.append((int(side) if side else 0, int(w)))
+    return out
 
 
 def force_mapping(s, mapping):
@@ -931,6 +969,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving stream in the default run")
+    ap.add_argument("--no-blocks", action="store_true",
+                    help="skip the block configurations (K (x) C 48^3, elasticity beam)")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the roofline entries on working sets above the Infinity Cache")
     ap.add_argument("--mapping", default=None,
@@ -946,8 +986,9 @@ def main():
                     help="kron48 / elasticity: comma-separated method labels to run (default all)")
     ap.add_argument("--requests", type=int, default=512, help="serving: stream length")
     ap.add_argument("--max-side", type=int, default=96, help="serving: largest mesh side")
-    ap.add_argument("--bih-width", type=int, default=8,
-                    help="serving: FSAI width of the factored-biharmonic systems")
+    ap.add_argument("--bih-width", default="8",
+                    help="serving: FSAI width of the factored-biharmonic systems, or a "
+                         "policy by mesh side such as 8,64:16 (width 16 from side 64)")
     ap.add_argument("--group", action="store_true",
                     help="serving: advance same-shape small solves with one launch per pass")
     ap.add_argument("--profile-large", action="store_true",
@@ -1096,6 +1137,20 @@ def main():
             record["cpu_baseline"] = cpu_baseline(wl, s, max(s.iterations))
         except Exception as exc:  # the baseline is reported, never required
             record["cpu_baseline"] = {"error": str(exc)}
+    if not args.no_blocks:
+        # configurations 3 and 4 (3x3 BSR): the block product and the methods of
+        # DEFAULT_BLOCK_METHODS, each with its error against a float64 solve
+        del s
+        torch.cuda.empty_cache()
+        record["block_configs"] = {}
+        for name in ("kron48", "elasticity"):
+            try:
+                record["block_configs"][name] = block_record(
+                    name, DEFAULT_BLOCK_METHODS[name], DEFAULT_BLOCK_CAPS)
+            except Exception as exc:  # reported, never required
+                record["block_configs"][name] = {"error": str(exc)}
+            torch.cuda.empty_cache()
+        s = None
     if not args.no_serving:
         # the serving stream (its p50/p99 latencies complete the metric), under a
         # wall-clock guard that keeps the whole run inside SERVING_BUDGET_S

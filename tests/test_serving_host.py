@@ -43,3 +43,41 @@ def test_memory_estimates_scale_with_size():
     s_sys, _ = ServingEngine.estimate(small, "f64")
     b_sys, _ = ServingEngine.estimate(big, "f64")
     assert b_sys / s_sys == (96 / 16) ** 3
+
+
+def test_width_policy_by_kind_and_side():
+    import pytest
+
+    from warp_preconditioners.serving import fsai_width
+
+    assert fsai_width(8, "biharmonic", 96) == 8
+    policy = {"laplacian": 8, "biharmonic": [(0, 8), (64, 16)]}
+    assert fsai_width(policy, "laplacian", 96) == 8
+    assert fsai_width(policy, "biharmonic", 63) == 8
+    assert fsai_width(policy, "biharmonic", 64) == 16
+    assert fsai_width(policy, "biharmonic", 96) == 16
+    with pytest.raises(ValueError, match="no FSAI width"):
+        fsai_width({"biharmonic": [(32, 16)]}, "biharmonic", 16)
+
+
+def test_setup_worker_takes_the_smallest_pending_job_first():
+    import threading
+
+    from warp_preconditioners.serving import PrioritySetup
+
+    gate = threading.Event()
+    order = []
+    w = PrioritySetup()
+    first = w.submit(0.0, gate.wait)  # holds the worker while the rest queue up
+    futs = [w.submit(p, order.append, name) for p, name in
+            ((5.0, "large"), (1.0, "small"), (3.0, "mid"), (1.0, "small-later"))]
+    gate.set()
+    for f in [first, *futs]:
+        f.result(timeout=10)
+    assert order == ["small", "small-later", "mid", "large"]
+    boom = w.submit(0.0, lambda: 1 / 0)
+    import pytest
+
+    with pytest.raises(ZeroDivisionError):
+        boom.result(timeout=10)
+    w.shutdown()

@@ -112,7 +112,9 @@ class ServingConfig:
     budget_bytes: float = 200e9
     streams: int = 4
     max_active: int = 64
-    width: object = 8  # FSAI width: an int, or {kind: width}
+    # FSAI width: an int, or {kind: int | [(smallest side, width), ...]} (the width of
+    # the last entry whose side the mesh reaches, e.g. [(0, 8), (64, 16)])
+    width: object = 8
     tol: float = 1e-8
     precision: dict = field(default_factory=lambda: {"laplacian": "mixed", "biharmonic": "f64"})
     factor_dtype: dict = field(default_factory=lambda: {"mixed": torch.bfloat16,
@@ -200,7 +202,7 @@ class System:
             self.topology = template.topology
             self.provenance = "refit"
         else:
-            width = cfg.width[kind] if isinstance(cfg.width, dict) else cfg.width
+            width = fsai_width(cfg.width, kind, spec.side)
             self.P = FSAI(source, max_row_size=width, factor_dtype=cfg.factor_dtype[self.policy])
             self.topology = (source.offsets, source.columns)
             self.provenance = "built"
@@ -608,6 +610,73 @@ def estimated_seconds(req: Request, cached: bool) -> float:
     return work + (0.0 if cached else 0.2 * n / 96 ** 3)
 
 
+def fsai_width(width, kind: str, side: int) -> int:
+    """The FSAI width of a system of `kind` on a mesh of `side` under the config's
+    width policy (ServingConfig.width)."""
+    w = width[kind] if isinstance(width, dict) else width
+    if isinstance(w, int):
+        return w
+    chosen = None
+    for smallest, value in sorted(w):
+        if side >= smallest:
+            chosen = value
+    if chosen is None:
+        raise ValueError(f"no FSAI width for side {side} in {w}")
+    return int(chosen)
+
+
+class PrioritySetup:
+    """One setup worker thread that takes the pending job with the smallest
+    priority first (ties in submission order), so a small request's system is not
+    built behind a large one's."""
+
+    def __init__(self):
+        import heapq
+        import itertools
+        import threading
+
+        self._heapq = heapq
+        self._heap = []
+        self._seq = itertools.count()
+        self._cv = threading.Condition()
+        self._closed = False
+        self._thread = threading.Thread(target=self._run, name="wpc-setup", daemon=True)
+        self._thread.start()
+
+    def submit(self, priority, fn, *args):
+        from concurrent.futures import Future
+
+        fut = Future()
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("setup worker is shut down")
+            self._heapq.heappush(self._heap, (priority, next(self._seq), fut, fn, args))
+            self._cv.notify()
+        return fut
+
+    def _run(self):
+        while True:
+            with self._cv:
+                while not self._heap and not self._closed:
+                    self._cv.wait()
+                if not self._heap:
+                    return
+                _, _, fut, fn, args = self._heapq.heappop(self._heap)
+            if not fut.set_running_or_notify_cancel():
+                continue
+            try:
+                fut.set_result(fn(*args))
+            except BaseException as exc:  # handed to the scheduling thread
+                fut.set_exception(exc)
+
+    def shutdown(self, wait=True):
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        if wait:
+            self._thread.join()
+
+
 def gpu_fill(n: int, nrhs: int) -> float:
     """Fraction of the device one system's products keep busy (~1 from about two
     million unknown-RHS pairs up)."""
@@ -623,7 +692,6 @@ class ServingEngine:
 
     def __init__(self, config: ServingConfig | None = None, device=None):
         import threading
-        from concurrent.futures import ThreadPoolExecutor
 
         self.cfg = config or ServingConfig()
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
@@ -634,7 +702,7 @@ class ServingEngine:
         self.systems = OrderedDict()  # (kind, mesh, alpha) -> System, LRU order
         self.templates = {}  # (kind, side, alpha, policy) -> System whose factor is refit
         self._templates_lock = threading.Lock()
-        self._pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="wpc-setup")
+        self._pool = PrioritySetup()
         self.building = {}  # key -> (future, estimated bytes)
         self.waiting = {}  # key -> [(request, admitted at)]
         self.t0 = None
@@ -731,7 +799,8 @@ class ServingEngine:
         else:
             stream = self.streams[self._next % len(self.streams)]
             self._next += 1
-            self.building[key] = (self._pool.submit(self._build, req, stream), sys_b)
+            self.building[key] = (self._pool.submit(estimated_seconds(req, False), self._build,
+                                                    req, stream), sys_b)
             self.waiting[key] = [(req, self.clock())]
         self.stats["peak_bytes"] = max(self.stats["peak_bytes"], self.used_bytes())
         return True
