@@ -713,6 +713,11 @@ class ServingEngine:
     def clock(self):
         return time.perf_counter() - self.t0
 
+    def __del__(self):
+        pool = getattr(self, "_pool", None)
+        if pool is not None:
+            pool.shutdown(wait=False)  # its thread ends once no setup is pending
+
     def used_bytes(self):
         return (sum(s.bytes + sum(x.bytes for v in s.slots.values() for x in v)
                     for s in self.systems.values())
