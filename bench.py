@@ -289,8 +289,10 @@ def kernel_times(s, reps=30):
     # else the separate update step (the folds in place when the partial counts allow)
     from warp_preconditioners import linear
 
-    fused = (linear.FUSE_OPERATOR_UPDATE and op.cg_update_parts(k, dt) is not None
-             and max(pk, pz) * k <= FOLD_IN_PLACE)
+    pcg = op.cg_update_parts(k, dt) if linear.FUSE_OPERATOR_UPDATE else None
+    fused = pcg is not None and max(pcg, pz) * k <= FOLD_IN_PLACE
+    if fused:
+        pk = pcg  # the fused passes' partials (curvature, then r.r)
     prr = pk if fused else npart  # r.r partials the direction step folds
     if fused:
         rr = torch.zeros(pk * k, dtype=torch.float64, device="cuda:0")
@@ -519,6 +521,12 @@ def block_record(name, only=None, caps=None):
                          torch.tensor(val, dtype=dtype, device=dev))
 
     A64, A32 = bsr(torch.float64), bsr(torch.float32)
+    import scipy.sparse as ss
+
+    from warp_preconditioners.ingest import backward_error
+
+    a_host = ss.bsr_matrix((val, col, off), shape=(3 * nb, 3 * nb)).tocsr()
+    abs_a = abs(a_host)
     # the BSR matrices themselves are the operators (Warp's bsr_mv in the reference's
     # block benchmarks, block_challenge.py:155,197): the 3x3 sliced block product or
     # the scalarised CSR copy, whichever the operator's timing picks
@@ -629,9 +637,12 @@ def block_record(name, only=None, caps=None):
             rel = float(nr[0]) ** 0.5 / float(torch.linalg.vector_norm(b))
             out = dict(iterations=o["iterations"][0], outer=1, relative_residual=rel,
                        converged=rel <= 1e-8)
+        xh = x.cpu().numpy()
         rec = dict(method=label, setup_s=setup, solve_s=solve, iterations=out["iterations"],
                    outer=out["outer"], relative_residual=out["relative_residual"],
-                   error_vs_f64=error(x), converged=bool(out["converged"]))
+                   error_vs_f64=error(x),
+                   backward_error=backward_error(a_host, abs_a, rhs, xh),
+                   converged=bool(out["converged"]))
         if (caps or {}).get(label) is not None:
             rec["iteration_cap"] = caps[label]
         log(json.dumps(rec))
@@ -750,12 +761,13 @@ def beam_product_row():
                 frac=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
 
-def large_rooflines():
+def large_rooflines(rows=None):
     """Roofline entries whose working set exceeds the 256 MB Infinity Cache: the
-    128^3 FSAI apply (1 and 3 right-hand sides) and the elasticity beam's block
-    product."""
+    128^3 FSAI apply (1 and 3 right-hand sides; `rows` from apply_rows when already
+    timed) and the elasticity beam's block product."""
     out = []
-    for r in apply_rows((128,), (1, 3)):
+    rows = apply_rows((128,), (1, 3)) if rows is None else [r for r in rows if r["side"] == 128]
+    for r in rows:
         out.append({"kernel": f"fsai_apply_128cube_{r['nvec']}rhs", "bound": "hbm",
                     "achieved": r["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": r["frac"], "bytes": r["bytes"], "avg_us": r["us"],
@@ -962,9 +974,10 @@ def main():
                              "batched-mixed-bf16"])
     ap.add_argument("--rtol", type=float, default=1e-5,
                     help="recursive relative-residual stopping tolerance")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain one-right-hand-side paths beside the headline")
     ap.add_argument("--compare", action="store_true",
-                    help="also time the plain one-RHS-at-a-time paths (float32 and float64 "
-                         "Krylov vectors; serving: the one-at-a-time engine)")
+                    help="serving: also run the one-request, one-RHS-at-a-time engine")
     ap.add_argument("--side", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
@@ -1103,7 +1116,7 @@ def main():
         "converged": ok,
         "setup_s": s.setup_s,
     }
-    if args.compare:
+    if not args.no_compare:
         # the plain one-right-hand-side-at-a-time paths on the same GPU and stopping rule
         record["compare"] = {}
         for mode in ("plain-f32", "plain-f64"):
@@ -1127,9 +1140,12 @@ def main():
         f64 = record["compare"]["plain-f64"]["solves_per_s"]
         record["compare"]["batched_vs_plain_f64"] = record["value"] / f64
     if not args.no_large:
+        # the FSAI apply alone (64^3 to 128^3, 1 and 3 right-hand sides) and the
         # kernels on working sets above the Infinity Cache, timed after the solves
         try:
-            record["roofline_large"] = large_rooflines()
+            rows = apply_rows((64, 96, 128), (1, 3))
+            record["fsai_apply"] = rows
+            record["roofline_large"] = large_rooflines(rows)
         except Exception as exc:  # reported, never required
             record["roofline_large"] = {"error": str(exc)}
     if not args.no_cpu:

@@ -508,12 +508,13 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int 
  * NULL for w = 0; L symmetric).  Pass 2: Ap = L t + W p row by row (not stored),
  * ALPHA = RZ / (fold of the pass-1 partials) for slots with RR > TARGET (else 0),
  * r -= ALPHA Ap, per-workgroup partials of r.r; workgroup 0 writes RZ_OLD, DENOM,
- * ACTIVE, ALPHA as wpc_cg_update_fold does.  The direction step then folds
- * ceil(n / 256) r.r partials from part_rr and advances x (wpc_cg_direction_fold
- * with x).  The sliced copy of L (rows in natural or given order) as for
- * wpc_sell_stream; values, inverse_mass, weighted_mass and vectors all F64 or all
- * F32; nvec 1..8; part_energy and part_rr hold ceil(n / 256) * nvec doubles each
- * (folded in place by every workgroup: meant for ceil(n / 256) * nvec <= 4096).
+ * ACTIVE, ALPHA as wpc_cg_update_fold does.  The direction step then folds the
+ * wpc_sqlap_cg_partials(n) r.r partials from part_rr and advances x
+ * (wpc_cg_direction_fold with x).  The sliced copy of L (rows in natural or given
+ * order) as for wpc_sell_stream; values, inverse_mass, weighted_mass and vectors
+ * all F64 or all F32; nvec 1..8; part_energy and part_rr hold
+ * wpc_sqlap_cg_partials(n) * nvec doubles each (the energy partials are folded in
+ * place by every workgroup of pass 2: meant for counts times nvec <= 4096).
  * The curvature differs from p . (A p) by rounding only.  passes = 3 runs both;
  * 1 or 2 runs one of them (for timing each kernel alone). */
 int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
@@ -522,6 +523,10 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
                         const void* inverse_mass, const void* weighted_mass, const void* p,
                         void* t, void* r, int vec_dtype, int nvec, double* part_energy,
                         double* part_rr, double* state, int passes, void* stream);
+
+/* Partials per vector that each pass of wpc_sqlap_cg_update leaves (one per
+ * workgroup: ceil(n / 512)).  Host-only helper; no reference counterpart. */
+int wpc_sqlap_cg_partials(int n);
 
 /* ------------------------------------------------------------------ grouped CG steps
  * No reference counterpart: a serving extension that advances the fused CG of

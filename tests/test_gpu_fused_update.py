@@ -70,7 +70,7 @@ def test_fused_step_is_the_update_step_with_the_energy_curvature(dtype, weight, 
              nparts, nat.ptr(part), nat.ptr(st_sep), nat.ptr(rr_sep), s)
 
     count = A.cg_update_parts(nvec, dtype)
-    assert count == (n + 255) // 256
+    assert count == (n + 511) // 512
     r_f, st_f = r.clone(), _state(nvec, inactive)
     rr_f = torch.zeros(count * nvec, dtype=torch.float64, device=dev())
     A.cg_update(p, r_f, nvec, st_f, rr_f)

@@ -66,25 +66,27 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 constexpr int kBlockWaves = kStreamRows / kWave;
 
-// Deterministic block sum of v[NV] (all threads) into v[0..NV) of thread 0.
-// red: kStreamRows doubles of scratch (NV * kBlockWaves are used, NV <= 16).
-template <int NV>
+// Deterministic block sum of v[NV] (all NT threads) into v[0..NV) of thread 0:
+// wave sums, then thread 0 adds the NT / 64 wave sums in wave order.
+// red: kStreamRows doubles of scratch (NV * NT / 64 are used).
+template <int NV, int NT = kStreamRows>
 __device__ void block_sum(double* v, double* red) {
-    static_assert(NV * kBlockWaves <= kStreamRows, "scratch too small");
+    constexpr int W = NT / kWave;
+    static_assert(NV * W <= kStreamRows, "scratch too small");
     double s[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = wave_sum(v[q]);
     __syncthreads();  // red may still be read by an earlier use
     if ((threadIdx.x & (kWave - 1)) == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) red[q * kBlockWaves + threadIdx.x / kWave] = s[q];
+        for (int q = 0; q < NV; ++q) red[q * W + threadIdx.x / kWave] = s[q];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             double t = 0.0;
-            for (int w = 0; w < kBlockWaves; ++w) t += red[q * kBlockWaves + w];
+            for (int w = 0; w < W; ++w) t += red[q * W + w];
             v[q] = t;
         }
     }

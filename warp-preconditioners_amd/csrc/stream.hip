@@ -70,6 +70,10 @@ int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int
     return stream_dispatch(nvec, a);
 }
 
+int wpc_sqlap_cg_partials(int n) {
+    return n <= 0 ? 0 : (int)blocks_for(n, kSqlapCgRows);
+}
+
 int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
                         const int* slice_base, const int* rows, const void* sell_columns,
                         int col16, const void* sell_values, int val_dtype,

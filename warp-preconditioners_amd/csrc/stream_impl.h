@@ -130,6 +130,13 @@ struct SqlapCgArgs {
     hipStream_t st;
 };
 
+// Both passes run workgroups of kSqlapCgRows threads (eight wavefronts), which
+// leave one partial per workgroup and vector: half the partials of 256-thread
+// workgroups, for the second pass to fold in every one of its workgroups (those
+// L2 reads compete with its gathers) and for the direction step to fold after it
+// (1,024 threads made the first pass slower).
+constexpr int kSqlapCgRows = 512;
+
 int sqlap_cg_entry_1(const SqlapCgArgs& a);
 int sqlap_cg_entry_2(const SqlapCgArgs& a);
 int sqlap_cg_entry_3(const SqlapCgArgs& a);
@@ -629,13 +636,13 @@ struct SellHead {
     long base;
 };
 
-template <class TX, int NV>
+template <class TX, int NV, int NT = kStreamRows>
 __device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restrict__ slice_ptr,
                                               const int* __restrict__ slice_len,
                                               const int* __restrict__ slice_base,
                                               const int* __restrict__ rows, bool col16,
                                               const TX* x, __amdgpu_buffer_rsrc_t* xr) {
-    const int idx = blk * kStreamRows + threadIdx.x;
+    const int idx = blk * NT + threadIdx.x;
     const int wave0 = __builtin_amdgcn_readfirstlane(idx & ~63);
     const int slice = min(wave0, n - 1) >> 6;
     const int sp = slice_ptr[slice];
@@ -655,8 +662,9 @@ __device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restr
 }
 
 // Pass 1: t = s .* (L p) (s = 1/mass), partials of (L p)_i t_i + W_i p_i^2.
+
 template <class TV, class TX, int NV, bool HW>
-__global__ __launch_bounds__(kStreamRows) void sell_energy_kernel(
+__global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
     const void* __restrict__ col, int col16, const TV* __restrict__ val,
@@ -666,8 +674,8 @@ __global__ __launch_bounds__(kStreamRows) void sell_energy_kernel(
     __shared__ double red[kStreamRows];
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
-    const SellHead h = sell_head<TX, NV>(blk, n, slice_ptr, slice_len, slice_base, rows,
-                                         col16 != 0, p, &xr);
+    const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len, slice_base,
+                                                      rows, col16 != 0, p, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     const TA sv = St<TX>::template ld<TA>(s, h.row);
     TA wv = TA(0), pv[NV];
@@ -699,14 +707,14 @@ __global__ __launch_bounds__(kStreamRows) void sell_energy_kernel(
 #pragma unroll
         for (int v = 0; v < NV; ++v) t[(long)h.row * NV + v] = out[v];
     }
-    block_sum<NV>(e, red);
+    block_sum<NV, kSqlapCgRows>(e, red);
     write_partials<NV>(e, partials, blk);
 }
 
 // Pass 2: Ap = L t + W p (not stored); ALPHA from the P pass-1 partials;
 // r -= ALPHA Ap for stepping slots; partials of r.r.
 template <class TV, class TX, int NV, bool HW>
-__global__ __launch_bounds__(kStreamRows) void sell_cg_update_kernel(
+__global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
     const void* __restrict__ col, int col16, const TV* __restrict__ val,
@@ -718,8 +726,8 @@ __global__ __launch_bounds__(kStreamRows) void sell_cg_update_kernel(
     __shared__ double sums[8];
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
-    const SellHead h = sell_head<TX, NV>(blk, n, slice_ptr, slice_len, slice_base, rows,
-                                         col16 != 0, t, &xr);
+    const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len,
+                                                       slice_base, rows, col16 != 0, t, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     TA wv = TA(0), pv[NV];
     if (HW) {
@@ -739,7 +747,7 @@ __global__ __launch_bounds__(kStreamRows) void sell_cg_update_kernel(
     // partials and the residual rows, all in one round trip.
     StateLanes<WPC_KS_RZ, WPC_KS_RR, WPC_KS_TARGET> st;
     st.load(state, NV);
-    FoldRegs<kStreamRows, NV> fold;
+    FoldRegs<kSqlapCgRows, NV> fold;
     fold.load(part_e, P);
     TX rv[NV];
 #pragma unroll
@@ -777,7 +785,7 @@ __global__ __launch_bounds__(kStreamRows) void sell_cg_update_kernel(
 #pragma unroll
         for (int v = 0; v < NV; ++v) r[(long)h.row * NV + v] = rn[v];
     }
-    block_sum<NV>(rr, red);
+    block_sum<NV, kSqlapCgRows>(rr, red);
     write_partials<NV>(rr, part_rr, blk);
 }
 
@@ -959,17 +967,17 @@ int stream_for_nv(const StreamArgs& a) {
 
 template <class TV, class TX, int NV>
 int sqlap_cg_launch(const SqlapCgArgs& a) {
-    const int grid = (int)blocks_for(a.n, kStreamRows);
+    const int grid = (int)blocks_for(a.n, kSqlapCgRows);  // each pass's partial count
 #define SC(HW)                                                                                  \
     do {                                                                                        \
         if (a.passes & 1)                                                                       \
-        hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW>), dim3(grid), dim3(kStreamRows), \
+        hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW>), dim3(grid), dim3(kSqlapCgRows), \
                            0, a.st, a.n, a.slice_ptr, a.slice_len, a.slice_base, a.rows, a.col, \
                            (int)a.col16, (const TV*)a.val, (const TX*)a.s, (const TX*)a.wm,     \
                            (const TX*)a.p, (TX*)a.t, a.part_e);                                 \
         if (a.passes & 2)                                                                       \
         hipLaunchKernelGGL((sell_cg_update_kernel<TV, TX, NV, HW>), dim3(grid),                 \
-                           dim3(kStreamRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
+                           dim3(kSqlapCgRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
                            a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
                            (const TX*)a.wm, (const TX*)a.p, (const TX*)a.t, (TX*)a.r, grid,     \
                            a.part_e, a.state, a.part_rr);                                       \

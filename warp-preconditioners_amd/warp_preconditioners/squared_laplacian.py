@@ -71,6 +71,7 @@ class SquaredLaplacianOperator(LinearOperator):
         self._tmp = {}
         self._plan = None
         self._wm = None
+        self._part_energy = None  # curvature partials of the fused CG pair (cg_update)
         nf = self.free.numel()
         super().__init__((nf, nf), L.dtype, dev, self._apply)
 
@@ -130,7 +131,7 @@ class SquaredLaplacianOperator(LinearOperator):
             return None
         if not plan.sliced_fits(nvec, torch.finfo(dtype).bits // 8):
             return None
-        return plan.workgroups(StreamPlan.SLICED)
+        return nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
 
     def cg_update(self, p, r, nvec, state, part_rr, passes=3):
         """The operator's two passes on p inside fused CG with the residual update
@@ -140,8 +141,12 @@ class SquaredLaplacianOperator(LinearOperator):
         runs one pass alone (timing)."""
         plan = self._stream()
         sp, sl, sc, sv = plan._sliced()
-        count = plan.workgroups(StreamPlan.SLICED)
-        part_e = plan.partials(nvec, StreamPlan.SLICED)
+        count = nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
+        if self._part_energy is None or self._part_energy.numel() < count * nvec:
+            self._part_energy = torch.zeros(max(count * nvec, 1), dtype=torch.float64,
+                                            device=self.L.device)
+            nat.bump_generation()
+        part_e = self._part_energy
         tmp = self._scratch(p.dtype, self.L.nrow * nvec)
         nat.call("wpc_sqlap_cg_update", self.L.nrow, nat.ptr(sp), nat.ptr(sl),
                  nat.ptr(plan._sell[7]), nat.ptr(plan._sell[5]), nat.ptr(sc), int(plan._sell[6]),
