@@ -516,13 +516,18 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int 
  * wpc_sqlap_cg_partials(n) * nvec doubles each (the energy partials are folded in
  * place by every workgroup of pass 2: meant for counts times nvec <= 4096).
  * The curvature differs from p . (A p) by rounding only.  passes = 3 runs both;
- * 1 or 2 runs one of them (for timing each kernel alone). */
+ * 1 or 2 runs one of them (for timing each kernel alone).  Bit 2 (passes 6 or 7),
+ * for partial counts too large to fold in every workgroup: pass 2 takes ALPHA as
+ * wpc_cg_alpha left it in the state after folding the pass-1 partials (no fold,
+ * no state writes) and also advances x += ALPHA p, as wpc_cg_update does; the
+ * step then ends with wpc_cg_beta (these r.r partials) and wpc_cg_direction.  x
+ * is read only with bit 2. */
 int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
                         const int* slice_base, const int* rows, const void* sell_columns,
                         int col16, const void* sell_values, int val_dtype,
                         const void* inverse_mass, const void* weighted_mass, const void* p,
                         void* t, void* r, int vec_dtype, int nvec, double* part_energy,
-                        double* part_rr, double* state, int passes, void* stream);
+                        double* part_rr, double* state, int passes, void* x, void* stream);
 
 /* Partials per vector that each pass of wpc_sqlap_cg_update leaves (one per
  * workgroup: ceil(n / 512)).  Host-only helper; no reference counterpart. */

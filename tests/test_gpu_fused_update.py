@@ -153,3 +153,59 @@ def test_fused_steps_replay_from_a_graph_as_run_eagerly():
         xs.append((host(X), o["iterations"]))
     np.testing.assert_array_equal(xs[0][0], xs[1][0])
     assert xs[0][1] == xs[1][1]
+
+
+@pytest.mark.parametrize("weight", [0.0, 1.0])
+def test_fused_solves_with_scalar_folds(weight, monkeypatch):
+    """Partial counts too large for the in-place folds (forced here): the curvature
+    folded by wpc_cg_alpha, the second pass taking ALPHA from the state and
+    advancing x, then wpc_cg_beta and wpc_cg_direction -- the same solves as the
+    separate update step with the same scalar folds."""
+    import warp_preconditioners as wp
+    from warp_preconditioners import linear
+
+    monkeypatch.setattr(linear, "FOLD_IN_PLACE", 1)
+    n, A, q = _operator(44, 21, torch.float64, weight)
+    P = wp.FSAI(upload(q), max_row_size=8, factor_dtype=torch.float32)
+    rng = np.random.default_rng(7)
+    B = _vec(rng.normal(size=(n, 3)), torch.float64)
+    out = {}
+    for fuse in (False, True):
+        work = linear.KrylovWork("cg", A, P, n, 3, torch.float64, dev())
+        work.fuse_update = fuse
+        X = torch.zeros_like(B)
+        o = linear.solve_batched("cg", A, B, X, M=P, tol=1e-10, maxiter=20000, check_every=25,
+                                 work=work)
+        assert (work._rr_parts is not None) == fuse
+        out[fuse] = (o["iterations"], host(X))
+    for fuse, (its, X) in out.items():
+        for v in range(3):
+            res = np.linalg.norm(host(B)[:, v] - q @ X[:, v]) / np.linalg.norm(host(B)[:, v])
+            assert res <= 1.5e-10, (fuse, v, res)
+    for a, b in zip(out[True][0], out[False][0]):
+        assert abs(a - b) <= max(2, 0.02 * b), (out[True][0], out[False][0])
+    # replayed from a graph as run eagerly
+    xs = []
+    for graph in (False, True):
+        X = torch.zeros_like(B)
+        o = linear.solve_batched("cg", A, B, X, M=P, tol=1e-9, maxiter=5000, check_every=25,
+                                 use_cuda_graph=graph)
+        xs.append((host(X), o["iterations"]))
+    np.testing.assert_array_equal(xs[0][0], xs[1][0])
+    assert xs[0][1] == xs[1][1]
+
+
+def test_energy_partials_never_move_for_up_to_8_vectors():
+    """Graphs captured by served requests bake in this buffer: it is sized for 8
+    vectors when first made, so a request with more vectors never reallocates it
+    under a graph another request has in flight."""
+    from warp_preconditioners import _native as nat
+
+    n, A, _ = _operator(30, 1, torch.float64, 1.0)
+    gen = nat.generation()
+    first = A.energy_partials(1)
+    assert first.numel() >= nat.lib().wpc_sqlap_cg_partials(n) * 8
+    after = nat.generation()
+    for nvec in range(1, 9):
+        assert A.energy_partials(nvec).data_ptr() == first.data_ptr()
+    assert nat.generation() == after and after >= gen

@@ -79,8 +79,8 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
                         int col16, const void* sell_values, int val_dtype,
                         const void* inverse_mass, const void* weighted_mass, const void* p,
                         void* t, void* r, int vec_dtype, int nvec, double* part_energy,
-                        double* part_rr, double* state, int passes, void* stream) {
-    if (passes < 1 || passes > 3) return WPC_EARG;
+                        double* part_rr, double* state, int passes, void* x, void* stream) {
+    if (passes < 1 || passes > 7 || passes == 4 || passes == 5) return WPC_EARG;
     if (n <= 0) return WPC_OK;
     if (slice_ptr == nullptr || slice_len == nullptr || slice_base == nullptr ||
         sell_columns == nullptr || sell_values == nullptr || inverse_mass == nullptr ||
@@ -88,9 +88,10 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
         part_rr == nullptr || state == nullptr)
         return WPC_EARG;
     if (p == t || p == r || t == r) return WPC_EARG;
+    if ((passes & 4) && (x == nullptr || x == p || x == t || x == r)) return WPC_EARG;
     const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
                         sell_values, val_dtype, inverse_mass, weighted_mass, p, t, r, vec_dtype,
-                        part_energy, part_rr, state, passes, as_stream(stream)};
+                        part_energy, part_rr, state, passes, x, as_stream(stream)};
     switch (nvec) {
         case 1: return sqlap_cg_entry_1(a);
         case 2: return sqlap_cg_entry_2(a);

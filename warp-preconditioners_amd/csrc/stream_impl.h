@@ -126,7 +126,8 @@ struct SqlapCgArgs {
     double* part_e;
     double* part_rr;
     double* state;
-    int passes;  // bit 0: the first pass, bit 1: the second
+    int passes;  // bit 0: the first pass, bit 1: the second, bit 2: ALPHA from the state
+    void* x;     // advanced by the second pass with bit 2
     hipStream_t st;
 };
 
@@ -712,15 +713,18 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
 }
 
 // Pass 2: Ap = L t + W p (not stored); ALPHA from the P pass-1 partials;
-// r -= ALPHA Ap for stepping slots; partials of r.r.
-template <class TV, class TX, int NV, bool HW>
+// r -= ALPHA Ap for stepping slots; partials of r.r.  FS (for partial counts too
+// large to fold in every workgroup): ALPHA as the scalar fold wpc_cg_alpha left it
+// in the state (no fold, no state writes), and x += ALPHA p here, as the update
+// step would.
+template <class TV, class TX, int NV, bool HW, bool FS>
 __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
     const void* __restrict__ col, int col16, const TV* __restrict__ val,
     const TX* __restrict__ wm, const TX* __restrict__ p, const TX* __restrict__ t,
     TX* __restrict__ r, int P, const double* __restrict__ part_e, double* state,
-    double* __restrict__ part_rr) {
+    double* __restrict__ part_rr, TX* __restrict__ x) {
     using TA = typename AccOf<TX>::type;
     __shared__ double red[kStreamRows];
     __shared__ double sums[8];
@@ -743,29 +747,40 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
                                                (const unsigned short*)col, val, NV);
     else
         sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
-    // After the gathers (their registers are free again): the state fields, the
-    // partials and the residual rows, all in one round trip.
-    StateLanes<WPC_KS_RZ, WPC_KS_RR, WPC_KS_TARGET> st;
-    st.load(state, NV);
-    FoldRegs<kSqlapCgRows, NV> fold;
-    fold.load(part_e, P);
-    TX rv[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) rv[v] = r[(long)h.row * NV + v];
-    fold.finish(part_e, P, sums);
     TX alpha[NV];
+    TX rv[NV], xv[NV], px[NV];
+    if constexpr (FS) {
+        // the residual, x and p rows and ALPHA, in one round trip after the gathers
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const double denom = sums[v];
-        const double rz = st.get(0, NV, v);
-        const bool active = st.get(1, NV, v) > st.get(2, NV, v);
-        const double a = active ? rz / denom : 0.0;
-        alpha[v] = (TX)a;
-        if (blk == 0 && threadIdx.x == v) {
-            state[WPC_KS_RZ_OLD * NV + v] = rz;
-            state[WPC_KS_DENOM * NV + v] = denom;
-            state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
-            state[WPC_KS_ALPHA * NV + v] = a;
+        for (int v = 0; v < NV; ++v) {
+            rv[v] = r[(long)h.row * NV + v];
+            xv[v] = x[(long)h.row * NV + v];
+            px[v] = HW ? (TX)pv[v] : p[(long)h.row * NV + v];  // (TA is TX here)
+            alpha[v] = (TX)state[WPC_KS_ALPHA * NV + v];
+        }
+    } else {
+        // After the gathers (their registers are free again): the state fields, the
+        // partials and the residual rows, all in one round trip.
+        StateLanes<WPC_KS_RZ, WPC_KS_RR, WPC_KS_TARGET> st;
+        st.load(state, NV);
+        FoldRegs<kSqlapCgRows, NV> fold;
+        fold.load(part_e, P);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) rv[v] = r[(long)h.row * NV + v];
+        fold.finish(part_e, P, sums);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double denom = sums[v];
+            const double rz = st.get(0, NV, v);
+            const bool active = st.get(1, NV, v) > st.get(2, NV, v);
+            const double a = active ? rz / denom : 0.0;
+            alpha[v] = (TX)a;
+            if (blk == 0 && threadIdx.x == v) {
+                state[WPC_KS_RZ_OLD * NV + v] = rz;
+                state[WPC_KS_DENOM * NV + v] = denom;
+                state[WPC_KS_ACTIVE * NV + v] = active ? 1.0 : 0.0;
+                state[WPC_KS_ALPHA * NV + v] = a;
+            }
         }
     }
     double rr[NV];
@@ -784,6 +799,11 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     if (h.live) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) r[(long)h.row * NV + v] = rn[v];
+        if constexpr (FS) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                if (alpha[v] != TX(0)) x[(long)h.row * NV + v] = xv[v] + alpha[v] * px[v];
+        }
     }
     block_sum<NV, kSqlapCgRows>(rr, red);
     write_partials<NV>(rr, part_rr, blk);
@@ -968,7 +988,7 @@ int stream_for_nv(const StreamArgs& a) {
 template <class TV, class TX, int NV>
 int sqlap_cg_launch(const SqlapCgArgs& a) {
     const int grid = (int)blocks_for(a.n, kSqlapCgRows);  // each pass's partial count
-#define SC(HW)                                                                                  \
+#define SC(HW, FS)                                                                              \
     do {                                                                                        \
         if (a.passes & 1)                                                                       \
         hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW>), dim3(grid), dim3(kSqlapCgRows), \
@@ -976,14 +996,17 @@ int sqlap_cg_launch(const SqlapCgArgs& a) {
                            (int)a.col16, (const TV*)a.val, (const TX*)a.s, (const TX*)a.wm,     \
                            (const TX*)a.p, (TX*)a.t, a.part_e);                                 \
         if (a.passes & 2)                                                                       \
-        hipLaunchKernelGGL((sell_cg_update_kernel<TV, TX, NV, HW>), dim3(grid),                 \
+        hipLaunchKernelGGL((sell_cg_update_kernel<TV, TX, NV, HW, FS>), dim3(grid),             \
                            dim3(kSqlapCgRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
                            a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
                            (const TX*)a.wm, (const TX*)a.p, (const TX*)a.t, (TX*)a.r, grid,     \
-                           a.part_e, a.state, a.part_rr);                                       \
+                           a.part_e, a.state, a.part_rr, (TX*)a.x);                             \
     } while (0)
-    if (a.wm != nullptr) SC(true);
-    else SC(false);
+    const bool fs = (a.passes & 4) != 0;
+    if (a.wm != nullptr && fs) SC(true, true);
+    else if (a.wm != nullptr) SC(true, false);
+    else if (fs) SC(false, true);
+    else SC(false, false);
 #undef SC
     return launch_status();
 }

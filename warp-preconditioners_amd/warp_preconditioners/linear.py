@@ -330,31 +330,47 @@ class KrylovWork:
 
     def _fused_update_step(self, x, s, st):
         """One CG step with the update folded into the operator's last pass
-        (operators with `cg_update`: the factored squared Laplacian) -- four launches
-        on the headline path instead of five: A p with the update (+r.r partials) |
-        M r (+r.z partials) | direction (+x).  Taken once the preconditioner's
-        partial count is known (from the first step) and both folds fit in place;
-        returns False to leave the step to the general path."""
+        (operators with `cg_update`: the factored squared Laplacian) -- five launches
+        on the headline path instead of six: A p with the update (+r.r partials) in
+        two passes | M r (+r.z partials) in two | direction (+x).  With partial
+        counts too large to fold in every workgroup, the curvature and the r.z / r.r
+        folds run as single-workgroup launches instead (the first pass | ALPHA |
+        the second pass with the update and x | M r | BETA | direction: six launches
+        instead of eight).  Taken once the preconditioner's partial count is known
+        (from the first step); returns False to leave the step to the general path."""
         if not self.fuse_update or self._m_parts is None:
             return False
         parts_of = getattr(self.A, "cg_update_parts", None)
         if parts_of is None:
             return False
         count = parts_of(self.nvec, self.dtype)
-        if count is None or max(count, self._m_parts) * self.nvec > FOLD_IN_PLACE:
+        if count is None:
             return False
         need = count * self.nvec
         if self._rr_parts is None or self._rr_parts.numel() < need:
             self._rr_parts = torch.zeros(need, dtype=torch.float64, device=self.r.device)
             nat.bump_generation()
-        self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts)
+        if max(count, self._m_parts) * self.nvec <= FOLD_IN_PLACE:
+            self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts)
+            part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
+                                              dot_w=self.r)
+            self._m_parts = nparts
+            # (exact for any partial count; only slower past FOLD_IN_PLACE)
+            nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
+                     nat.ptr(self.p), nat.ptr(x), self.code, nparts, nat.ptr(part), count,
+                     nat.ptr(self._rr_parts), st, s)
+            return True
+        self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts, passes=1)
+        nat.call("wpc_cg_alpha", self.nvec, count, nat.ptr(self.A.energy_partials(self.nvec)),
+                 st, s)
+        self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts, passes=6, x=x)
         part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                           dot_w=self.r)
         self._m_parts = nparts
-        # (exact for any partial count; only slower past FOLD_IN_PLACE)
-        nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
-                 nat.ptr(x), self.code, nparts, nat.ptr(part), count, nat.ptr(self._rr_parts), st,
-                 s)
+        nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), count,
+                 nat.ptr(self._rr_parts), st, s)
+        nat.call("wpc_cg_direction", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
+                 self.code, st, s)
         return True
 
     def run(self, x, steps, use_graph):

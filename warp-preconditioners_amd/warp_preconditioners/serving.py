@@ -236,6 +236,7 @@ class System:
         self.P._scratch(self.vdtype, n * 8)
         if isinstance(self.inner, SquaredLaplacianOperator):
             self.inner._scratch(self.vdtype, n * 8)
+            self.inner.energy_partials(8)  # the fused CG passes' curvature partials
 
     def capture_stream(self):
         """A stream that only records graphs (never runs work): the capturing

@@ -133,27 +133,36 @@ class SquaredLaplacianOperator(LinearOperator):
             return None
         return nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
 
-    def cg_update(self, p, r, nvec, state, part_rr, passes=3):
+    def energy_partials(self, nvec):
+        """The buffer of the first fused pass's curvature partials
+        (`cg_update_parts` per vector), sized for 8 vectors when first made, so a
+        graph captured for fewer vectors never sees it move."""
+        count = nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
+        if self._part_energy is None or self._part_energy.numel() < count * nvec:
+            self._part_energy = torch.zeros(max(count * max(nvec, 8), 1), dtype=torch.float64,
+                                            device=self.L.device)
+            nat.bump_generation()
+        return self._part_energy
+
+    def cg_update(self, p, r, nvec, state, part_rr, passes=3, x=None):
         """The operator's two passes on p inside fused CG with the residual update
         folded into the second (`wpc_sqlap_cg_update`): r -= ALPHA A p with ALPHA =
         RZ / p.Ap from `state`, A p never stored; leaves `cg_update_parts` r.r
         partials in `part_rr` and the step's scalars in `state`.  `passes` 1 or 2
-        runs one pass alone (timing)."""
+        runs one pass alone (timing); 6 runs the second pass with ALPHA taken from
+        `state` (as wpc_cg_alpha leaves it after folding `energy_partials`) and
+        advances x += ALPHA p."""
         plan = self._stream()
         sp, sl, sc, sv = plan._sliced()
         count = nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
-        if self._part_energy is None or self._part_energy.numel() < count * nvec:
-            self._part_energy = torch.zeros(max(count * nvec, 1), dtype=torch.float64,
-                                            device=self.L.device)
-            nat.bump_generation()
-        part_e = self._part_energy
+        part_e = self.energy_partials(nvec)
         tmp = self._scratch(p.dtype, self.L.nrow * nvec)
         nat.call("wpc_sqlap_cg_update", self.L.nrow, nat.ptr(sp), nat.ptr(sl),
                  nat.ptr(plan._sell[7]), nat.ptr(plan._sell[5]), nat.ptr(sc), int(plan._sell[6]),
                  nat.ptr(sv), nat.code(sv.dtype), nat.ptr(self.inverse_mass),
                  nat.ptr(self._wm) if self.mass_weight != 0.0 else None, nat.ptr(p), nat.ptr(tmp),
                  nat.ptr(r), nat.code(p.dtype), nvec, nat.ptr(part_e), nat.ptr(part_rr),
-                 nat.ptr(state), int(passes), nat.stream())
+                 nat.ptr(state), int(passes), nat.ptr(x), nat.stream())
         return count
 
     def _apply(self, x, y, z, alpha, beta):
