@@ -295,11 +295,14 @@ def kernel_times(s, reps=30):
         pk = pcg  # the fused passes' partials (curvature, then r.r)
     prr = pk if fused else npart  # r.r partials the direction step folds
     if fused:
+        # (x advances in the second K pass: it reads p anyway)
         rr = torch.zeros(pk * k, dtype=torch.float64, device="cuda:0")
+        W = torch.randn_like(X)
         kernels["sqlap_forward_energy"] = lambda: op.cg_update(X, Y, k, state, rr, passes=1)
-        kernels["sqlap_backward_update"] = lambda: op.cg_update(X, Y, k, state, rr, passes=2)
+        kernels["sqlap_backward_update"] = lambda: op.cg_update(X, Y, k, state, rr, passes=2,
+                                                                x=W)
         kernels["cg_direction_fold"] = lambda: nat.call(
-            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), nat.ptr(X), code, pz,
+            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), None, code, pz,
             nat.ptr(pgt.partials(k)), pk, nat.ptr(rr), nat.ptr(state), cur())
     else:
         kernels["sqlap_forward"] = lambda: pl.run(X, None, T, 1.0, 0.0, k, s=op.inverse_mass)
@@ -378,15 +381,17 @@ def kernel_times(s, reps=30):
         # p gathered and t written, 1/mass (and w * mass) read
         "sqlap_forward_energy": nnzK * (vb_k + 4) + idx + 2 * n * k * xb
         + n * vb_k * (2 if op.mass_weight != 0.0 else 1),
-        # t gathered, p read, r read and written, w * mass read, the p.Ap partials
-        "sqlap_backward_update": nnzK * (vb_k + 4) + idx + 4 * n * k * xb
+        # t gathered, p read, r and x read and written, w * mass read, the
+        # curvature partials
+        "sqlap_backward_update": nnzK * (vb_k + 4) + idx + 6 * n * k * xb
         + (n * vb_k if op.mass_weight != 0.0 else 0) + 8 * pk * k,
         "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
         "cg_update_fold": (3 if fold_b else 6) * n * k * xb + 8 * pk * k,
         "cg_beta": 8 * (pz + npart) * k,
         "cg_direction": 3 * n * k * xb,
-        "cg_direction_fold": 5 * n * k * xb + 8 * (pz + prr) * k,
+        # z, p read, p written (+ x read and written where x advances here)
+        "cg_direction_fold": (3 if fused else 5) * n * k * xb + 8 * (pz + prr) * k,
     }
     return out, {name: bytes_[name] for name in out}, tuned
 

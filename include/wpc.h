@@ -508,9 +508,10 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int 
  * NULL for w = 0; L symmetric).  Pass 2: Ap = L t + W p row by row (not stored),
  * ALPHA = RZ / (fold of the pass-1 partials) for slots with RR > TARGET (else 0),
  * r -= ALPHA Ap, per-workgroup partials of r.r; workgroup 0 writes RZ_OLD, DENOM,
- * ACTIVE, ALPHA as wpc_cg_update_fold does.  The direction step then folds the
- * wpc_sqlap_cg_partials(n) r.r partials from part_rr and advances x
- * (wpc_cg_direction_fold with x).  The sliced copy of L (rows in natural or given
+ * ACTIVE, ALPHA as wpc_cg_update_fold does; with x non-NULL pass 2 also advances
+ * x += ALPHA p (it reads p anyway) and the direction step folds the
+ * wpc_sqlap_cg_partials(n) r.r partials from part_rr with x = NULL (else
+ * wpc_cg_direction_fold advances x).  The sliced copy of L (rows in natural or given
  * order) as for wpc_sell_stream; values, inverse_mass, weighted_mass and vectors
  * all F64 or all F32; nvec 1..8; part_energy and part_rr hold
  * wpc_sqlap_cg_partials(n) * nvec doubles each (the energy partials are folded in
@@ -519,9 +520,9 @@ int wpc_cg_direction_fold(int n, int nvec, const void* z, void* p, void* x, int 
  * 1 or 2 runs one of them (for timing each kernel alone).  Bit 2 (passes 6 or 7),
  * for partial counts too large to fold in every workgroup: pass 2 takes ALPHA as
  * wpc_cg_alpha left it in the state after folding the pass-1 partials (no fold,
- * no state writes) and also advances x += ALPHA p, as wpc_cg_update does; the
- * step then ends with wpc_cg_beta (these r.r partials) and wpc_cg_direction.  x
- * is read only with bit 2. */
+ * no state writes) and advances x += ALPHA p (x required), as wpc_cg_update does;
+ * the step then ends with wpc_cg_beta (these r.r partials) and wpc_cg_direction.
+ * x, when given, must not alias p, t or r. */
 int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
                         const int* slice_base, const int* rows, const void* sell_columns,
                         int col16, const void* sell_values, int val_dtype,

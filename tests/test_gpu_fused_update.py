@@ -209,3 +209,39 @@ def test_energy_partials_never_move_for_up_to_8_vectors():
     for nvec in range(1, 9):
         assert A.energy_partials(nvec).data_ptr() == first.data_ptr()
     assert nat.generation() == after and after >= gen
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("weight", [0.0, 1.0])
+def test_second_pass_advances_x(dtype, weight):
+    """With x the second pass also applies x += ALPHA p (in the vector type, only
+    for stepping slots), leaving r and the state exactly as without x."""
+    nvec = 3
+    n, A, _ = _operator(29, 4, dtype, weight)
+    rng = np.random.default_rng(12)
+    p = _vec(rng.normal(size=n * nvec), dtype)
+    r = _vec(rng.normal(size=n * nvec), dtype)
+    x0 = _vec(rng.normal(size=n * nvec), dtype)
+    count = A.cg_update_parts(nvec, dtype)
+    outs = []
+    for with_x in (False, True):
+        rr = torch.zeros(count * nvec, dtype=torch.float64, device=dev())
+        st, rv, xv = _state(nvec, 1), r.clone(), x0.clone()
+        A.cg_update(p, rv, nvec, st, rr, x=xv if with_x else None)
+        torch.cuda.synchronize()
+        outs.append((host(rv), host(st), host(xv), host(rr)))
+    for a, b in zip(outs[0][:2], outs[1][:2]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(outs[0][3], outs[1][3])
+    np.testing.assert_array_equal(outs[0][2], host(x0))  # untouched without x
+    F = outs[1][1].size // nvec
+    alpha = outs[1][1].reshape(F, nvec)[3]
+    T = NP[dtype]
+    want = host(x0).reshape(n, nvec).copy()
+    ph = host(p).reshape(n, nvec)
+    for v in range(nvec):
+        a = T(alpha[v])
+        if a != 0:
+            want[:, v] = want[:, v] + a * ph[:, v]
+    np.testing.assert_array_equal(outs[1][2].reshape(n, nvec), want)
+    assert alpha[1] == 0.0  # the resting slot's x is unchanged

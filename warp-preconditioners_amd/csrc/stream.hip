@@ -88,7 +88,8 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
         part_rr == nullptr || state == nullptr)
         return WPC_EARG;
     if (p == t || p == r || t == r) return WPC_EARG;
-    if ((passes & 4) && (x == nullptr || x == p || x == t || x == r)) return WPC_EARG;
+    if ((passes & 4) && x == nullptr) return WPC_EARG;
+    if (x != nullptr && (x == p || x == t || x == r)) return WPC_EARG;
     const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
                         sell_values, val_dtype, inverse_mass, weighted_mass, p, t, r, vec_dtype,
                         part_energy, part_rr, state, passes, x, as_stream(stream)};

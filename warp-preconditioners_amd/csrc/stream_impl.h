@@ -767,6 +767,13 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
         fold.load(part_e, P);
 #pragma unroll
         for (int v = 0; v < NV; ++v) rv[v] = r[(long)h.row * NV + v];
+        if (x != nullptr) {  // (uniform) x += ALPHA p here as well
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                xv[v] = x[(long)h.row * NV + v];
+                px[v] = HW ? (TX)pv[v] : p[(long)h.row * NV + v];
+            }
+        }
         fold.finish(part_e, P, sums);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -799,7 +806,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     if (h.live) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) r[(long)h.row * NV + v] = rn[v];
-        if constexpr (FS) {
+        if (FS || x != nullptr) {
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 if (alpha[v] != TX(0)) x[(long)h.row * NV + v] = xv[v] + alpha[v] * px[v];

@@ -331,8 +331,8 @@ class KrylovWork:
     def _fused_update_step(self, x, s, st):
         """One CG step with the update folded into the operator's last pass
         (operators with `cg_update`: the factored squared Laplacian) -- five launches
-        on the headline path instead of six: A p with the update (+r.r partials) in
-        two passes | M r (+r.z partials) in two | direction (+x).  With partial
+        on the headline path instead of six: A p with the update and x += ALPHA p
+        (+r.r partials) in two passes | M r (+r.z partials) in two | direction.  With partial
         counts too large to fold in every workgroup, the curvature and the r.z / r.r
         folds run as single-workgroup launches instead (the first pass | ALPHA |
         the second pass with the update and x | M r | BETA | direction: six launches
@@ -351,13 +351,14 @@ class KrylovWork:
             self._rr_parts = torch.zeros(need, dtype=torch.float64, device=self.r.device)
             nat.bump_generation()
         if max(count, self._m_parts) * self.nvec <= FOLD_IN_PLACE:
-            self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts)
+            # (x += ALPHA p in the second pass, which reads p anyway)
+            self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts, x=x)
             part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                               dot_w=self.r)
             self._m_parts = nparts
             # (exact for any partial count; only slower past FOLD_IN_PLACE)
             nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
-                     nat.ptr(self.p), nat.ptr(x), self.code, nparts, nat.ptr(part), count,
+                     nat.ptr(self.p), None, self.code, nparts, nat.ptr(part), count,
                      nat.ptr(self._rr_parts), st, s)
             return True
         self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts, passes=1)

@@ -148,10 +148,10 @@ class SquaredLaplacianOperator(LinearOperator):
         """The operator's two passes on p inside fused CG with the residual update
         folded into the second (`wpc_sqlap_cg_update`): r -= ALPHA A p with ALPHA =
         RZ / p.Ap from `state`, A p never stored; leaves `cg_update_parts` r.r
-        partials in `part_rr` and the step's scalars in `state`.  `passes` 1 or 2
-        runs one pass alone (timing); 6 runs the second pass with ALPHA taken from
-        `state` (as wpc_cg_alpha leaves it after folding `energy_partials`) and
-        advances x += ALPHA p."""
+        partials in `part_rr` and the step's scalars in `state`; with `x` the
+        second pass also advances x += ALPHA p.  `passes` 1 or 2 runs one pass alone
+        (timing); 6 runs the second pass with ALPHA taken from `state` (as
+        wpc_cg_alpha leaves it after folding `energy_partials`), x required."""
         plan = self._stream()
         sp, sl, sc, sv = plan._sliced()
         count = nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
