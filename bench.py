@@ -298,12 +298,24 @@ def kernel_times(s, reps=30):
         # (x advances in the second K pass: it reads p anyway)
         rr = torch.zeros(pk * k, dtype=torch.float64, device="cuda:0")
         W = torch.randn_like(X)
-        kernels["sqlap_forward_energy"] = lambda: op.cg_update(X, Y, k, state, rr, passes=1)
+        if not linear.CG_RECURRENCE:
+            kernels["sqlap_forward_energy"] = lambda: op.cg_update(X, Y, k, state, rr,
+                                                                   passes=1)
         kernels["sqlap_backward_update"] = lambda: op.cg_update(X, Y, k, state, rr, passes=2,
                                                                 x=W)
-        kernels["cg_direction_fold"] = lambda: nat.call(
-            "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), None, code, pz,
-            nat.ptr(pgt.partials(k)), pk, nat.ptr(rr), nat.ptr(state), cur())
+        if linear.CG_RECURRENCE:
+            # the first pass with the previous step's direction step folded in (every
+            # step of a run but the first; timed last: it writes the state).  RZ_OLD
+            # is huge so that BETA is tiny and p, t stay finite over the replays.
+            st_rec = state.clone()
+            st_rec[6 * k:7 * k] = 1.0e30
+            p_rec, t_rec = torch.randn_like(X), torch.randn_like(X)
+            kernels["sqlap_direction_energy"] = lambda: op.direction_energy(
+                T, p_rec, t_rec, k, pz, pgt.partials(k), rr, st_rec)
+        else:
+            kernels["cg_direction_fold"] = lambda: nat.call(
+                "wpc_cg_direction_fold", n, k, nat.ptr(T), nat.ptr(Y), None, code, pz,
+                nat.ptr(pgt.partials(k)), pk, nat.ptr(rr), nat.ptr(state), cur())
     else:
         kernels["sqlap_forward"] = lambda: pl.run(X, None, T, 1.0, 0.0, k, s=op.inverse_mass)
         kernels["sqlap_backward_dot"] = lambda: pl.run(T, None, Y, 1.0, 0.0, k, d=op._wm, u=X,
@@ -385,6 +397,10 @@ def kernel_times(s, reps=30):
         # curvature partials
         "sqlap_backward_update": nnzK * (vb_k + 4) + idx + 6 * n * k * xb
         + (n * vb_k if op.mass_weight != 0.0 else 0) + 8 * pk * k,
+        # z gathered (its own rows too), p and t read and written, 1/mass (and
+        # w * mass) read, the r.z and r.r partials
+        "sqlap_direction_energy": nnzK * (vb_k + 4) + idx + 5 * n * k * xb
+        + n * vb_k * (2 if op.mass_weight != 0.0 else 1) + 8 * (pz + pk) * k,
         "cg_alpha": 8 * pk * k,
         "cg_update": 6 * n * k * xb,
         "cg_update_fold": (3 if fold_b else 6) * n * k * xb + 8 * pk * k,

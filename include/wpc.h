@@ -534,6 +534,29 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
  * workgroup: ceil(n / 512)).  Host-only helper; no reference counterpart. */
 int wpc_sqlap_cg_partials(int n);
 
+/* No reference counterpart (the reference's solver runs the direction step of
+ * pcg_solver.py on its own): the first pass of wpc_sqlap_cg_update with the
+ * previous step's direction step folded in, for the steps of a CG run between the
+ * first and the last, whose direction step it replaces.  Folds nparts_rz r.z
+ * partials (part_rz, the preconditioner's last pass) and the
+ * wpc_sqlap_cg_partials(n) r.r partials that pass 2 left (part_rr) into BETA and
+ * writes the state as wpc_cg_direction_fold does (x is advanced by pass 2, not
+ * here); then, per stepping slot, p = z + BETA p and t = inverse_mass .* (L z) +
+ * BETA t, where t is what the previous first pass (either form) left; resting
+ * slots keep p and t.  Leaves pass 2's curvature partials in part_energy, from
+ * (L p)_i = t_i / inverse_mass_i.  Pass 2 (wpc_sqlap_cg_update, passes = 2) then
+ * reads this t.  t is the carried vector and differs from inverse_mass .* (L p)
+ * by the recurrence's rounding; z, p and t must not alias.  Other arguments as
+ * for wpc_sqlap_cg_update. */
+int wpc_sqlap_cg_direction_energy(int n, const int* slice_ptr, const int* slice_len,
+                                  const int* slice_base, const int* rows,
+                                  const void* sell_columns, int col16, const void* sell_values,
+                                  int val_dtype, const void* inverse_mass,
+                                  const void* weighted_mass, const void* z, void* p, void* t,
+                                  int vec_dtype, int nvec, int nparts_rz, const double* part_rz,
+                                  const double* part_rr, double* part_energy, double* state,
+                                  void* stream);
+
 /* ------------------------------------------------------------------ grouped CG steps
  * No reference counterpart: a serving extension that advances the fused CG of
  * several systems (uneven sizes and right-hand-side counts, same vector format

@@ -128,3 +128,69 @@ def test_graph_replay_keyed_on_scratch_generation(monkeypatch):
     assert captured[-1] == "replay" and captured.count("capture") == 2
     w.run(torch.zeros(8, dtype=torch.float64), 25, True)  # another x: recapture
     assert captured.count("capture") == 3
+
+
+class _LogOp(_Op):
+    """The factored operator's fused pair, recorded (in-place partial counts)."""
+
+    def __init__(self, parts, log, tag):
+        super().__init__(parts)
+        self.log, self.tag = log, tag
+
+    def apply_batch(self, x, y, z, alpha, beta, nvec, dot_w=None, dot_out=None):
+        self.log.append((self.tag, ()))
+        return super().apply_batch(x, y, z, alpha, beta, nvec, dot_w, dot_out)
+
+    def cg_update_parts(self, nvec, dtype):
+        return self.parts
+
+    def cg_update(self, p, r, nvec, state, part_rr, passes=3, x=None, t=None):
+        self.log.append(("cg_update", (passes, x is not None, t)))
+        return self.parts
+
+    def direction_energy(self, z, p, t, nvec, nparts_rz, part_rz, part_rr, state):
+        self.log.append(("direction_energy", (t, nparts_rz)))
+
+    def energy_partials(self, nvec):
+        return torch.zeros(self.parts * 8, dtype=torch.float64)
+
+
+@pytest.mark.parametrize("recurrence", [True, False])
+def test_recurrence_steps_defer_the_direction_step_inside_a_run(launches, recurrence):
+    nvec = 3
+    nat.lib()
+    A, M = _LogOp(512, launches, "A"), _LogOp(1024, launches, "M")
+    w = KrylovWork("cg", A, M, 262144, nvec, torch.float64, "cpu")
+    w.recurrence = recurrence
+    x = torch.zeros(w.n * nvec, dtype=torch.float64)
+    w.steps(x, 1)  # the first step: the general sequence (the partial count is learnt)
+    assert [n for n, _ in launches] == ["A", "wpc_cg_update_fold", "M", "wpc_cg_direction_fold"]
+    del launches[:]
+    w.steps(x, 4)
+    names = [n for n, _ in launches]
+    if recurrence:
+        assert names == (["cg_update", "M"] + ["direction_energy", "cg_update", "M"] * 3
+                         + ["wpc_cg_direction_fold"])
+        ups = [a for n, a in launches if n == "cg_update"]
+        assert [u[0] for u in ups] == [3, 2, 2, 2]          # both passes, then pass 2 alone
+        assert all(u[1] and u[2] is w.ap for u in ups)       # x advanced; t carried in ap
+        assert all(a == (w.ap, 1024) for n, a in launches if n == "direction_energy")
+        fold = launches[-1][1]
+        assert fold[4] is None and fold[6] == 1024 and fold[8] == 512
+    else:
+        assert names == ["cg_update", "M", "wpc_cg_direction_fold"] * 4
+    # a run of one step is one whole step
+    del launches[:]
+    w.steps(x, 1)
+    assert [n for n, _ in launches] == ["cg_update", "M", "wpc_cg_direction_fold"]
+
+
+def test_recurrence_off_for_counts_past_the_in_place_fold(launches):
+    nvec = 3
+    nat.lib()
+    A = _LogOp(FOLD_IN_PLACE // nvec + 1, launches, "A")
+    w = KrylovWork("cg", A, _LogOp(64, launches, "M"), 262144, nvec, torch.float64, "cpu")
+    x = torch.zeros(w.n * nvec, dtype=torch.float64)
+    w.steps(x, 3)
+    names = [n for n, _ in launches]
+    assert "direction_energy" not in names and names.count("wpc_cg_direction") == 3

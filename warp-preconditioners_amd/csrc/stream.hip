@@ -6,6 +6,7 @@
 using namespace wpc;
 
 static int stream_dispatch(int nvec, const StreamArgs& a);
+static int sqlap_cg_dispatch(int nvec, const SqlapCgArgs& a);
 
 extern "C" {
 
@@ -92,18 +93,31 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
     if (x != nullptr && (x == p || x == t || x == r)) return WPC_EARG;
     const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
                         sell_values, val_dtype, inverse_mass, weighted_mass, p, t, r, vec_dtype,
-                        part_energy, part_rr, state, passes, x, as_stream(stream)};
-    switch (nvec) {
-        case 1: return sqlap_cg_entry_1(a);
-        case 2: return sqlap_cg_entry_2(a);
-        case 3: return sqlap_cg_entry_3(a);
-        case 4: return sqlap_cg_entry_4(a);
-        case 5: return sqlap_cg_entry_5(a);
-        case 6: return sqlap_cg_entry_6(a);
-        case 7: return sqlap_cg_entry_7(a);
-        case 8: return sqlap_cg_entry_8(a);
-        default: return WPC_EARG;
-    }
+                        part_energy, part_rr, state, passes, x, nullptr, 0, nullptr,
+                        as_stream(stream)};
+    return sqlap_cg_dispatch(nvec, a);
+}
+
+int wpc_sqlap_cg_direction_energy(int n, const int* slice_ptr, const int* slice_len,
+                                  const int* slice_base, const int* rows,
+                                  const void* sell_columns, int col16, const void* sell_values,
+                                  int val_dtype, const void* inverse_mass,
+                                  const void* weighted_mass, const void* z, void* p, void* t,
+                                  int vec_dtype, int nvec, int nparts_rz, const double* part_rz,
+                                  const double* part_rr, double* part_energy, double* state,
+                                  void* stream) {
+    if (n <= 0) return WPC_OK;
+    if (slice_ptr == nullptr || slice_len == nullptr || slice_base == nullptr ||
+        sell_columns == nullptr || sell_values == nullptr || inverse_mass == nullptr ||
+        z == nullptr || p == nullptr || t == nullptr || part_rz == nullptr ||
+        part_rr == nullptr || part_energy == nullptr || state == nullptr || nparts_rz <= 0)
+        return WPC_EARG;
+    if (z == p || z == t || p == t) return WPC_EARG;
+    const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
+                        sell_values, val_dtype, inverse_mass, weighted_mass, p, t, nullptr,
+                        vec_dtype, part_energy, const_cast<double*>(part_rr), state, 1 | 8,
+                        nullptr, z, nparts_rz, part_rz, as_stream(stream)};
+    return sqlap_cg_dispatch(nvec, a);
 }
 
 int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int val_dtype,
@@ -119,6 +133,20 @@ int wpc_sell_stream_grouped(int ntasks, const void* tasks, int total_blocks, int
 }
 
 }  // extern "C"
+
+static int sqlap_cg_dispatch(int nvec, const SqlapCgArgs& a) {
+    switch (nvec) {
+        case 1: return sqlap_cg_entry_1(a);
+        case 2: return sqlap_cg_entry_2(a);
+        case 3: return sqlap_cg_entry_3(a);
+        case 4: return sqlap_cg_entry_4(a);
+        case 5: return sqlap_cg_entry_5(a);
+        case 6: return sqlap_cg_entry_6(a);
+        case 7: return sqlap_cg_entry_7(a);
+        case 8: return sqlap_cg_entry_8(a);
+        default: return WPC_EARG;
+    }
+}
 
 static int stream_dispatch(int nvec, const StreamArgs& a) {
     switch (nvec) {

@@ -126,8 +126,12 @@ struct SqlapCgArgs {
     double* part_e;
     double* part_rr;
     double* state;
-    int passes;  // bit 0: the first pass, bit 1: the second, bit 2: ALPHA from the state
-    void* x;     // advanced by the second pass with bit 2
+    int passes;  // bit 0: the first pass, bit 1: the second, bit 2: ALPHA from the state,
+                 // bit 3: the first pass in recurrence form (gathers z, updates p and t)
+    void* x;     // advanced by the second pass (required with bit 2)
+    const void* z;          // bit 3: the preconditioned residual
+    int P_rz;               // bit 3: count of the preconditioner's r.z partials
+    const double* part_rz;  // bit 3: those partials
     hipStream_t st;
 };
 
@@ -712,6 +716,107 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     write_partials<NV>(e, partials, blk);
 }
 
+// Pass 1 in recurrence form, with the previous step's direction step folded in:
+// p' = z + BETA p and t' = M^-1 L p' = M^-1 L z + BETA t (t as the previous step's
+// first pass or this kernel left it), so the pass gathers z instead of p and the
+// direction launch leaves the iteration.  Every workgroup folds the r.z partials
+// of the preconditioner's last pass and the r.r partials of pass 2 into BETA =
+// RZ_NEW / RZ_OLD exactly as the direction step does, and workgroup 0 records
+// RZ_NEW, RR_NEW, BETA and, for stepping slots (ACTIVE), RZ, RR and ITER += 1; a
+// resting slot keeps p and t.  Curvature partials as pass 1, (L p')_i t'_i +
+// W_i p'_i^2 with (L p')_i = t'_i / s_i.
+template <class TV, class TX, int NV, bool HW>
+__global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
+    int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
+    const int* __restrict__ slice_base, const int* __restrict__ rows,
+    const void* __restrict__ col, int col16, const TV* __restrict__ val,
+    const TX* __restrict__ s, const TX* __restrict__ wm, const TX* __restrict__ z,
+    TX* __restrict__ p, TX* __restrict__ t, int P_rz, const double* __restrict__ part_rz,
+    int P_rr, const double* __restrict__ part_rr, double* state,
+    double* __restrict__ partials) {
+    using TA = typename AccOf<TX>::type;
+    __shared__ double red[kStreamRows];
+    __shared__ double s_rz[8], s_rr[8];
+    const int blk = xcd_block();
+    __amdgpu_buffer_rsrc_t xr;
+    const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len, slice_base,
+                                                      rows, col16 != 0, z, &xr);
+    const unsigned stride = (unsigned)(NV * sizeof(TX));
+    const TA sv = St<TX>::template ld<TA>(s, h.row);
+    TA wv = TA(0);
+    if (HW) wv = St<TX>::template ld<TA>(wm, h.row);
+    TA acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = TA(0);
+    if (col16)
+        sell_accum<unsigned short, TV, TX, NV>(acc, h.base, h.len, xr, stride,
+                                               (const unsigned short*)col, val, NV);
+    else
+        sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
+    // After the gathers: the state fields, both partial sets and the row's z, p and
+    // t, in one round trip.
+    StateLanes<WPC_KS_ACTIVE, WPC_KS_RZ_OLD> st;
+    st.load(state, NV);
+    FoldRegs<kSqlapCgRows, NV> fold_z, fold_r;
+    fold_z.load(part_rz, P_rz);
+    fold_r.load(part_rr, P_rr);
+    TX zv[NV], pv[NV], tv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        zv[v] = z[(long)h.row * NV + v];
+        pv[v] = p[(long)h.row * NV + v];
+        tv[v] = t[(long)h.row * NV + v];
+    }
+    if constexpr (NV <= 4) {
+        // both folds through one cross-wavefront pass
+        double both_acc[8];
+        __shared__ double both[8];
+        fold_z.sums(part_rz, P_rz, both_acc);
+        fold_r.sums(part_rr, P_rr, both_acc + NV);
+        fold_reduce<kSqlapCgRows>(both_acc, 2 * NV, both);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { s_rz[v] = both[v]; s_rr[v] = both[NV + v]; }
+    } else {
+        fold_z.finish(part_rz, P_rz, s_rz);
+        fold_r.finish(part_rr, P_rr, s_rr);
+    }
+    double e[NV];
+    TX pn[NV], tn[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const bool act = st.get(0, NV, v) != 0.0;
+        const double b = act ? s_rz[v] / st.get(1, NV, v) : 0.0;
+        const TX beta = (TX)b;
+        if (blk == 0 && threadIdx.x == v) {
+            state[WPC_KS_RZ_NEW * NV + v] = s_rz[v];
+            state[WPC_KS_RR_NEW * NV + v] = s_rr[v];
+            state[WPC_KS_BETA * NV + v] = b;
+            if (act) {
+                state[WPC_KS_RZ * NV + v] = s_rz[v];
+                state[WPC_KS_RR * NV + v] = s_rr[v];
+                state[WPC_KS_ITER * NV + v] += 1.0;
+            }
+        }
+        TX tz[1];
+        St<TX>::template st<TA>(tz, 0, sv * acc[v]);  // M^-1 L z as the plain pass rounds it
+        pn[v] = act ? zv[v] + beta * pv[v] : pv[v];
+        tn[v] = act ? tz[0] + beta * tv[v] : tv[v];
+        double ev = (double)tn[v] * (double)tn[v] / (double)sv;
+        if (HW) ev += (double)wv * ((double)pn[v] * (double)pn[v]);
+        e[v] = h.live ? ev : 0.0;
+        asm volatile("" : "+v"(e[v]));
+    }
+    if (h.live) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            p[(long)h.row * NV + v] = pn[v];
+            t[(long)h.row * NV + v] = tn[v];
+        }
+    }
+    block_sum<NV, kSqlapCgRows>(e, red);
+    write_partials<NV>(e, partials, blk);
+}
+
 // Pass 2: Ap = L t + W p (not stored); ALPHA from the P pass-1 partials;
 // r -= ALPHA Ap for stepping slots; partials of r.r.  FS (for partial counts too
 // large to fold in every workgroup): ALPHA as the scalar fold wpc_cg_alpha left it
@@ -997,7 +1102,13 @@ int sqlap_cg_launch(const SqlapCgArgs& a) {
     const int grid = (int)blocks_for(a.n, kSqlapCgRows);  // each pass's partial count
 #define SC(HW, FS)                                                                              \
     do {                                                                                        \
-        if (a.passes & 1)                                                                       \
+        if ((a.passes & 1) && (a.passes & 8))                                                   \
+        hipLaunchKernelGGL((sell_direction_energy_kernel<TV, TX, NV, HW>), dim3(grid),          \
+                           dim3(kSqlapCgRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
+                           a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
+                           (const TX*)a.s, (const TX*)a.wm, (const TX*)a.z, (TX*)a.p, (TX*)a.t, \
+                           a.P_rz, a.part_rz, grid, a.part_rr, a.state, a.part_e);              \
+        else if (a.passes & 1)                                                                  \
         hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW>), dim3(grid), dim3(kSqlapCgRows), \
                            0, a.st, a.n, a.slice_ptr, a.slice_len, a.slice_base, a.rows, a.col, \
                            (int)a.col16, (const TV*)a.val, (const TX*)a.s, (const TX*)a.wm,     \

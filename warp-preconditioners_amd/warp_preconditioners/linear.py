@@ -192,6 +192,11 @@ FOLD_IN_PLACE = 4096
 # (SquaredLaplacianOperator.cg_update); False keeps the separate update launch.  Read
 # when a KrylovWork is made (its `fuse_update`).
 FUSE_OPERATOR_UPDATE = True
+# Inside a run of fused CG steps (KrylovWork.steps), fold each step's direction
+# step into the next step's first operator pass (recurrence form: four launches
+# per step instead of five); the run's last step still ends with the direction
+# step, so the state between runs is the same.  Read when a KrylovWork is made.
+CG_RECURRENCE = True
 
 
 class KrylovWork:
@@ -222,6 +227,8 @@ class KrylovWork:
         # Off for works that also advance in grouped launches (grouped.py), whose
         # batched kernels keep the separate update: both then take the same steps.
         self.fuse_update = FUSE_OPERATOR_UPDATE
+        self.recurrence = CG_RECURRENCE
+        self._m_part = None  # the preconditioner's r.z partials of its last pass
         self.use_fused = (self.method == 0 and M is not None and bool(getattr(A, "fused", False))
                           and bool(getattr(M, "fused", False)) and hasattr(A, "apply_batch")
                           and hasattr(M, "apply_batch"))
@@ -293,7 +300,7 @@ class KrylovWork:
                          nat.ptr(self.p), nat.ptr(self.ap), self.code, st, pr, s)
             part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                               dot_w=self.r)
-            self._m_parts = nparts
+            self._m_parts, self._m_part = nparts, part
             if xd or (fold_a and max(nparts, self.P) * self.nvec <= FOLD_IN_PLACE):
                 nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
                          nat.ptr(self.p), nat.ptr(x) if xd else None, self.code, nparts,
@@ -328,6 +335,68 @@ class KrylovWork:
             nat.call("wpc_krylov_direction", 1, self.n, self.nvec, nat.ptr(self.z),
                      nat.ptr(self.p), nat.ptr(self.az), nat.ptr(self.ap), self.code, st, pz, pr, s)
 
+    def steps(self, x, count):
+        """`count` CG steps.  Where the fused step applies in place (see
+        `_fused_update_step`) and `recurrence` is set, a step other than the last
+        leaves its direction step to the next step's first operator pass
+        (`direction_energy`), which also carries t = M^-1 L p in `ap` across steps;
+        the last step ends with the direction step, so between calls p and the
+        state are those of `count` calls of `step`."""
+        pending = False
+        for i in range(count):
+            if not (self.recurrence and self._in_place_parts() is not None):
+                if pending:  # (cannot happen: eligibility never ends once it begins)
+                    self._direction_fold(self._in_place_parts())
+                    pending = False
+                self.step(x)
+                continue
+            self._recurrence_step(x, pending, i == count - 1)
+            pending = i != count - 1
+
+    def _in_place_parts(self):
+        """The operator's partial count when the fused step runs its in-place form
+        (every fold repeated by each workgroup), else None."""
+        if not self.use_fused or not self.fuse_update or self._m_parts is None:
+            return None
+        parts_of = getattr(self.A, "cg_update_parts", None)
+        count = None if parts_of is None else parts_of(self.nvec, self.dtype)
+        if count is None or max(count, self._m_parts) * self.nvec > FOLD_IN_PLACE:
+            return None
+        return count
+
+    def _rr_buffer(self, count):
+        need = count * self.nvec
+        if self._rr_parts is None or self._rr_parts.numel() < need:
+            self._rr_parts = torch.zeros(need, dtype=torch.float64, device=self.r.device)
+            nat.bump_generation()
+        return self._rr_parts
+
+    def _direction_fold(self, count):
+        nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
+                 nat.ptr(self.p), None, self.code, self._m_parts, nat.ptr(self._m_part), count,
+                 nat.ptr(self._rr_parts), nat.ptr(self.state), nat.stream())
+
+    def _recurrence_step(self, x, pending, last):
+        """One in-place fused step with t in `ap`: the first pass gathers z and takes
+        the previous step's direction step when `pending`, else it is the plain
+        first pass on p; the direction step runs only when `last`."""
+        count = self._in_place_parts()
+        rr = self._rr_buffer(count)
+        if pending:
+            self.A.direction_energy(self.z, self.p, self.ap, self.nvec, self._m_parts,
+                                    self._m_part, rr, self.state)
+            self.A.cg_update(self.p, self.r, self.nvec, self.state, rr, passes=2, x=x,
+                             t=self.ap)
+        else:
+            self.A.cg_update(self.p, self.r, self.nvec, self.state, rr, x=x, t=self.ap)
+        part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
+                                          dot_w=self.r)
+        if nparts != self._m_parts:
+            raise RuntimeError("the preconditioner's partial count changed inside a run")
+        self._m_part = part
+        if last:
+            self._direction_fold(count)
+
     def _fused_update_step(self, x, s, st):
         """One CG step with the update folded into the operator's last pass
         (operators with `cg_update`: the factored squared Laplacian) -- five launches
@@ -355,7 +424,7 @@ class KrylovWork:
             self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts, x=x)
             part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                               dot_w=self.r)
-            self._m_parts = nparts
+            self._m_parts, self._m_part = nparts, part
             # (exact for any partial count; only slower past FOLD_IN_PLACE)
             nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
                      nat.ptr(self.p), None, self.code, nparts, nat.ptr(part), count,
@@ -367,7 +436,7 @@ class KrylovWork:
         self.A.cg_update(self.p, self.r, self.nvec, self.state, self._rr_parts, passes=6, x=x)
         part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                           dot_w=self.r)
-        self._m_parts = nparts
+        self._m_parts, self._m_part = nparts, part
         nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), count,
                  nat.ptr(self._rr_parts), st, s)
         nat.call("wpc_cg_direction", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
@@ -376,8 +445,7 @@ class KrylovWork:
 
     def run(self, x, steps, use_graph):
         if not use_graph or steps <= 0:
-            for _ in range(steps):
-                self.step(x)
+            self.steps(x, steps)
             return
         # The graph bakes in x and the operators' own scratch (product partials,
         # two-pass temporaries); it is replayed only while none of them moved.
@@ -387,15 +455,13 @@ class KrylovWork:
             self.graph = None
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                for _ in range(steps):
-                    self.step(x)
+                self.steps(x, steps)
             self.graph, self.graph_steps = g, steps
             # (capture itself may size a buffer: key on the generation after it)
             self._graph_key = (steps, x.data_ptr(), nat.generation())
             if self._graph_key != key:
                 self.graph = None  # recorded against a buffer that has since moved
-                for _ in range(steps):
-                    self.step(x)
+                self.steps(x, steps)
                 return
         self.graph.replay()
 

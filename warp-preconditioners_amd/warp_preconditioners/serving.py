@@ -306,8 +306,7 @@ class Slot:
             nat.call("wpc_graph_launch", self.graph, nat.stream())
             return
         if not self.warm:
-            for _ in range(steps):
-                self.work.step(self.x_iter)
+            self.work.steps(self.x_iter, steps)
             self.warm = True
             return
         # recorded on a private stream (no pending cross-stream waits), launched here
@@ -317,8 +316,7 @@ class Slot:
         with torch.cuda.stream(self.system.capture_stream()):
             nat.call("wpc_graph_begin", nat.stream())
             try:
-                for _ in range(steps):
-                    self.work.step(self.x_iter)
+                self.work.steps(self.x_iter, steps)
             except nat.NativeError:
                 recorded = False
             finally:
@@ -327,8 +325,7 @@ class Slot:
             # invalidated from outside: nothing recorded ran; run this chunk directly
             if rc == 0 and handle.value:
                 nat.lib().wpc_graph_destroy(handle.value)
-            for _ in range(steps):
-                self.work.step(self.x_iter)
+            self.work.steps(self.x_iter, steps)
             return
         if self.graph is not None:
             nat.lib().wpc_graph_destroy(self.graph)

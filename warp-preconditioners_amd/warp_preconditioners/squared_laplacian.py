@@ -144,19 +144,21 @@ class SquaredLaplacianOperator(LinearOperator):
             nat.bump_generation()
         return self._part_energy
 
-    def cg_update(self, p, r, nvec, state, part_rr, passes=3, x=None):
+    def cg_update(self, p, r, nvec, state, part_rr, passes=3, x=None, t=None):
         """The operator's two passes on p inside fused CG with the residual update
         folded into the second (`wpc_sqlap_cg_update`): r -= ALPHA A p with ALPHA =
         RZ / p.Ap from `state`, A p never stored; leaves `cg_update_parts` r.r
         partials in `part_rr` and the step's scalars in `state`; with `x` the
         second pass also advances x += ALPHA p.  `passes` 1 or 2 runs one pass alone
-        (timing); 6 runs the second pass with ALPHA taken from `state` (as
-        wpc_cg_alpha leaves it after folding `energy_partials`), x required."""
+        (timing; 2 also after `direction_energy`); 6 runs the second pass with ALPHA
+        taken from `state` (as wpc_cg_alpha leaves it after folding
+        `energy_partials`), x required.  `t` holds M^-1 L p between the passes (the
+        operator's scratch when None)."""
         plan = self._stream()
         sp, sl, sc, sv = plan._sliced()
         count = nat.lib().wpc_sqlap_cg_partials(self.L.nrow)
         part_e = self.energy_partials(nvec)
-        tmp = self._scratch(p.dtype, self.L.nrow * nvec)
+        tmp = self._scratch(p.dtype, self.L.nrow * nvec) if t is None else t
         nat.call("wpc_sqlap_cg_update", self.L.nrow, nat.ptr(sp), nat.ptr(sl),
                  nat.ptr(plan._sell[7]), nat.ptr(plan._sell[5]), nat.ptr(sc), int(plan._sell[6]),
                  nat.ptr(sv), nat.code(sv.dtype), nat.ptr(self.inverse_mass),
@@ -164,6 +166,21 @@ class SquaredLaplacianOperator(LinearOperator):
                  nat.ptr(r), nat.code(p.dtype), nvec, nat.ptr(part_e), nat.ptr(part_rr),
                  nat.ptr(state), int(passes), nat.ptr(x), nat.stream())
         return count
+
+    def direction_energy(self, z, p, t, nvec, nparts_rz, part_rz, part_rr, state):
+        """The previous CG step's direction step folded into this step's first pass
+        (`wpc_sqlap_cg_direction_energy`): BETA from the r.z and r.r partials,
+        p = z + BETA p, t = M^-1 L z + BETA t (t as the previous first pass left
+        it), curvature partials for `cg_update(..., passes=2, t=t)`."""
+        plan = self._stream()
+        sp, sl, sc, sv = plan._sliced()
+        part_e = self.energy_partials(nvec)
+        nat.call("wpc_sqlap_cg_direction_energy", self.L.nrow, nat.ptr(sp), nat.ptr(sl),
+                 nat.ptr(plan._sell[7]), nat.ptr(plan._sell[5]), nat.ptr(sc), int(plan._sell[6]),
+                 nat.ptr(sv), nat.code(sv.dtype), nat.ptr(self.inverse_mass),
+                 nat.ptr(self._wm) if self.mass_weight != 0.0 else None, nat.ptr(z), nat.ptr(p),
+                 nat.ptr(t), nat.code(p.dtype), nvec, int(nparts_rz), nat.ptr(part_rz),
+                 nat.ptr(part_rr), nat.ptr(part_e), nat.ptr(state), nat.stream())
 
     def _apply(self, x, y, z, alpha, beta):
         self.product(x, y, z, alpha, beta, False)
