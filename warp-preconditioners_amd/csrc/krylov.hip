@@ -404,7 +404,7 @@ __device__ __forceinline__ void cg_direction_fold_body(
     FoldRegs<kKrylovThreads, (NV > 0 ? NV : 1)> fold_z, fold_r;
     if constexpr (NV > 0) {
         fold_z.load(part_rz, P_rz);
-        fold_r.load(part_rr, P_rr);
+        if (blk == 0) fold_r.load(part_rr, P_rr);  // r.r is only recorded, by workgroup 0
     }
     T zv[PF][NVM], pv[PF][NVM], xv[PF][NVM];
 #pragma unroll
@@ -444,21 +444,25 @@ __device__ __forceinline__ void cg_direction_fold_body(
             }
         }
     }
-    if constexpr (NV > 0 && NV <= 4) {
-        // both folds through one cross-wavefront pass (one barrier pair, not two)
-        double acc[8];
-        __shared__ double both[8];
-        fold_z.sums(part_rz, P_rz, acc);
-        fold_r.sums(part_rr, P_rr, acc + NV);
-        fold_reduce<kKrylovThreads>(acc, 2 * NV, both);
+    if constexpr (NV > 0) {
+        if (blk != 0) {
+            fold_z.finish(part_rz, P_rz, s_rz);
+        } else if constexpr (NV <= 4) {
+            // both folds through one cross-wavefront pass (one barrier pair, not two)
+            double acc[8];
+            __shared__ double both[8];
+            fold_z.sums(part_rz, P_rz, acc);
+            fold_r.sums(part_rr, P_rr, acc + NV);
+            fold_reduce<kKrylovThreads>(acc, 2 * NV, both);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { s_rz[v] = both[v]; s_rr[v] = both[NV + v]; }
-    } else if constexpr (NV > 0) {
-        fold_z.finish(part_rz, P_rz, s_rz);
-        fold_r.finish(part_rr, P_rr, s_rr);
+            for (int v = 0; v < NV; ++v) { s_rz[v] = both[v]; s_rr[v] = both[NV + v]; }
+        } else {
+            fold_z.finish(part_rz, P_rz, s_rz);
+            fold_r.finish(part_rr, P_rr, s_rr);
+        }
     } else {
         fold_all<kKrylovThreads>(part_rz, P_rz, nv, s_rz);
-        fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
+        if (blk == 0) fold_all<kKrylovThreads>(part_rr, P_rr, nv, s_rr);
     }
     T beta[NVM];
     bool act[NVM];

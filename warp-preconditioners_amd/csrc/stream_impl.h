@@ -720,10 +720,10 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
 // p' = z + BETA p and t' = M^-1 L p' = M^-1 L z + BETA t (t as the previous step's
 // first pass or this kernel left it), so the pass gathers z instead of p and the
 // direction launch leaves the iteration.  Every workgroup folds the r.z partials
-// of the preconditioner's last pass and the r.r partials of pass 2 into BETA =
-// RZ_NEW / RZ_OLD exactly as the direction step does, and workgroup 0 records
-// RZ_NEW, RR_NEW, BETA and, for stepping slots (ACTIVE), RZ, RR and ITER += 1; a
-// resting slot keeps p and t.  Curvature partials as pass 1, (L p')_i t'_i +
+// of the preconditioner's last pass into BETA = RZ_NEW / RZ_OLD exactly as the
+// direction step does; workgroup 0 also folds the r.r partials of pass 2 and
+// records RZ_NEW, RR_NEW, BETA and, for stepping slots (ACTIVE), RZ, RR and
+// ITER += 1.  A resting slot keeps p and t.  Curvature partials as pass 1, (L p')_i t'_i +
 // W_i p'_i^2 with (L p')_i = t'_i / s_i.
 template <class TV, class TX, int NV, bool HW>
 __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     st.load(state, NV);
     FoldRegs<kSqlapCgRows, NV> fold_z, fold_r;
     fold_z.load(part_rz, P_rz);
-    fold_r.load(part_rr, P_rr);
+    if (blk == 0) fold_r.load(part_rr, P_rr);  // r.r is only recorded, by workgroup 0
     TX zv[NV], pv[NV], tv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -767,7 +767,9 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
         pv[v] = p[(long)h.row * NV + v];
         tv[v] = t[(long)h.row * NV + v];
     }
-    if constexpr (NV <= 4) {
+    if (blk != 0) {
+        fold_z.finish(part_rz, P_rz, s_rz);
+    } else if constexpr (NV <= 4) {
         // both folds through one cross-wavefront pass
         double both_acc[8];
         __shared__ double both[8];
