@@ -546,8 +546,11 @@ int wpc_sqlap_cg_partials(int n);
  * slots keep p and t.  Leaves pass 2's curvature partials in part_energy, from
  * (L p)_i = t_i / inverse_mass_i.  Pass 2 (wpc_sqlap_cg_update, passes = 2) then
  * reads this t.  t is the carried vector and differs from inverse_mass .* (L p)
- * by the recurrence's rounding; z, p and t must not alias.  Other arguments as
- * for wpc_sqlap_cg_update. */
+ * by the recurrence's rounding; z, p and t must not alias.  With part_rz = NULL
+ * (partial counts too large to fold in every workgroup; nparts_rz and part_rr
+ * ignored) BETA and ACTIVE are read from the state as wpc_cg_beta left them:
+ * nothing is folded and the state is not written, and pass 2 follows as
+ * wpc_cg_alpha + passes = 6.  Other arguments as for wpc_sqlap_cg_update. */
 int wpc_sqlap_cg_direction_energy(int n, const int* slice_ptr, const int* slice_len,
                                   const int* slice_base, const int* rows,
                                   const void* sell_columns, int col16, const void* sell_values,

@@ -185,12 +185,27 @@ def test_recurrence_steps_defer_the_direction_step_inside_a_run(launches, recurr
     assert [n for n, _ in launches] == ["cg_update", "M", "wpc_cg_direction_fold"]
 
 
-def test_recurrence_off_for_counts_past_the_in_place_fold(launches):
+def test_recurrence_with_scalar_folds_past_the_in_place_fold(launches):
+    # Partial counts too large to fold in every workgroup: the recurrence pass takes
+    # BETA from the state (wpc_cg_beta each step), and only the run's last step
+    # launches the direction step.
     nvec = 3
     nat.lib()
-    A = _LogOp(FOLD_IN_PLACE // nvec + 1, launches, "A")
+    big = FOLD_IN_PLACE // nvec + 1
+    A = _LogOp(big, launches, "A")
     w = KrylovWork("cg", A, _LogOp(64, launches, "M"), 262144, nvec, torch.float64, "cpu")
     x = torch.zeros(w.n * nvec, dtype=torch.float64)
+    w.steps(x, 1)
+    assert [n for n, _ in launches] == ["A", "wpc_cg_alpha", "wpc_cg_update", "M",
+                                        "wpc_cg_beta", "wpc_cg_direction"]
+    del launches[:]
     w.steps(x, 3)
     names = [n for n, _ in launches]
-    assert "direction_energy" not in names and names.count("wpc_cg_direction") == 3
+    step = ["wpc_cg_alpha", "cg_update", "M", "wpc_cg_beta"]
+    assert names == (["cg_update"] + step + ["direction_energy"] + step
+                     + ["direction_energy"] + step + ["wpc_cg_direction"])
+    ups = [a for n, a in launches if n == "cg_update"]
+    assert [u[0] for u in ups] == [1, 6, 6, 6]
+    assert [u[1] for u in ups] == [False, True, True, True]   # x in the second pass
+    assert all(u[2] is w.ap for u in ups)
+    assert all(a == (w.ap, 0) for n, a in launches if n == "direction_energy")

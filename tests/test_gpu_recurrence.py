@@ -182,3 +182,88 @@ def test_recurrence_float32_solve():
         assert res[True][v] <= 2.0 * res[False][v] + 1e-5, res
     for a, b in zip(out[True][0], out[False][0]):
         assert abs(a - b) <= max(3, 0.05 * b), (out[True][0], out[False][0])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nvec", [1, 3, 8])
+def test_direction_energy_with_beta_from_the_state(dtype, nvec):
+    """part_rz None (partial counts too large to fold in every workgroup): BETA and
+    ACTIVE as wpc_cg_beta left them, p exactly as wpc_cg_direction writes it, the
+    state untouched."""
+    from warp_preconditioners import _native as nat
+
+    n, A, q = _operator(33, 50 + nvec, dtype, 1.0)
+    rng = np.random.default_rng(60 + nvec)
+    p = _vec(rng.normal(size=n * nvec), dtype)
+    z = _vec(rng.normal(size=n * nvec), dtype)
+    count = A.cg_update_parts(nvec, dtype)
+    inactive = 1 if nvec > 1 else None
+    base = _state(nvec, inactive)
+    F = base.numel() // nvec
+    b = host(base).reshape(F, nvec)
+    b[7] = 1.0                                    # ACTIVE
+    b[10] = np.linspace(0.3, 0.9, nvec)           # BETA
+    if inactive is not None:
+        b[7, inactive], b[10, inactive] = 0.0, 0.0
+    base = torch.tensor(b.reshape(-1), dtype=torch.float64, device=dev())
+    s = nat.stream()
+    scratch_r = torch.zeros_like(p)
+    rr_tmp = torch.zeros(count * nvec, dtype=torch.float64, device=dev())
+
+    def first_pass(pv, t):
+        A.cg_update(pv, scratch_r, nvec, base.clone(), rr_tmp, passes=1, t=t)
+        return A.energy_partials(nvec)[:count * nvec].clone()
+
+    t0 = torch.empty_like(p)
+    first_pass(p, t0)
+    p_ref, t_ref = p.clone(), torch.empty_like(p)
+    nat.call("wpc_cg_direction", n, nvec, nat.ptr(z), nat.ptr(p_ref), nat.code(dtype),
+             nat.ptr(base), s)
+    e_ref = first_pass(p_ref, t_ref)
+    p_rec, st_rec, t_rec = p.clone(), base.clone(), t0.clone()
+    A.direction_energy(z, p_rec, t_rec, nvec, 0, None, None, st_rec)
+    e_rec = A.energy_partials(nvec)[:count * nvec].clone()
+    torch.cuda.synchronize()
+
+    np.testing.assert_array_equal(host(st_rec), host(base))
+    np.testing.assert_array_equal(host(p_rec), host(p_ref))
+    tr, tref = host(t_rec).reshape(n, nvec), host(t_ref).reshape(n, nvec)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    for v in range(nvec):
+        if v == inactive:
+            np.testing.assert_array_equal(tr[:, v], host(t0).reshape(n, nvec)[:, v])
+            continue
+        err = np.linalg.norm(tr[:, v].astype(np.float64) - tref[:, v]) / np.linalg.norm(tref[:, v])
+        assert err <= tol, (v, err)
+    er = host(e_rec).reshape(count, nvec).sum(axis=0)
+    ef = host(e_ref).reshape(count, nvec).sum(axis=0)
+    np.testing.assert_allclose(er, ef, rtol=tol)
+
+
+@pytest.mark.parametrize("weight", [0.0, 1.0])
+def test_recurrence_with_scalar_folds_matches_the_direction_step(weight, monkeypatch):
+    import warp_preconditioners as wp
+    from warp_preconditioners import linear
+
+    monkeypatch.setattr(linear, "FOLD_IN_PLACE", 1)  # every count takes the scalar folds
+    n, A, q = _operator(44, 64, torch.float64, weight)
+    P = wp.FSAI(upload(q), max_row_size=8, factor_dtype=torch.float32)
+    B = _vec(np.random.default_rng(29).normal(size=(n, 3)), torch.float64)
+    calls = _counted(A)
+    out = {}
+    for rec in (False, True):
+        work = linear.KrylovWork("cg", A, P, n, 3, torch.float64, dev())
+        work.recurrence = rec
+        before = len(calls)
+        X = torch.zeros_like(B)
+        o = linear.solve_batched("cg", A, B, X, M=P, tol=1e-10, maxiter=20000, check_every=25,
+                                 work=work)
+        assert (len(calls) > before) == rec
+        out[rec] = (o["iterations"], host(X))
+    Bh = host(B)
+    for rec, (its, X) in out.items():
+        for v in range(3):
+            res = np.linalg.norm(Bh[:, v] - q @ X[:, v]) / np.linalg.norm(Bh[:, v])
+            assert res <= 1.5e-10, (rec, v, res)
+    for a, b in zip(out[True][0], out[False][0]):
+        assert abs(a - b) <= max(2, 0.02 * b), (out[True][0], out[False][0])

@@ -109,14 +109,18 @@ int wpc_sqlap_cg_direction_energy(int n, const int* slice_ptr, const int* slice_
     if (n <= 0) return WPC_OK;
     if (slice_ptr == nullptr || slice_len == nullptr || slice_base == nullptr ||
         sell_columns == nullptr || sell_values == nullptr || inverse_mass == nullptr ||
-        z == nullptr || p == nullptr || t == nullptr || part_rz == nullptr ||
-        part_rr == nullptr || part_energy == nullptr || state == nullptr || nparts_rz <= 0)
+        z == nullptr || p == nullptr || t == nullptr || part_energy == nullptr ||
+        state == nullptr)
         return WPC_EARG;
+    // part_rz NULL: BETA from the state (wpc_cg_beta); else both partial sets folded
+    const bool from_state = part_rz == nullptr;
+    if (!from_state && (part_rr == nullptr || nparts_rz <= 0)) return WPC_EARG;
     if (z == p || z == t || p == t) return WPC_EARG;
     const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
                         sell_values, val_dtype, inverse_mass, weighted_mass, p, t, nullptr,
-                        vec_dtype, part_energy, const_cast<double*>(part_rr), state, 1 | 8,
-                        nullptr, z, nparts_rz, part_rz, as_stream(stream)};
+                        vec_dtype, part_energy, const_cast<double*>(part_rr), state,
+                        1 | 8 | (from_state ? 4 : 0), nullptr, z, from_state ? 0 : nparts_rz,
+                        part_rz, as_stream(stream)};
     return sqlap_cg_dispatch(nvec, a);
 }
 

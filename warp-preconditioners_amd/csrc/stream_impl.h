@@ -126,8 +126,9 @@ struct SqlapCgArgs {
     double* part_e;
     double* part_rr;
     double* state;
-    int passes;  // bit 0: the first pass, bit 1: the second, bit 2: ALPHA from the state,
-                 // bit 3: the first pass in recurrence form (gathers z, updates p and t)
+    int passes;  // bit 0: the first pass, bit 1: the second, bit 2: ALPHA (and with
+                 // bit 3 BETA) from the state, bit 3: the first pass in recurrence form
+                 // (gathers z, updates p and t)
     void* x;     // advanced by the second pass (required with bit 2)
     const void* z;          // bit 3: the preconditioned residual
     int P_rz;               // bit 3: count of the preconditioner's r.z partials
@@ -723,9 +724,11 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
 // of the preconditioner's last pass into BETA = RZ_NEW / RZ_OLD exactly as the
 // direction step does; workgroup 0 also folds the r.r partials of pass 2 and
 // records RZ_NEW, RR_NEW, BETA and, for stepping slots (ACTIVE), RZ, RR and
-// ITER += 1.  A resting slot keeps p and t.  Curvature partials as pass 1, (L p')_i t'_i +
-// W_i p'_i^2 with (L p')_i = t'_i / s_i.
-template <class TV, class TX, int NV, bool HW>
+// ITER += 1.  A resting slot keeps p and t.  Curvature partials as pass 1,
+// (L p')_i t'_i + W_i p'_i^2 with (L p')_i = t'_i / s_i.  FS (for partial counts
+// too large to fold in every workgroup): BETA and ACTIVE as the scalar fold
+// wpc_cg_beta left them in the state (no fold, no state writes).
+template <class TV, class TX, int NV, bool HW, bool FS>
 __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
@@ -753,13 +756,15 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
                                                (const unsigned short*)col, val, NV);
     else
         sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
-    // After the gathers: the state fields, both partial sets and the row's z, p and
-    // t, in one round trip.
-    StateLanes<WPC_KS_ACTIVE, WPC_KS_RZ_OLD> st;
+    // After the gathers: the state fields, the partials and the row's z, p and t,
+    // in one round trip.
+    StateLanes<WPC_KS_ACTIVE, (FS ? WPC_KS_BETA : WPC_KS_RZ_OLD)> st;
     st.load(state, NV);
     FoldRegs<kSqlapCgRows, NV> fold_z, fold_r;
-    fold_z.load(part_rz, P_rz);
-    if (blk == 0) fold_r.load(part_rr, P_rr);  // r.r is only recorded, by workgroup 0
+    if constexpr (!FS) {
+        fold_z.load(part_rz, P_rz);
+        if (blk == 0) fold_r.load(part_rr, P_rr);  // r.r is only recorded, by workgroup 0
+    }
     TX zv[NV], pv[NV], tv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -767,38 +772,45 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
         pv[v] = p[(long)h.row * NV + v];
         tv[v] = t[(long)h.row * NV + v];
     }
-    if (blk != 0) {
-        fold_z.finish(part_rz, P_rz, s_rz);
-    } else if constexpr (NV <= 4) {
-        // both folds through one cross-wavefront pass
-        double both_acc[8];
-        __shared__ double both[8];
-        fold_z.sums(part_rz, P_rz, both_acc);
-        fold_r.sums(part_rr, P_rr, both_acc + NV);
-        fold_reduce<kSqlapCgRows>(both_acc, 2 * NV, both);
+    if constexpr (!FS) {
+        if (blk != 0) {
+            fold_z.finish(part_rz, P_rz, s_rz);
+        } else if constexpr (NV <= 4) {
+            // both folds through one cross-wavefront pass
+            double both_acc[8];
+            __shared__ double both[8];
+            fold_z.sums(part_rz, P_rz, both_acc);
+            fold_r.sums(part_rr, P_rr, both_acc + NV);
+            fold_reduce<kSqlapCgRows>(both_acc, 2 * NV, both);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { s_rz[v] = both[v]; s_rr[v] = both[NV + v]; }
-    } else {
-        fold_z.finish(part_rz, P_rz, s_rz);
-        fold_r.finish(part_rr, P_rr, s_rr);
+            for (int v = 0; v < NV; ++v) { s_rz[v] = both[v]; s_rr[v] = both[NV + v]; }
+        } else {
+            fold_z.finish(part_rz, P_rz, s_rz);
+            fold_r.finish(part_rr, P_rr, s_rr);
+        }
     }
     double e[NV];
     TX pn[NV], tn[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         const bool act = st.get(0, NV, v) != 0.0;
-        const double b = act ? s_rz[v] / st.get(1, NV, v) : 0.0;
-        const TX beta = (TX)b;
-        if (blk == 0 && threadIdx.x == v) {
-            state[WPC_KS_RZ_NEW * NV + v] = s_rz[v];
-            state[WPC_KS_RR_NEW * NV + v] = s_rr[v];
-            state[WPC_KS_BETA * NV + v] = b;
-            if (act) {
-                state[WPC_KS_RZ * NV + v] = s_rz[v];
-                state[WPC_KS_RR * NV + v] = s_rr[v];
-                state[WPC_KS_ITER * NV + v] += 1.0;
+        double b;
+        if constexpr (FS) {
+            b = act ? st.get(1, NV, v) : 0.0;
+        } else {
+            b = act ? s_rz[v] / st.get(1, NV, v) : 0.0;
+            if (blk == 0 && threadIdx.x == v) {
+                state[WPC_KS_RZ_NEW * NV + v] = s_rz[v];
+                state[WPC_KS_RR_NEW * NV + v] = s_rr[v];
+                state[WPC_KS_BETA * NV + v] = b;
+                if (act) {
+                    state[WPC_KS_RZ * NV + v] = s_rz[v];
+                    state[WPC_KS_RR * NV + v] = s_rr[v];
+                    state[WPC_KS_ITER * NV + v] += 1.0;
+                }
             }
         }
+        const TX beta = (TX)b;
         TX tz[1];
         St<TX>::template st<TA>(tz, 0, sv * acc[v]);  // M^-1 L z as the plain pass rounds it
         pn[v] = act ? zv[v] + beta * pv[v] : pv[v];
@@ -1105,7 +1117,7 @@ int sqlap_cg_launch(const SqlapCgArgs& a) {
 #define SC(HW, FS)                                                                              \
     do {                                                                                        \
         if ((a.passes & 1) && (a.passes & 8))                                                   \
-        hipLaunchKernelGGL((sell_direction_energy_kernel<TV, TX, NV, HW>), dim3(grid),          \
+        hipLaunchKernelGGL((sell_direction_energy_kernel<TV, TX, NV, HW, FS>), dim3(grid),      \
                            dim3(kSqlapCgRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
                            a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
                            (const TX*)a.s, (const TX*)a.wm, (const TX*)a.z, (TX*)a.p, (TX*)a.t, \

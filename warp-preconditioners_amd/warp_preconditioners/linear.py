@@ -336,33 +336,38 @@ class KrylovWork:
                      nat.ptr(self.p), nat.ptr(self.az), nat.ptr(self.ap), self.code, st, pz, pr, s)
 
     def steps(self, x, count):
-        """`count` CG steps.  Where the fused step applies in place (see
-        `_fused_update_step`) and `recurrence` is set, a step other than the last
-        leaves its direction step to the next step's first operator pass
-        (`direction_energy`), which also carries t = M^-1 L p in `ap` across steps;
-        the last step ends with the direction step, so between calls p and the
-        state are those of `count` calls of `step`."""
-        pending = False
+        """`count` CG steps.  Where the fused step applies (see `_fused_update_step`)
+        and `recurrence` is set, a step other than the last leaves its direction
+        step to the next step's first operator pass (`direction_energy`), which also
+        carries t = M^-1 L p in `ap` across steps; the last step ends with the
+        direction step, so between calls p and the state are those of `count` calls
+        of `step`."""
+        pending = None
         for i in range(count):
-            if not (self.recurrence and self._in_place_parts() is not None):
-                if pending:  # (cannot happen: eligibility never ends once it begins)
-                    self._direction_fold(self._in_place_parts())
-                    pending = False
+            mode = self._recurrence_mode() if self.recurrence else None
+            if mode is None:
+                if pending is not None:  # (cannot happen: the mode never ends once set)
+                    self._direction(pending)
+                    pending = None
                 self.step(x)
                 continue
-            self._recurrence_step(x, pending, i == count - 1)
-            pending = i != count - 1
+            last = i == count - 1
+            self._recurrence_step(x, mode, pending is not None, last)
+            pending = None if last else mode
 
-    def _in_place_parts(self):
-        """The operator's partial count when the fused step runs its in-place form
-        (every fold repeated by each workgroup), else None."""
+    def _recurrence_mode(self):
+        """("fold", count) where the fused step folds its partials in every
+        workgroup, ("scalar", count) where it takes the single-workgroup folds
+        (count: the operator's partials per vector), None where it does not apply
+        (the preconditioner's partial count is learnt on a solve's first step)."""
         if not self.use_fused or not self.fuse_update or self._m_parts is None:
             return None
         parts_of = getattr(self.A, "cg_update_parts", None)
         count = None if parts_of is None else parts_of(self.nvec, self.dtype)
-        if count is None or max(count, self._m_parts) * self.nvec > FOLD_IN_PLACE:
+        if count is None:
             return None
-        return count
+        fold = max(count, self._m_parts) * self.nvec <= FOLD_IN_PLACE
+        return ("fold" if fold else "scalar", count)
 
     def _rr_buffer(self, count):
         need = count * self.nvec
@@ -371,31 +376,49 @@ class KrylovWork:
             nat.bump_generation()
         return self._rr_parts
 
-    def _direction_fold(self, count):
-        nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
-                 nat.ptr(self.p), None, self.code, self._m_parts, nat.ptr(self._m_part), count,
-                 nat.ptr(self._rr_parts), nat.ptr(self.state), nat.stream())
+    def _direction(self, mode):
+        kind, count = mode
+        st, s = nat.ptr(self.state), nat.stream()
+        if kind == "fold":
+            nat.call("wpc_cg_direction_fold", self.n, self.nvec, nat.ptr(self.z),
+                     nat.ptr(self.p), None, self.code, self._m_parts, nat.ptr(self._m_part),
+                     count, nat.ptr(self._rr_parts), st, s)
+        else:  # BETA as wpc_cg_beta left it
+            nat.call("wpc_cg_direction", self.n, self.nvec, nat.ptr(self.z), nat.ptr(self.p),
+                     self.code, st, s)
 
-    def _recurrence_step(self, x, pending, last):
-        """One in-place fused step with t in `ap`: the first pass gathers z and takes
-        the previous step's direction step when `pending`, else it is the plain
-        first pass on p; the direction step runs only when `last`."""
-        count = self._in_place_parts()
+    def _recurrence_step(self, x, mode, pending, last):
+        """One fused step with t in `ap`: the first pass gathers z and takes the
+        previous step's direction step when `pending`, else it is the plain first
+        pass on p; the direction step runs only when `last`."""
+        kind, count = mode
         rr = self._rr_buffer(count)
-        if pending:
-            self.A.direction_energy(self.z, self.p, self.ap, self.nvec, self._m_parts,
-                                    self._m_part, rr, self.state)
-            self.A.cg_update(self.p, self.r, self.nvec, self.state, rr, passes=2, x=x,
-                             t=self.ap)
+        st, s = nat.ptr(self.state), nat.stream()
+        A = self.A
+        if kind == "fold":
+            if pending:
+                A.direction_energy(self.z, self.p, self.ap, self.nvec, self._m_parts,
+                                   self._m_part, rr, self.state)
+                A.cg_update(self.p, self.r, self.nvec, self.state, rr, passes=2, x=x, t=self.ap)
+            else:
+                A.cg_update(self.p, self.r, self.nvec, self.state, rr, x=x, t=self.ap)
         else:
-            self.A.cg_update(self.p, self.r, self.nvec, self.state, rr, x=x, t=self.ap)
+            if pending:
+                A.direction_energy(self.z, self.p, self.ap, self.nvec, 0, None, None, self.state)
+            else:
+                A.cg_update(self.p, self.r, self.nvec, self.state, rr, passes=1, t=self.ap)
+            nat.call("wpc_cg_alpha", self.nvec, count, nat.ptr(A.energy_partials(self.nvec)),
+                     st, s)
+            A.cg_update(self.p, self.r, self.nvec, self.state, rr, passes=6, x=x, t=self.ap)
         part, nparts = self.M.apply_batch(self.r, self.z, self.z, 1.0, 0.0, self.nvec,
                                           dot_w=self.r)
         if nparts != self._m_parts:
             raise RuntimeError("the preconditioner's partial count changed inside a run")
         self._m_part = part
+        if kind == "scalar":
+            nat.call("wpc_cg_beta", self.nvec, nparts, nat.ptr(part), count, nat.ptr(rr), st, s)
         if last:
-            self._direction_fold(count)
+            self._direction(mode)
 
     def _fused_update_step(self, x, s, st):
         """One CG step with the update folded into the operator's last pass

@@ -171,7 +171,9 @@ class SquaredLaplacianOperator(LinearOperator):
         """The previous CG step's direction step folded into this step's first pass
         (`wpc_sqlap_cg_direction_energy`): BETA from the r.z and r.r partials,
         p = z + BETA p, t = M^-1 L z + BETA t (t as the previous first pass left
-        it), curvature partials for `cg_update(..., passes=2, t=t)`."""
+        it), curvature partials for `cg_update(..., passes=2, t=t)`.  With
+        `part_rz` None, BETA as wpc_cg_beta left it in `state` (nothing folded; then
+        wpc_cg_alpha and `cg_update(..., passes=6, t=t)` follow)."""
         plan = self._stream()
         sp, sl, sc, sv = plan._sliced()
         part_e = self.energy_partials(nvec)
