@@ -534,8 +534,8 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
  * workgroup: ceil(n / 512)).  Host-only helper; no reference counterpart. */
 int wpc_sqlap_cg_partials(int n);
 
-/* No reference counterpart (the reference's solver runs the direction step of
- * pcg_solver.py on its own): the first pass of wpc_sqlap_cg_update with the
+/* No reference counterpart (warp.optim.linear.cg, the reference's solver, runs its
+ * direction step on its own): the first pass of wpc_sqlap_cg_update with the
  * previous step's direction step folded in, for the steps of a CG run between the
  * first and the last, whose direction step it replaces.  Folds nparts_rz r.z
  * partials (part_rz, the preconditioner's last pass) and the
