@@ -733,8 +733,9 @@ def apply_rows(sides, nvecs, reps=50):
                     us = min(us, e0.elapsed_time(e1) / reps * 1e3)
             del graph
             pg, pgt = P.plans()
-            ib = [2 if (pl.mode(k, torch.float64) == pl.SLICED and pl._sell is not None
-                        and pl._sell[6]) else 4 for pl in (pg, pgt)]
+            ib = [{1: 2, 2: 0}.get(int(pl._sell[6]), 4)
+                  if (pl.mode(k, torch.float64) == pl.SLICED and pl._sell is not None) else 4
+                  for pl in (pg, pgt)]
             nbytes = 2 * nnz * (4 + 4) + 8 * (n + 1) + 4 * n * k * 8
             rows.append(dict(side=side, n=n, nnzG=nnz, nvec=k, us=us, bytes=nbytes,
                              GBs=nbytes / us / 1e3, frac=nbytes / us / 1e3 / HBM_PEAK_GBS,

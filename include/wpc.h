@@ -145,14 +145,33 @@ int wpc_sell_values(long count, const int* sell_map, const void* values, int ele
 int wpc_sell_col16(int n, const int* offsets, const int* rows, const int* slice_ptr,
                    const int* slice_len, const int* sell_columns, short* sell_col16,
                    int* slice_base, int* overflow, void* stream);
+/* Stencil form of the sliced copy (natural row order, no column stream): the
+ * offsets column - row of a slice's rows, merged in ascending order, are the
+ * slice's entries.  Entry k of the row in position r of slice s has column
+ * r + deltas[32 * s + k] where bit k of row_mask[r] is set; a clear bit is
+ * padding (value 0).  Each row's entries stay in the order of its columns, so the
+ * products' sums equal the plain slices' (up to the sign of a zero sum).  Run
+ * twice: with slice_ptr = NULL it writes slice_len[s] = the number of distinct
+ * offsets of slice s and adds to status[0] (zeroed by the caller) the slices with
+ * more than 32 (then the form does not apply); after slice_ptr[0] = 0, slice_ptr[1..]
+ * = inclusive scan of slice_len, it writes deltas (32 * nslices ints), row_mask
+ * (n ints) and sell_map (64 * slice_ptr[nslices] ints, -1 for padding) for
+ * wpc_sell_values.  The products take it as col16 = 2 with sell_columns = deltas
+ * and slice_base = row_mask (ncol * nvec * sizeof(vector) must not exceed 2^31).
+ * No reference counterpart: a layout of this library for mesh-ordered matrices,
+ * whose slices share a few offsets (the 15 of a Kuhn grid's P1 Laplacian). */
+int wpc_sell_stencil(int n, const int* offsets, const int* columns, const int* slice_ptr,
+                     int* slice_len, int* deltas, int* row_mask, int* sell_map, int* status,
+                     void* stream);
 
 /* The product of wpc_csr_stream on the sliced copy: one thread per row, storage
  * order serial sums (bit-identical to the streamed mapping), same epilogue and
  * fused inner product; rows as given to wpc_sell_fill; sell_columns are the int32
- * columns (col16 = 0; then ncol * nvec * sizeof(vector) must not exceed 2^31) or
- * the 16-bit offsets of wpc_sell_col16 (col16 = 1).  slice_base (one int32 per
- * slice) is always read: the bases of wpc_sell_col16, or zeros with int32
- * columns.
+ * columns (col16 = 0; then ncol * nvec * sizeof(vector) must not exceed 2^31), the
+ * 16-bit offsets of wpc_sell_col16 (col16 = 1) or the deltas of wpc_sell_stencil
+ * (col16 = 2, rows NULL, slice_base = its row masks; range as for int32).
+ * slice_base (one int32 per slice) is always read: the bases of wpc_sell_col16, or
+ * zeros with int32 columns.
  * Padding adds exactly +0 whatever x holds (its gathers fall outside the buffer
  * descriptor's range and read 0).  partials: ceil(n / 256) workgroups (workgroup b
  * covers positions 256b..+255). */

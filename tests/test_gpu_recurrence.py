@@ -267,3 +267,36 @@ def test_recurrence_with_scalar_folds_matches_the_direction_step(weight, monkeyp
             assert res <= 1.5e-10, (rec, v, res)
     for a, b in zip(out[True][0], out[False][0]):
         assert abs(a - b) <= max(2, 0.02 * b), (out[True][0], out[False][0])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fused_passes_on_the_stencil_form_equal_the_plain_slices(dtype, monkeypatch):
+    """The factored operator's fused passes on the stencil form of L (no column
+    stream) against the same passes on the plain slices: bit for bit."""
+    from warp_preconditioners import _native as nat
+    from warp_preconditioners.sparse import StreamPlan
+
+    nvec = 3
+    outs = []
+    for stencil in (True, False):
+        monkeypatch.setattr(StreamPlan, "STENCIL", stencil)
+        n, A, _ = _operator(41, 77, dtype, 1.0)
+        count = A.cg_update_parts(nvec, dtype)
+        A._stream()._sliced()
+        assert (int(A._stream()._sell[6]) == StreamPlan.COL_STENCIL) == stencil
+        rng = np.random.default_rng(78)
+        p = _vec(rng.normal(size=n * nvec), dtype)
+        z = _vec(rng.normal(size=n * nvec), dtype)
+        r = _vec(rng.normal(size=n * nvec), dtype)
+        x = _vec(rng.normal(size=n * nvec), dtype)
+        t = torch.empty_like(p)
+        st = _state(nvec, 1)
+        rr = torch.zeros(count * nvec, dtype=torch.float64, device=dev())
+        A.cg_update(p, r, nvec, st, rr, x=x, t=t)
+        part_rz = _vec(rng.uniform(0.5, 1.5, size=300 * nvec) / 300, torch.float64)
+        A.direction_energy(z, p, t, nvec, 300, part_rz, rr, st)
+        A.cg_update(p, r, nvec, st, rr, passes=2, x=x, t=t)
+        torch.cuda.synchronize()
+        outs.append([host(v) for v in (p, t, r, x, st, rr)])
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)

@@ -439,3 +439,94 @@ def test_graph_replay_after_scratch_grew():
     np.testing.assert_array_equal(host(x1), first)
     for s in sentinels:
         assert bool((s == 7.0).all())
+
+
+def _grid_laplacian(nx, ny, nz, rng):
+    """7-point pattern on an nx*ny*nz grid (natural order), random values, one row
+    emptied: slices share at most 7 offsets, boundary rows miss some."""
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    for axis, step in ((2, 1), (1, nx), (0, nx * ny)):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[axis], hi[axis] = slice(0, -1), slice(1, None)
+        a, b = idx[tuple(lo)].ravel(), idx[tuple(hi)].ravel()
+        rows += [a, b]
+        cols += [b, a]
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    m = ss.csr_matrix((rng.normal(size=r.size), (r, c)), shape=(n, n)).tolil()
+    m[100, :] = 0  # an empty row
+    m = m.tocsr()
+    m.eliminate_zeros()
+    m.sort_indices()
+    return m
+
+
+@pytest.mark.parametrize("nvec", [1, 3, 8])
+def test_stencil_form_matches_oracle_and_streamed_bitwise(oracle, nvec):
+    # The stencil form (per-slice offsets from the row, per-row masks, no column
+    # stream) sums each row's entries in column order like the plain slices: equal
+    # to the oracle's serial product and, with the fused epilogue and dot, to the
+    # streamed mapping, bit for bit.  Missing neighbours of boundary rows read
+    # nothing: NaN and inf in x next to them do not leak in.
+    from warp_preconditioners.sparse import StreamPlan
+
+    rng = np.random.default_rng(70 + nvec)
+    a = _grid_laplacian(21, 13, 11, rng)
+    n = a.shape[0]
+    A = upload(a)
+    plan = StreamPlan(A, lanes=StreamPlan.SLICED)
+    plan._sliced()
+    assert int(plan._sell[6]) == StreamPlan.COL_STENCIL
+    deltas = host(plan._sell[2]).reshape(-1, 32)
+    lens = host(plan._sell[1])
+    assert lens.max() == 7 and (deltas[0, :lens[0]] == np.sort(deltas[0, :lens[0]])).all()
+    x = rng.normal(size=(n, nvec))
+    # x[row + d] for a missing neighbour of a boundary row: row 20 (x = 20, the +x
+    # boundary) misses column 21 (the next grid line's first node)
+    x[21, :] = np.nan
+    x[22, 0] = np.inf
+    y = rng.normal(size=(n, nvec))
+    z = torch.empty(n * nvec, dtype=torch.float64, device=dev())
+    plan.run(torch.tensor(x.ravel(), device=dev()), torch.tensor(y.ravel(), device=dev()), z,
+             0.7, -0.3, nvec)
+    ref = np.stack([oracle.spmv(oracle_csr(oracle, a, np.float64),
+                                np.ascontiguousarray(x[:, v]), np.ascontiguousarray(y[:, v]),
+                                0.7, -0.3) for v in range(nvec)], axis=1)
+    got = host(z).reshape(n, nvec)
+    np.testing.assert_array_equal(got, ref)
+    assert np.isfinite(got[20]).all() and got[100, 0] == -0.3 * y[100, 0]
+    X = torch.tensor(rng.normal(size=(n, nvec)), device=dev())
+    U = torch.tensor(rng.normal(size=(n, nvec)), device=dev())
+    S = torch.tensor(rng.uniform(0.5, 2, size=n), dtype=torch.float32, device=dev())
+    D = torch.tensor(rng.uniform(0.5, 2, size=n), device=dev())
+    outs = []
+    for mode in (0, StreamPlan.SLICED):
+        p = StreamPlan(A, lanes=mode)
+        Z = torch.empty_like(X)
+        dots = torch.zeros(nvec, dtype=torch.float64, device=dev())
+        p.run(X.reshape(-1), None, Z.reshape(-1), 0.9, 0.0, nvec, s=S, d=D, u=U.reshape(-1),
+              w=X.reshape(-1), out=dots)
+        outs.append((host(Z), host(dots)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-12)
+
+
+def test_stencil_form_only_where_it_stores_no_more(monkeypatch):
+    # Slices whose rows share more than 32 offsets, or whose union of offsets
+    # exceeds their longest row, keep the plain slices; the switch turns it off.
+    from warp_preconditioners.sparse import StreamPlan
+
+    rng = np.random.default_rng(5)
+    n = 640
+    a = ss.random(n, n, density=0.03, random_state=rng, format="csr") + ss.eye(n)
+    a.sort_indices()
+    plan = StreamPlan(upload(a), lanes=StreamPlan.SLICED)
+    plan._sliced()
+    assert int(plan._sell[6]) != StreamPlan.COL_STENCIL
+    g = _grid_laplacian(16, 8, 5, rng)
+    monkeypatch.setattr(StreamPlan, "STENCIL", False)
+    plan = StreamPlan(upload(g), lanes=StreamPlan.SLICED)
+    plan._sliced()
+    assert int(plan._sell[6]) == StreamPlan.COL_U16

@@ -100,6 +100,59 @@ __global__ __launch_bounds__(256) void sell_col16_kernel(int n, int nslices,
     }
 }
 
+// Stencil form of the slices (natural row order): the offsets d = column - row of
+// a slice's rows, merged in ascending order, give the slice's entries; entry k of
+// every row of slice s has column row + deltas[32 s + k] where bit k of
+// the row's mask is set, and is padding (value 0, no gather) where it is clear.
+// Entries then sit in the order of their columns, as in the plain slices.  One
+// wavefront per slice walks the union with wavefront minima (each row's columns
+// ascend, so each step advances at least one row).  Without slice_ptr it only
+// counts (slice_len[s] = union size; status[0] += slices with more than 32
+// offsets, which a 32-bit mask cannot hold); with slice_ptr it writes deltas, the
+// row masks and the map (every position of the slice's union, -1 for padding).
+constexpr int kStencilMax = 32;
+
+__global__ __launch_bounds__(256) void sell_stencil_kernel(
+    int n, int nslices, const int* __restrict__ off, const int* __restrict__ col,
+    const int* __restrict__ slice_ptr, int* __restrict__ slice_len, int* __restrict__ deltas,
+    int* __restrict__ row_mask, int* __restrict__ smap, int* status) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nslices * kSlice) return;  // whole wavefronts: kSlice == kWave
+    const int s = (int)(idx / kSlice);
+    const int lane = (int)(idx % kSlice);
+    const bool fill = slice_ptr != nullptr;
+    const long sp = fill ? (long)slice_ptr[s] : 0;
+    const int row = (int)idx;
+    int j = row < n ? off[row] : 0;
+    const int b = row < n ? off[row + 1] : 0;
+    unsigned mask = 0;
+    int count = 0;
+    for (;;) {
+        const int d = j < b ? col[j] - row : INT_MAX;
+        int m = d;
+        for (int w = kWave / 2; w > 0; w >>= 1) m = min(m, __shfl_xor(m, w, kWave));
+        if (m == INT_MAX) break;  // (uniform: every lane holds the minimum)
+        const bool has = d == m;
+        if (fill && count < kStencilMax) {
+            smap[(sp + count) * kSlice + lane] = has ? j : -1;
+            if (lane == 0) deltas[(long)s * kStencilMax + count] = m;
+        }
+        if (has) {
+            if (count < kStencilMax) mask |= 1u << count;
+            ++j;
+        }
+        ++count;
+    }
+    if (!fill) {
+        if (lane == 0) {
+            slice_len[s] = count;
+            if (count > kStencilMax) atomicAdd(status, 1);
+        }
+    } else if (row < n) {
+        row_mask[row] = (int)mask;
+    }
+}
+
 template <class E>
 __global__ __launch_bounds__(256) void sell_values_kernel(long count, const int* __restrict__ smap,
                                                           const E* __restrict__ src,
@@ -146,6 +199,21 @@ int wpc_sell_col16(int n, const int* offsets, const int* rows, const int* slice_
     hipLaunchKernelGGL(sell_col16_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
                        as_stream(stream), n, ns, offsets, rows, slice_ptr, slice_len,
                        sell_columns, (unsigned short*)sell_col16, slice_base, overflow);
+    return launch_status();
+}
+
+int wpc_sell_stencil(int n, const int* offsets, const int* columns, const int* slice_ptr,
+                     int* slice_len, int* deltas, int* row_mask, int* sell_map, int* status,
+                     void* stream) {
+    const int ns = wpc_sell_slices(n);
+    if (ns == 0) return WPC_OK;
+    if (offsets == nullptr || columns == nullptr || slice_len == nullptr || status == nullptr)
+        return WPC_EARG;
+    if (slice_ptr != nullptr && (deltas == nullptr || row_mask == nullptr || sell_map == nullptr))
+        return WPC_EARG;
+    hipLaunchKernelGGL(sell_stencil_kernel, dim3(blocks_for((long)ns * kSlice, 256)), dim3(256), 0,
+                       as_stream(stream), n, ns, offsets, columns, slice_ptr, slice_len, deltas,
+                       row_mask, sell_map, status);
     return launch_status();
 }
 

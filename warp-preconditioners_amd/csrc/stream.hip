@@ -6,6 +6,11 @@
 using namespace wpc;
 
 static int stream_dispatch(int nvec, const StreamArgs& a);
+// A column form the sliced products accept: int32, 16-bit offsets, or stencil
+// deltas (natural row order only).
+static bool column_form_ok(int col16, const int* rows) {
+    return col16 >= kColI32 && col16 <= kColStencil && (col16 != kColStencil || rows == nullptr);
+}
 static int sqlap_cg_dispatch(int nvec, const SqlapCgArgs& a);
 
 extern "C" {
@@ -63,8 +68,9 @@ int wpc_sell_stream(int n, const int* slice_ptr, const int* slice_len, const int
                     const void* w, double* partials, double* out, void* stream) {
     if (n > 0 && (slice_ptr == nullptr || slice_len == nullptr)) return WPC_EARG;
     if (n > 0 && slice_base == nullptr) return WPC_EARG;
+    if (!column_form_ok(col16, rows)) return WPC_EARG;
     if (s != nullptr && s_dtype != WPC_F32 && s_dtype != vec_dtype) return WPC_EDTYPE;
-    const StreamArgs a{0, slice_ptr, slice_len, slice_base, rows, col16 != 0, n, -2, nullptr,
+    const StreamArgs a{0, slice_ptr, slice_len, slice_base, rows, col16, n, -2, nullptr,
                        (const int*)sell_columns, sell_values,
                        val_dtype, s, s != nullptr && s_dtype == WPC_F32, d, u, x, y, z, vec_dtype,
                        alpha, beta, w, partials, out, as_stream(stream)};
@@ -89,9 +95,10 @@ int wpc_sqlap_cg_update(int n, const int* slice_ptr, const int* slice_len,
         part_rr == nullptr || state == nullptr)
         return WPC_EARG;
     if (p == t || p == r || t == r) return WPC_EARG;
+    if (!column_form_ok(col16, rows)) return WPC_EARG;
     if ((passes & 4) && x == nullptr) return WPC_EARG;
     if (x != nullptr && (x == p || x == t || x == r)) return WPC_EARG;
-    const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
+    const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16, sell_columns,
                         sell_values, val_dtype, inverse_mass, weighted_mass, p, t, r, vec_dtype,
                         part_energy, part_rr, state, passes, x, nullptr, 0, nullptr,
                         as_stream(stream)};
@@ -116,7 +123,8 @@ int wpc_sqlap_cg_direction_energy(int n, const int* slice_ptr, const int* slice_
     const bool from_state = part_rz == nullptr;
     if (!from_state && (part_rr == nullptr || nparts_rz <= 0)) return WPC_EARG;
     if (z == p || z == t || p == t) return WPC_EARG;
-    const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16 != 0, sell_columns,
+    if (!column_form_ok(col16, rows)) return WPC_EARG;
+    const SqlapCgArgs a{n, slice_ptr, slice_len, slice_base, rows, col16, sell_columns,
                         sell_values, val_dtype, inverse_mass, weighted_mass, p, t, nullptr,
                         vec_dtype, part_energy, const_cast<double*>(part_rr), state,
                         1 | 8 | (from_state ? 4 : 0), nullptr, z, from_state ? 0 : nparts_rz,

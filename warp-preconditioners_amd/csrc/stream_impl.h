@@ -31,8 +31,9 @@ struct StreamArgs {
     const int* blocks;  // streamed mapping: row partition; sliced mapping: slice_ptr
     const int* slice_len;  // sliced mapping only
     const int* slice_base; // sliced mapping, 16-bit columns: smallest column per slice
+                           // (stencil deltas: the row masks)
     const int* rows;       // sliced mapping: row order of the slices (NULL: natural)
-    bool col16;            // sliced mapping: 16-bit column offsets from slice_base
+    int col16;             // sliced mapping: column form (kColI32, kColU16, kColStencil)
     int n;
     int lg;  // log2(lanes per row), -1 for the streamed mapping, -2 for the sliced one
     const int* off;
@@ -113,7 +114,7 @@ struct SqlapCgArgs {
     const int* slice_len;
     const int* slice_base;
     const int* rows;
-    bool col16;
+    int col16;  // column form (kColI32, kColU16, kColStencil)
     const void* col;
     const void* val;
     int val_dtype;
@@ -151,6 +152,11 @@ int sqlap_cg_entry_5(const SqlapCgArgs& a);
 int sqlap_cg_entry_6(const SqlapCgArgs& a);
 int sqlap_cg_entry_7(const SqlapCgArgs& a);
 int sqlap_cg_entry_8(const SqlapCgArgs& a);
+
+// Column forms of the sliced copy: int32 columns, 16-bit offsets from a per-slice
+// base, or stencil deltas (32 per slice) with per-row masks.
+constexpr int kColI32 = 0, kColU16 = 1, kColStencil = 2;
+constexpr int kStencilStride = 32;
 
 }  // namespace wpc
 
@@ -394,14 +400,34 @@ __global__ __launch_bounds__(kStreamRows) void lanes_kernel(
 // padding offset lies outside the descriptor's range, so its load returns 0 with
 // no memory access and padding adds exactly +0, whatever x holds.
 
-// Byte offset of an entry's x row inside the descriptor (row stride `stride`).
-template <class TI> __device__ __forceinline__ unsigned sell_off(const TI* col, long p, unsigned stride);
-template <> __device__ __forceinline__ unsigned sell_off<int>(const int* col, long p, unsigned stride) {
+// Stencil columns (column form 2): entry kk of the row has column row + delta[kk]
+// (the slice's deltas, uniform across the wavefront: scalar loads) where bit kk of
+// the row's mask is set; a clear bit is padding, whose offset 2^32 - stride (also
+// with the vector offsets added) lies outside the descriptor over all of x.
+struct StencilCol {
+    const int* delta;
+    unsigned mask;
+    int row;
+};
+
+// Byte offset of entry kk (stored at p) of a row inside the descriptor (row
+// stride `stride`).
+template <class TI>
+__device__ __forceinline__ unsigned sell_off(const TI* col, long p, int kk, unsigned stride);
+template <>
+__device__ __forceinline__ unsigned sell_off<int>(const int* col, long p, int, unsigned stride) {
     return (unsigned)col[p] * stride;
 }
-template <> __device__ __forceinline__ unsigned sell_off<unsigned short>(const unsigned short* col,
-                                                                         long p, unsigned stride) {
+template <>
+__device__ __forceinline__ unsigned sell_off<unsigned short>(const unsigned short* col, long p,
+                                                             int, unsigned stride) {
     return (unsigned)col[p] * stride;
+}
+template <>
+__device__ __forceinline__ unsigned sell_off<StencilCol>(const StencilCol* col, long, int kk,
+                                                         unsigned stride) {
+    const int d = col->delta[kk];
+    return (col->mask >> kk) & 1u ? (unsigned)(col->row + d) * stride : 0u - stride;
 }
 
 
@@ -433,7 +459,7 @@ __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long b
     for (int j = 0; j < G; ++j) {
         const int e = FULL || j < m ? j : m - 1;
         const long p = base + (long)(k + e) * 64;
-        o[j] = sell_off<TI>(col, p, stride);
+        o[j] = sell_off<TI>(col, p, k + e, stride);
         a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
     }
 #pragma unroll
@@ -507,14 +533,40 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
     if (r == 1) sell_group<TI, TV, TX, NV, 1>(acc, base, k, xr, stride, col, val, nv);
 }
 
+// sell_accum in column form cm (slice: the wavefront's slice; mask, row: this
+// lane's stencil mask and row, read only in the stencil form).
+template <class TV, class TX, int NV>
+__device__ __forceinline__ void sell_accum_cm(typename AccOf<TX>::type* acc, long base, int len,
+                                              __amdgpu_buffer_rsrc_t xr, unsigned stride, int cm,
+                                              const void* __restrict__ col, int slice,
+                                              unsigned mask, int row, const TV* __restrict__ val,
+                                              int nv) {
+    if (cm == kColStencil) {
+        const StencilCol sc{(const int*)col + (long)slice * kStencilStride, mask, row};
+        sell_accum<StencilCol, TV, TX, NV>(acc, base, len, xr, stride, &sc, val, nv);
+    } else if (cm == kColU16) {
+        sell_accum<unsigned short, TV, TX, NV>(acc, base, len, xr, stride,
+                                               (const unsigned short*)col, val, nv);
+    } else {
+        sell_accum<int, TV, TX, NV>(acc, base, len, xr, stride, (const int*)col, val, nv);
+    }
+}
+
+// Descriptor range over x (from the slice base) for column form cm.
+__device__ __forceinline__ int sell_range(int cm, unsigned stride) {
+    return (int)(cm == kColU16 ? 0xFFFFu * stride : kRange32);
+}
+
 // blk must be uniform across the workgroup (the slice header is read with scalar
-// loads, and the gather descriptor is built from it).  col16: the columns are
-// 16-bit offsets from slice_base[s]; else int32 and slice_base holds zeros.
+// loads, and the gather descriptor is built from it).  cm: the column form
+// (kColU16: 16-bit offsets from slice_base[s]; kColI32: int32 columns and
+// slice_base holds zeros; kColStencil: deltas, slice_base holds the row masks,
+// natural row order).
 template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
 __device__ __forceinline__ void sell_body(
     int blk, int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
-    const void* __restrict__ col, bool col16,
+    const void* __restrict__ col, int cm,
     const TV* __restrict__ val, const TS* __restrict__ s, const TX* __restrict__ d,
     const TX* __restrict__ u, const TX* __restrict__ x, const TX* y, TX* z,
     typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
@@ -533,8 +585,10 @@ __device__ __forceinline__ void sell_body(
     const int slice = min(wave0, n - 1) >> 6;
     const int sp = slice_ptr[slice];
     const int sl = slice_len[slice];
-    const int sb = slice_base[slice];
+    const bool stencil = cm == kColStencil;
+    const int sb = stencil ? 0 : slice_base[slice];
     const int idc = live ? idx : n - 1;
+    const unsigned mask = stencil ? (unsigned)slice_base[idc] : 0u;
     const int row = rows != nullptr ? rows[idc] : idc;
     const bool use_a = alpha != TA(0);
     const long base = (long)sp * 64 + (idx & 63);
@@ -542,8 +596,7 @@ __device__ __forceinline__ void sell_body(
     const unsigned stride = (unsigned)(nv * sizeof(TX));
     // (slice_base holds zeros with int32 columns, so the base needs no branch)
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(x + (long)sb * nv), (short)0, (int)(col16 ? 0xFFFFu * stride : kRange32),
-        kBufferWord3);
+        (void*)(x + (long)sb * nv), (short)0, sell_range(cm, stride), kBufferWord3);
     TA sv = TA(1), dv = TA(0), uv[NVM], wv[NVM];
     if (HS) sv = (TA)s[row];
     if (HD) dv = St<TX>::template ld<TA>(d, row);
@@ -558,11 +611,7 @@ __device__ __forceinline__ void sell_body(
 #pragma unroll
     for (int v = 0; v < NVM; ++v) acc[v] = TA(0);
     // (len is uniform: the entry loop's control is scalar)
-    if (col16)
-        sell_accum<unsigned short, TV, TX, NV>(acc, base, len, xr, stride,
-                                               (const unsigned short*)col, val, nv);
-    else
-        sell_accum<int, TV, TX, NV>(acc, base, len, xr, stride, (const int*)col, val, nv);
+    sell_accum_cm<TV, TX, NV>(acc, base, len, xr, stride, cm, col, slice, mask, row, val, nv);
     // Every lane computes (a lane past n on row n-1's operands) and only the stores
     // are predicated: a branch around the loads would leave them pending at its
     // join, where the compiler then waits for every outstanding memory operation
@@ -619,7 +668,7 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     double* __restrict__ partials) {
     __shared__ double red[kStreamRows];
     sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, slice_base, rows,
-                                           col, col16 != 0, val, s, d, u, x, y, z, alpha, beta, w,
+                                           col, col16, val, s, d, u, x, y, z, alpha, beta, w,
                                            partials, red);
 }
 
@@ -637,7 +686,8 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
 
 // Wavefront slice header and gather descriptor of the sliced mapping (as sell_body).
 struct SellHead {
-    int row, len;
+    int row, len, slice;
+    unsigned mask;  // the row's stencil mask (column form kColStencil)
     bool live;
     long base;
 };
@@ -646,24 +696,26 @@ template <class TX, int NV, int NT = kStreamRows>
 __device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restrict__ slice_ptr,
                                               const int* __restrict__ slice_len,
                                               const int* __restrict__ slice_base,
-                                              const int* __restrict__ rows, bool col16,
+                                              const int* __restrict__ rows, int cm,
                                               const TX* x, __amdgpu_buffer_rsrc_t* xr) {
     const int idx = blk * NT + threadIdx.x;
     const int wave0 = __builtin_amdgcn_readfirstlane(idx & ~63);
     const int slice = min(wave0, n - 1) >> 6;
     const int sp = slice_ptr[slice];
     const int sl = slice_len[slice];
-    const int sb = slice_base[slice];
+    const bool stencil = cm == kColStencil;
+    const int sb = stencil ? 0 : slice_base[slice];
     const int idc = idx < n ? idx : n - 1;
     SellHead h;
     h.live = idx < n;
     h.row = rows != nullptr ? rows[idc] : idc;
+    h.mask = stencil ? (unsigned)slice_base[idc] : 0u;
+    h.slice = slice;
     h.base = (long)sp * 64 + (idx & 63);
     h.len = wave0 < n ? sl : 0;
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     *xr = __builtin_amdgcn_make_buffer_rsrc((void*)(x + (long)sb * NV), (short)0,
-                                            (int)(col16 ? 0xFFFFu * stride : kRange32),
-                                            kBufferWord3);
+                                            sell_range(cm, stride), kBufferWord3);
     return h;
 }
 
@@ -681,7 +733,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
     const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len, slice_base,
-                                                      rows, col16 != 0, p, &xr);
+                                                      rows, col16, p, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     const TA sv = St<TX>::template ld<TA>(s, h.row);
     TA wv = TA(0), pv[NV];
@@ -693,11 +745,8 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    if (col16)
-        sell_accum<unsigned short, TV, TX, NV>(acc, h.base, h.len, xr, stride,
-                                               (const unsigned short*)col, val, NV);
-    else
-        sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
+    sell_accum_cm<TV, TX, NV>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask, h.row,
+                              val, NV);
     double e[NV];
     TX out[NV];
 #pragma unroll
@@ -743,7 +792,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
     const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len, slice_base,
-                                                      rows, col16 != 0, z, &xr);
+                                                      rows, col16, z, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     const TA sv = St<TX>::template ld<TA>(s, h.row);
     TA wv = TA(0);
@@ -751,11 +800,8 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    if (col16)
-        sell_accum<unsigned short, TV, TX, NV>(acc, h.base, h.len, xr, stride,
-                                               (const unsigned short*)col, val, NV);
-    else
-        sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
+    sell_accum_cm<TV, TX, NV>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask, h.row,
+                              val, NV);
     // After the gathers: the state fields, the partials and the row's z, p and t,
     // in one round trip.
     StateLanes<WPC_KS_ACTIVE, (FS ? WPC_KS_BETA : WPC_KS_RZ_OLD)> st;
@@ -850,7 +896,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
     const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len,
-                                                       slice_base, rows, col16 != 0, t, &xr);
+                                                       slice_base, rows, col16, t, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     TA wv = TA(0), pv[NV];
     if (HW) {
@@ -861,11 +907,8 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    if (col16)
-        sell_accum<unsigned short, TV, TX, NV>(acc, h.base, h.len, xr, stride,
-                                               (const unsigned short*)col, val, NV);
-    else
-        sell_accum<int, TV, TX, NV>(acc, h.base, h.len, xr, stride, (const int*)col, val, NV);
+    sell_accum_cm<TV, TX, NV>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask, h.row,
+                              val, NV);
     TX alpha[NV];
     TX rv[NV], xv[NV], px[NV];
     if constexpr (FS) {
@@ -962,7 +1005,7 @@ __global__ __launch_bounds__(kStreamRows) void grouped_sell_kernel(
     if (t.n == 0) return;  // capacity padding of a captured launch
     const int blk = g - (int)t.block0;
     sell_body<TV, TX, TS, NV, HS, HD, DOT>(
-        blk, (int)t.n, t.slice_ptr, t.slice_len, t.slice_base, t.rows, t.col, (t.nv >> 8) & 1,
+        blk, (int)t.n, t.slice_ptr, t.slice_len, t.slice_base, t.rows, t.col, (t.nv >> 8) & 3,
         (const TV*)t.val, (const TS*)t.s,
         (const TX*)t.d, (const TX*)t.u, (const TX*)t.x, nullptr, (TX*)t.z, alpha, TA(0),
         (const TX*)t.w, t.partials, red, (int)(t.nv & 0xFF));
@@ -1029,7 +1072,7 @@ __global__ __launch_bounds__(1024) void fold_kernel(int P, int nvec,
 
 template <class TV, class TX, class TS, int NV>
 int stream_launch(int nb, const int* blocks, const int* slice_len, const int* slice_base,
-                  const int* rows, bool col16, int n, int lg, const int* off,
+                  const int* rows, int col16, int n, int lg, const int* off,
                   const int* col, const void* val, const void* s, const void* d, const void* u,
                   const void* x, const void* y, void* z, double alpha, double beta, const void* w,
                   double* partials, double* out, hipStream_t st) {

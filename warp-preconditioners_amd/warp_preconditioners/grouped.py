@@ -97,13 +97,15 @@ def _sliced(plan):
 
 
 def _col16(plan):
-    """The task's slice_base word (the per-slice column bases)."""
+    """The task's slice_base word (the per-slice column bases, or the row masks of
+    the stencil form)."""
     return _p(plan._sell[7])
 
 
 def _nv_word(plan, nv):
-    """The task's vector-count word: nv, plus 256 for 16-bit columns."""
-    return nv | (256 if plan._sell[6] else 0)
+    """The task's vector-count word: nv, plus 256 times the column form (1: 16-bit
+    offsets, 2: stencil)."""
+    return nv | (256 * int(plan._sell[6]))
 
 
 class GroupedCG:

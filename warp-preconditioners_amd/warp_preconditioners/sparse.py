@@ -154,6 +154,13 @@ class StreamPlan:
     SORT_GAIN = 1.0
     # Store the sliced columns as int16 differences from the row when they all fit.
     COL16 = True
+    # Column forms of the sliced copy (the products' col16 argument).
+    COL_I32, COL_U16, COL_STENCIL = 0, 1, 2
+    # Use the stencil form (wpc_sell_stencil: per-slice column offsets from the row
+    # and per-row masks, no column stream) when every slice's rows share at most 32
+    # offsets and the form stores no more entries than the plain slices (true of
+    # mesh-ordered Laplacians: 15 offsets on a Kuhn grid).
+    STENCIL = True
 
     def _row_order(self):
         """Row order of the sliced copy (device int32 permutation) or None."""
@@ -190,30 +197,68 @@ class StreamPlan:
             if ns:
                 nat.call("wpc_scan_inclusive", ns, nat.ptr(slice_len), nat.ptr(slice_ptr[1:]), s)
             total = int(slice_ptr[-1].item()) * 64
-            cols = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
-            smap = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
-            nat.call("wpc_sell_fill", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns),
-                     nat.ptr(rows), nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols),
-                     nat.ptr(smap), s)
-            col16 = False
-            # per-slice column bases of the 16-bit copy (zeros with int32 columns)
-            base = torch.zeros(max(ns, 1), dtype=torch.int32, device=dev)
-            if self.COL16 and total:
-                c16 = torch.empty(total, dtype=torch.int16, device=dev)
-                over = torch.zeros(1, dtype=torch.int32, device=dev)
-                nat.call("wpc_sell_col16", A.nrow, nat.ptr(A.offsets), nat.ptr(rows),
-                         nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols), nat.ptr(c16),
-                         nat.ptr(base), nat.ptr(over), s)
-                if int(over.item()) == 0:
-                    cols, col16 = c16, True
-                else:
-                    base.zero_()  # int32 columns: the descriptor starts at x
-            self._sell = (slice_ptr, slice_len, cols, smap, total, rows, col16, base)
+            stencil = self._stencil(ns, total) if rows is None and total else None
+            if stencil is not None:
+                self._sell = stencil
+                total = stencil[4]
+            else:
+                self._sell = self._plain(ns, rows, slice_ptr, slice_len, total)
         if self._sell_values is None:
             self._sell_values = torch.empty(max(self._sell[4], 1), dtype=A.values.dtype,
                                             device=A.device)
             self.refresh_values()
         return self._sell[0], self._sell[1], self._sell[2], self._sell_values
+
+    def _stencil(self, ns, total):
+        """The stencil form of the slices (wpc_sell_stencil) as a `_sell` tuple, or
+        None where a slice has more than 32 offsets or the form would store more
+        entries than the `total` of the plain slices."""
+        if not self.STENCIL:
+            return None
+        A = self.A
+        dev, s = A.device, nat.stream()
+        slice_len = torch.empty(ns, dtype=torch.int32, device=dev)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        nat.call("wpc_sell_stencil", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns), None,
+                 nat.ptr(slice_len), None, None, None, nat.ptr(status), s)
+        slice_ptr = torch.zeros(ns + 1, dtype=torch.int32, device=dev)
+        nat.call("wpc_scan_inclusive", ns, nat.ptr(slice_len), nat.ptr(slice_ptr[1:]), s)
+        stotal = int(slice_ptr[-1].item()) * 64
+        if int(status.item()) != 0 or stotal > total:
+            return None
+        deltas = torch.empty(ns * 32, dtype=torch.int32, device=dev)
+        masks = torch.empty(A.nrow, dtype=torch.int32, device=dev)
+        smap = torch.empty(max(stotal, 1), dtype=torch.int32, device=dev)
+        nat.call("wpc_sell_stencil", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns),
+                 nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(deltas), nat.ptr(masks),
+                 nat.ptr(smap), nat.ptr(status), s)
+        # (the row masks travel in the slice_base argument)
+        return (slice_ptr, slice_len, deltas, smap, stotal, None, self.COL_STENCIL, masks)
+
+    def _plain(self, ns, rows, slice_ptr, slice_len, total):
+        """The plain slices (int32 columns, or 16-bit offsets from per-slice bases
+        when every slice's columns span at most 0xFFFE) as a `_sell` tuple."""
+        A = self.A
+        dev, s = A.device, nat.stream()
+        cols = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        smap = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        nat.call("wpc_sell_fill", A.nrow, nat.ptr(A.offsets), nat.ptr(A.columns),
+                 nat.ptr(rows), nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols),
+                 nat.ptr(smap), s)
+        col16 = self.COL_I32
+        # per-slice column bases of the 16-bit copy (zeros with int32 columns)
+        base = torch.zeros(max(ns, 1), dtype=torch.int32, device=dev)
+        if self.COL16 and total:
+            c16 = torch.empty(total, dtype=torch.int16, device=dev)
+            over = torch.zeros(1, dtype=torch.int32, device=dev)
+            nat.call("wpc_sell_col16", A.nrow, nat.ptr(A.offsets), nat.ptr(rows),
+                     nat.ptr(slice_ptr), nat.ptr(slice_len), nat.ptr(cols), nat.ptr(c16),
+                     nat.ptr(base), nat.ptr(over), s)
+            if int(over.item()) == 0:
+                cols, col16 = c16, self.COL_U16
+            else:
+                base.zero_()  # int32 columns: the descriptor starts at x
+        return (slice_ptr, slice_len, cols, smap, total, rows, col16, base)
 
     # The sliced product gathers int32 columns through a buffer descriptor whose
     # byte offsets (column * nvec * element size) must stay below 2^31.
@@ -223,7 +268,7 @@ class StreamPlan:
         return self.sliced_fits(nvec, x.element_size())
 
     def sliced_fits(self, nvec, elem_bytes):
-        if self._sell is not None and self._sell[6]:
+        if self._sell is not None and self._sell[6] == self.COL_U16:
             return True  # 16-bit offsets: the range is per slice
         return self.A.ncol * nvec * elem_bytes <= self.DESCRIPTOR_RANGE
 
