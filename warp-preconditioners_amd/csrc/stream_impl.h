@@ -157,6 +157,11 @@ int sqlap_cg_entry_8(const SqlapCgArgs& a);
 // base, or stencil deltas (32 per slice) with per-row masks.
 constexpr int kColI32 = 0, kColU16 = 1, kColStencil = 2;
 constexpr int kStencilStride = 32;
+// Column forms a kernel instantiation handles (each its own gather loop, so a
+// kernel carries only the loops it can run): the plain forms (int32 or 16-bit,
+// chosen at run time), the stencil form alone, or all three (batched launches
+// whose systems may differ).
+constexpr int kCfPlain = 0, kCfStencil = 1, kCfAny = 2;
 
 }  // namespace wpc
 
@@ -535,15 +540,17 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
 
 // sell_accum in column form cm (slice: the wavefront's slice; mask, row: this
 // lane's stencil mask and row, read only in the stencil form).
-template <class TV, class TX, int NV>
+template <class TV, class TX, int NV, int CF>
 __device__ __forceinline__ void sell_accum_cm(typename AccOf<TX>::type* acc, long base, int len,
                                               __amdgpu_buffer_rsrc_t xr, unsigned stride, int cm,
                                               const void* __restrict__ col, int slice,
                                               unsigned mask, int row, const TV* __restrict__ val,
                                               int nv) {
-    if (cm == kColStencil) {
-        const StencilCol sc{(const int*)col + (long)slice * kStencilStride, mask, row};
-        sell_accum<StencilCol, TV, TX, NV>(acc, base, len, xr, stride, &sc, val, nv);
+    if (CF == kCfStencil || (CF == kCfAny && cm == kColStencil)) {
+        if constexpr (CF != kCfPlain) {
+            const StencilCol sc{(const int*)col + (long)slice * kStencilStride, mask, row};
+            sell_accum<StencilCol, TV, TX, NV>(acc, base, len, xr, stride, &sc, val, nv);
+        }
     } else if (cm == kColU16) {
         sell_accum<unsigned short, TV, TX, NV>(acc, base, len, xr, stride,
                                                (const unsigned short*)col, val, nv);
@@ -562,7 +569,7 @@ __device__ __forceinline__ int sell_range(int cm, unsigned stride) {
 // (kColU16: 16-bit offsets from slice_base[s]; kColI32: int32 columns and
 // slice_base holds zeros; kColStencil: deltas, slice_base holds the row masks,
 // natural row order).
-template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT, int CF>
 __device__ __forceinline__ void sell_body(
     int blk, int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
@@ -585,7 +592,7 @@ __device__ __forceinline__ void sell_body(
     const int slice = min(wave0, n - 1) >> 6;
     const int sp = slice_ptr[slice];
     const int sl = slice_len[slice];
-    const bool stencil = cm == kColStencil;
+    const bool stencil = CF == kCfStencil || (CF == kCfAny && cm == kColStencil);
     const int sb = stencil ? 0 : slice_base[slice];
     const int idc = live ? idx : n - 1;
     const unsigned mask = stencil ? (unsigned)slice_base[idc] : 0u;
@@ -611,7 +618,7 @@ __device__ __forceinline__ void sell_body(
 #pragma unroll
     for (int v = 0; v < NVM; ++v) acc[v] = TA(0);
     // (len is uniform: the entry loop's control is scalar)
-    sell_accum_cm<TV, TX, NV>(acc, base, len, xr, stride, cm, col, slice, mask, row, val, nv);
+    sell_accum_cm<TV, TX, NV, CF>(acc, base, len, xr, stride, cm, col, slice, mask, row, val, nv);
     // Every lane computes (a lane past n on row n-1's operands) and only the stores
     // are predicated: a branch around the loads would leave them pending at its
     // join, where the compiler then waits for every outstanding memory operation
@@ -657,7 +664,7 @@ __device__ __forceinline__ void sell_body(
     }
 }
 
-template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT>
+template <class TV, class TX, class TS, int NV, bool HS, bool HD, bool DOT, int CF>
 __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
@@ -667,7 +674,7 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
     typename AccOf<TX>::type alpha, typename AccOf<TX>::type beta, const TX* __restrict__ w,
     double* __restrict__ partials) {
     __shared__ double red[kStreamRows];
-    sell_body<TV, TX, TS, NV, HS, HD, DOT>(xcd_block(), n, slice_ptr, slice_len, slice_base, rows,
+    sell_body<TV, TX, TS, NV, HS, HD, DOT, CF>(xcd_block(), n, slice_ptr, slice_len, slice_base, rows,
                                            col, col16, val, s, d, u, x, y, z, alpha, beta, w,
                                            partials, red);
 }
@@ -692,7 +699,7 @@ struct SellHead {
     long base;
 };
 
-template <class TX, int NV, int NT = kStreamRows>
+template <class TX, int NV, int NT, int CF>
 __device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restrict__ slice_ptr,
                                               const int* __restrict__ slice_len,
                                               const int* __restrict__ slice_base,
@@ -703,7 +710,7 @@ __device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restr
     const int slice = min(wave0, n - 1) >> 6;
     const int sp = slice_ptr[slice];
     const int sl = slice_len[slice];
-    const bool stencil = cm == kColStencil;
+    const bool stencil = CF == kCfStencil || (CF == kCfAny && cm == kColStencil);
     const int sb = stencil ? 0 : slice_base[slice];
     const int idc = idx < n ? idx : n - 1;
     SellHead h;
@@ -721,7 +728,7 @@ __device__ __forceinline__ SellHead sell_head(int blk, int n, const int* __restr
 
 // Pass 1: t = s .* (L p) (s = 1/mass), partials of (L p)_i t_i + W_i p_i^2.
 
-template <class TV, class TX, int NV, bool HW>
+template <class TV, class TX, int NV, bool HW, int CF>
 __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
@@ -732,7 +739,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     __shared__ double red[kStreamRows];
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
-    const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len, slice_base,
+    const SellHead h = sell_head<TX, NV, kSqlapCgRows, CF>(blk, n, slice_ptr, slice_len, slice_base,
                                                       rows, col16, p, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     const TA sv = St<TX>::template ld<TA>(s, h.row);
@@ -745,7 +752,8 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    sell_accum_cm<TV, TX, NV>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask, h.row,
+    sell_accum_cm<TV, TX, NV, CF>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask,
+                                  h.row,
                               val, NV);
     double e[NV];
     TX out[NV];
@@ -777,7 +785,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_energy_kernel(
 // (L p')_i t'_i + W_i p'_i^2 with (L p')_i = t'_i / s_i.  FS (for partial counts
 // too large to fold in every workgroup): BETA and ACTIVE as the scalar fold
 // wpc_cg_beta left them in the state (no fold, no state writes).
-template <class TV, class TX, int NV, bool HW, bool FS>
+template <class TV, class TX, int NV, bool HW, bool FS, int CF>
 __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
@@ -791,7 +799,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     __shared__ double s_rz[8], s_rr[8];
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
-    const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len, slice_base,
+    const SellHead h = sell_head<TX, NV, kSqlapCgRows, CF>(blk, n, slice_ptr, slice_len, slice_base,
                                                       rows, col16, z, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     const TA sv = St<TX>::template ld<TA>(s, h.row);
@@ -800,7 +808,8 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    sell_accum_cm<TV, TX, NV>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask, h.row,
+    sell_accum_cm<TV, TX, NV, CF>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask,
+                                  h.row,
                               val, NV);
     // After the gathers: the state fields, the partials and the row's z, p and t,
     // in one round trip.
@@ -882,7 +891,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_direction_energy_kernel(
 // large to fold in every workgroup): ALPHA as the scalar fold wpc_cg_alpha left it
 // in the state (no fold, no state writes), and x += ALPHA p here, as the update
 // step would.
-template <class TV, class TX, int NV, bool HW, bool FS>
+template <class TV, class TX, int NV, bool HW, bool FS, int CF>
 __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     int n, const int* __restrict__ slice_ptr, const int* __restrict__ slice_len,
     const int* __restrict__ slice_base, const int* __restrict__ rows,
@@ -895,7 +904,7 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     __shared__ double sums[8];
     const int blk = xcd_block();
     __amdgpu_buffer_rsrc_t xr;
-    const SellHead h = sell_head<TX, NV, kSqlapCgRows>(blk, n, slice_ptr, slice_len,
+    const SellHead h = sell_head<TX, NV, kSqlapCgRows, CF>(blk, n, slice_ptr, slice_len,
                                                        slice_base, rows, col16, t, &xr);
     const unsigned stride = (unsigned)(NV * sizeof(TX));
     TA wv = TA(0), pv[NV];
@@ -907,7 +916,8 @@ __global__ __launch_bounds__(kSqlapCgRows) void sell_cg_update_kernel(
     TA acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = TA(0);
-    sell_accum_cm<TV, TX, NV>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask, h.row,
+    sell_accum_cm<TV, TX, NV, CF>(acc, h.base, h.len, xr, stride, col16, col, h.slice, h.mask,
+                                  h.row,
                               val, NV);
     TX alpha[NV];
     TX rv[NV], xv[NV], px[NV];
@@ -1004,7 +1014,7 @@ __global__ __launch_bounds__(kStreamRows) void grouped_sell_kernel(
     const SellTask& t = tasks[__builtin_amdgcn_readfirstlane(s_task)];
     if (t.n == 0) return;  // capacity padding of a captured launch
     const int blk = g - (int)t.block0;
-    sell_body<TV, TX, TS, NV, HS, HD, DOT>(
+    sell_body<TV, TX, TS, NV, HS, HD, DOT, kCfAny>(
         blk, (int)t.n, t.slice_ptr, t.slice_len, t.slice_base, t.rows, t.col, (t.nv >> 8) & 3,
         (const TV*)t.val, (const TS*)t.s,
         (const TX*)t.d, (const TX*)t.u, (const TX*)t.x, nullptr, (TX*)t.z, alpha, TA(0),
@@ -1082,8 +1092,15 @@ int stream_launch(int nb, const int* blocks, const int* slice_len, const int* sl
     if (grid <= 0 || (lg == -1 && nb <= 0) || (lg != -1 && n <= 0)) return WPC_OK;
 #define SL(HS, HD, DT)                                                                          \
     do {                                                                                        \
-        if (lg == -2)                                                                           \
-            hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT>), dim3(grid),           \
+        if (lg == -2 && col16 == kColStencil)                                                   \
+            hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT, kCfStencil>), dim3(grid), \
+                               dim3(kStreamRows), 0, st, n, blocks, slice_len, slice_base,      \
+                               rows, (const void*)col, (int)col16,                              \
+                               (const TV*)val, (const TS*)s, (const TX*)d, (const TX*)u,        \
+                               (const TX*)x, (const TX*)y, (TX*)z, (TA)alpha, (TA)beta,         \
+                               (const TX*)w, partials);                                         \
+        else if (lg == -2)                                                                      \
+            hipLaunchKernelGGL((sell_kernel<TV, TX, TS, NV, HS, HD, DT, kCfPlain>), dim3(grid), \
                                dim3(kStreamRows), 0, st, n, blocks, slice_len, slice_base,      \
                                rows, (const void*)col, (int)col16,                              \
                                (const TV*)val, (const TS*)s, (const TX*)d, (const TX*)u,        \
@@ -1157,31 +1174,39 @@ int stream_for_nv(const StreamArgs& a) {
 template <class TV, class TX, int NV>
 int sqlap_cg_launch(const SqlapCgArgs& a) {
     const int grid = (int)blocks_for(a.n, kSqlapCgRows);  // each pass's partial count
-#define SC(HW, FS)                                                                              \
+#define SC(HW, FS, CF)                                                                          \
     do {                                                                                        \
         if ((a.passes & 1) && (a.passes & 8))                                                   \
-        hipLaunchKernelGGL((sell_direction_energy_kernel<TV, TX, NV, HW, FS>), dim3(grid),      \
+        hipLaunchKernelGGL((sell_direction_energy_kernel<TV, TX, NV, HW, FS, CF>), dim3(grid),  \
                            dim3(kSqlapCgRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
                            a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
                            (const TX*)a.s, (const TX*)a.wm, (const TX*)a.z, (TX*)a.p, (TX*)a.t, \
                            a.P_rz, a.part_rz, grid, a.part_rr, a.state, a.part_e);              \
         else if (a.passes & 1)                                                                  \
-        hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW>), dim3(grid), dim3(kSqlapCgRows), \
+        hipLaunchKernelGGL((sell_energy_kernel<TV, TX, NV, HW, CF>), dim3(grid),                \
+                           dim3(kSqlapCgRows),                                                  \
                            0, a.st, a.n, a.slice_ptr, a.slice_len, a.slice_base, a.rows, a.col, \
                            (int)a.col16, (const TV*)a.val, (const TX*)a.s, (const TX*)a.wm,     \
                            (const TX*)a.p, (TX*)a.t, a.part_e);                                 \
         if (a.passes & 2)                                                                       \
-        hipLaunchKernelGGL((sell_cg_update_kernel<TV, TX, NV, HW, FS>), dim3(grid),             \
+        hipLaunchKernelGGL((sell_cg_update_kernel<TV, TX, NV, HW, FS, CF>), dim3(grid),         \
                            dim3(kSqlapCgRows), 0, a.st, a.n, a.slice_ptr, a.slice_len,           \
                            a.slice_base, a.rows, a.col, (int)a.col16, (const TV*)a.val,         \
                            (const TX*)a.wm, (const TX*)a.p, (const TX*)a.t, (TX*)a.r, grid,     \
                            a.part_e, a.state, a.part_rr, (TX*)a.x);                             \
     } while (0)
     const bool fs = (a.passes & 4) != 0;
-    if (a.wm != nullptr && fs) SC(true, true);
-    else if (a.wm != nullptr) SC(true, false);
-    else if (fs) SC(false, true);
-    else SC(false, false);
+    if (a.col16 == kColStencil) {
+        if (a.wm != nullptr && fs) SC(true, true, kCfStencil);
+        else if (a.wm != nullptr) SC(true, false, kCfStencil);
+        else if (fs) SC(false, true, kCfStencil);
+        else SC(false, false, kCfStencil);
+    } else {
+        if (a.wm != nullptr && fs) SC(true, true, kCfPlain);
+        else if (a.wm != nullptr) SC(true, false, kCfPlain);
+        else if (fs) SC(false, true, kCfPlain);
+        else SC(false, false, kCfPlain);
+    }
 #undef SC
     return launch_status();
 }
