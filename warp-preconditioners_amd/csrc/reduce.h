@@ -198,20 +198,23 @@ struct FoldRegs {
     static constexpr int K = (kFoldStaged / NV + NT - 1) / NT;
     double val[K][NV];
 
-    // Branch-free loads through a buffer descriptor whose range ends at row P: a
-    // row past P reads 0 without a memory access.  (With the loads under a per-row
-    // condition the compiler moved the first row's add into its branch and waited
-    // there, so the remaining loads, and every load issued after them, went out
-    // one memory round trip late.)
+    // Loads through a buffer descriptor whose range ends at row P: a row past P
+    // reads 0 without a memory access.  The thread's rows within P go out at once,
+    // branch-free per row (with the loads under a per-row condition the compiler
+    // moved the first row's add into its branch and waited there, so the remaining
+    // loads, and every load issued after them, went out one memory round trip
+    // late); a whole group of rows past P (k NT >= P, uniform) is not issued.
     __device__ __forceinline__ void load(const double* __restrict__ partials, int P) {
         const __amdgpu_buffer_rsrc_t pr = __builtin_amdgcn_make_buffer_rsrc(
             (void*)partials, (short)0, P > 0 ? P * NV * (int)sizeof(double) : 0, kBufferWord3);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const unsigned row = threadIdx.x + k * NT;
+            const bool issue = k == 0 || k * NT < P;
 #pragma unroll
             for (int v = 0; v < NV; ++v)
-                val[k][v] = buffer_ld<double>(pr, (row * NV + v) * (unsigned)sizeof(double));
+                val[k][v] = issue ? buffer_ld<double>(pr, (row * NV + v) * (unsigned)sizeof(double))
+                                  : 0.0;
         }
     }
 

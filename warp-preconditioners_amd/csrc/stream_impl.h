@@ -428,12 +428,6 @@ __device__ __forceinline__ unsigned sell_off<unsigned short>(const unsigned shor
                                                              int, unsigned stride) {
     return (unsigned)col[p] * stride;
 }
-template <>
-__device__ __forceinline__ unsigned sell_off<StencilCol>(const StencilCol* col, long, int kk,
-                                                         unsigned stride) {
-    const int d = col->delta[kk];
-    return (col->mask >> kk) & 1u ? (unsigned)(col->row + d) * stride : 0u - stride;
-}
 
 
 // NV > 0: NV interleaved vectors; NV == 0: nv_rt (<= 8) vectors, the count a
@@ -460,12 +454,32 @@ __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long b
     constexpr int NVM = kNvMax<NV>;
     unsigned o[G];
     TA a[G];
+    if constexpr (std::is_same_v<TI, StencilCol>) {
+        // The group's deltas are loaded first and pinned in scalar registers:
+        // left to the compiler, each delta load sank into a branch on the row's
+        // mask bit and was waited for there, one scalar round trip per entry.
+        int dl[G];
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-        const int e = FULL || j < m ? j : m - 1;
-        const long p = base + (long)(k + e) * 64;
-        o[j] = sell_off<TI>(col, p, k + e, stride);
-        a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
+        for (int j = 0; j < G; ++j)  // (uniform; in batched launches the compiler cannot tell)
+            dl[j] = __builtin_amdgcn_readfirstlane(col->delta[k + (FULL || j < m ? j : m - 1)]);
+#pragma unroll
+        for (int j = 0; j < G; ++j) asm volatile("" : "+s"(dl[j]));
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int e = FULL || j < m ? j : m - 1;
+            const long p = base + (long)(k + e) * 64;
+            const unsigned on = (unsigned)(col->row + dl[j]) * stride;
+            o[j] = (col->mask >> (k + e)) & 1u ? on : 0u - stride;
+            a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int e = FULL || j < m ? j : m - 1;
+            const long p = base + (long)(k + e) * 64;
+            o[j] = sell_off<TI>(col, p, k + e, stride);
+            a[j] = lds_val<TA, TV>(raw_val<TV>(val, p));
+        }
     }
 #pragma unroll
     for (int j = 0; j < G; ++j) {
