@@ -174,31 +174,14 @@ __device__ void fold_reduce(const double* acc, int nvec, double* out_s) {
 
 // Fixed-order fold of partials[0..P) for all nvec (<= 8) vectors at once by a
 // workgroup of NT threads; results land in out_s[0..nvec) (shared memory).
-// Thread t sums rows t, t + NT, t + 2 NT, ... in that order, loading four rows
-// ahead of their additions (one memory round trip per four rows, not per row).
+// Thread t sums rows t, t + NT, t + 2 NT, ... in that order.
 template <int NT>
 __device__ void fold_all(const double* __restrict__ partials, int P, int nvec, double* out_s) {
     double acc[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) acc[v] = 0.0;
-    int b = threadIdx.x;
-    for (; b + 3 * NT < P; b += 4 * NT) {
-        double t[4][8];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int v = 0; v < 8; ++v)
-                if (v < nvec) t[q][v] = partials[(long)(b + q * NT) * nvec + v];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int v = 0; v < 8; ++v)
-                if (v < nvec) acc[v] += t[q][v];
-    }
-    for (; b < P; b += NT)
-#pragma unroll
-        for (int v = 0; v < 8; ++v)
-            if (v < nvec) acc[v] += partials[(long)b * nvec + v];
+    for (int b = threadIdx.x; b < P; b += NT)
+        for (int v = 0; v < nvec; ++v) acc[v] += partials[(long)b * nvec + v];
     fold_reduce<NT>(acc, nvec, out_s);
 }
 
