@@ -703,7 +703,8 @@ __global__ __launch_bounds__(kStreamRows) void sell_kernel(
 // second pass does and applies r -= ALPHA Ap instead of storing Ap, which no later
 // step reads.  The update launch and Ap's round trip through HBM leave the
 // iteration.  State fields as in the update step: reads RZ, RR, TARGET; workgroup 0
-// writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA; x is left to the direction step.
+// writes RZ_OLD (= RZ), DENOM, ACTIVE, ALPHA; given x, the second pass also
+// advances x += ALPHA p (it reads p anyway).
 
 // Wavefront slice header and gather descriptor of the sliced mapping (as sell_body).
 struct SellHead {
