@@ -438,13 +438,15 @@ template <int NV> constexpr int kNvMax = NV > 0 ? NV : 8;
 // together, then its gathers.  A slice's length (uniform across the wavefront) is
 // covered by groups of 8 until at most 12 entries remain; a remainder of 9-12 is
 // one masked group of 12 (where 12 entries' gathers fit in registers), then 8 a
-// group of 8, 6-7 a masked group of 8, 3 a masked group of 4, otherwise groups of
+// group of 8, 5-7 a masked group of 8, 3 a masked group of 4, otherwise groups of
 // 4, 2 and 1 (with many vectors, groups of 4 and the remainder 3 masked, 2 or 1:
 // kSellMain).  A masked group's entries past the slice re-read its last entry (a
 // cache hit; nothing past the slice is read) and their products are discarded by
 // a select, so the sums are those of storage order.  Masking only where it wastes
-// at most two gathers keeps the gathers (which bound these passes) close to the
-// stored entries.
+// at most three gathers keeps the gathers (which bound these passes) close to the
+// stored entries; a remainder of 5 as one masked group (not 4 + 1) saves a round
+// trip, which the slowest wavefronts of a launch (every wavefront of a pass is
+// resident at once) set the pass's time by.
 template <class TI, class TV, class TX, int NV, int G, bool FULL = true>
 __device__ __forceinline__ void sell_group(typename AccOf<TX>::type* acc, long base, int k,
                                            __amdgpu_buffer_rsrc_t xr, unsigned stride,
@@ -531,7 +533,7 @@ __device__ __forceinline__ void sell_accum(typename AccOf<TX>::type* acc, long b
         sell_group<TI, TV, TX, NV, 8>(acc, base, k, xr, stride, col, val, nv);
         return;
     }
-    if (r >= 6) {
+    if (r >= 5) {
         sell_group<TI, TV, TX, NV, 8, false>(acc, base, k, xr, stride, col, val, nv, r);
         return;
     }
