@@ -235,8 +235,7 @@ __global__ __launch_bounds__(1024) void cg_beta_kernel(
     int nvec, int P_rz, const double* __restrict__ part_rz, int P_rr,
     const double* __restrict__ part_rr, double* __restrict__ state) {
     __shared__ double s_rz[8], s_rr[8];
-    fold_all<1024>(part_rz, P_rz, nvec, s_rz);
-    fold_all<1024>(part_rr, P_rr, nvec, s_rr);
+    fold_two<1024>(part_rz, P_rz, part_rr, P_rr, nvec, s_rz, s_rr);
     const int v = threadIdx.x;
     {
         if (v < nvec) {

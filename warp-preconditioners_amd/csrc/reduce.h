@@ -185,6 +185,46 @@ __device__ void fold_all(const double* __restrict__ partials, int P, int nvec, d
     fold_reduce<NT>(acc, nvec, out_s);
 }
 
+// fold_all of two partial sets at once (results in out_a, out_b): each thread
+// adds its rows of both in fold_all's order, with the loads of a row of each set
+// issued together, and one cross-wavefront pass reduces all 2 nvec sums.
+template <int NT>
+__device__ void fold_two(const double* __restrict__ pa, int Pa, const double* __restrict__ pb,
+                         int Pb, int nvec, double* out_a, double* out_b) {
+    double acc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0;
+    const int P = Pa > Pb ? Pa : Pb;
+    for (int b = threadIdx.x; b < P; b += NT) {
+        double ta[8], tb[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            ta[v] = v < nvec && b < Pa ? pa[(long)b * nvec + v] : 0.0;
+            tb[v] = v < nvec && b < Pb ? pb[(long)b * nvec + v] : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            acc[v] += ta[v];  // (a row past P adds +0 to a sum that starts at +0)
+            acc[8 + v] += tb[v];
+        }
+    }
+    __shared__ double red[NT / kWave][16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        if ((v & 7) >= nvec) continue;
+        const double x = wave_sum(acc[v]);
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave][v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * nvec) {
+        const int v = threadIdx.x < nvec ? threadIdx.x : 8 + threadIdx.x - nvec;
+        double t = 0.0;
+        for (int w = 0; w < NT / kWave; ++w) t += red[w][v];
+        (threadIdx.x < nvec ? out_a[threadIdx.x] : out_b[threadIdx.x - nvec]) = t;
+    }
+    __syncthreads();
+}
+
 // Partial counts P * NV up to this many are staged in registers by FoldRegs.
 constexpr int kFoldStaged = 4096;
 
