@@ -513,9 +513,10 @@ def test_stencil_form_matches_oracle_and_streamed_bitwise(oracle, nvec):
     np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-12)
 
 
-def test_stencil_form_only_where_it_stores_no_more(monkeypatch):
-    # Slices whose rows share more than 32 offsets, or whose union of offsets
-    # exceeds their longest row, keep the plain slices; the switch turns it off.
+def test_stencil_form_only_where_slices_share_few_offsets(monkeypatch):
+    # Slices whose rows share more than 32 offsets, or whose unions of offsets
+    # exceed their longest rows by more than 1/64 in all, keep the plain slices;
+    # the switch turns the form off.
     from warp_preconditioners.sparse import StreamPlan
 
     rng = np.random.default_rng(5)
@@ -530,3 +531,30 @@ def test_stencil_form_only_where_it_stores_no_more(monkeypatch):
     plan = StreamPlan(upload(g), lanes=StreamPlan.SLICED)
     plan._sliced()
     assert int(plan._sell[6]) == StreamPlan.COL_U16
+
+
+@pytest.mark.parametrize("side", [33, 40])
+def test_stencil_form_on_served_kuhn_grids(side, monkeypatch):
+    # The served meshes' K (device assembly, Kuhn grids of any side): slices that
+    # span grid lines whose boundary rows miss different neighbours store a few
+    # more entries in the stencil form (0.1-0.3%), which is admitted; its products
+    # equal the plain slices' bit for bit.
+    from warp_preconditioners import assembly
+    from warp_preconditioners.sparse import StreamPlan
+
+    v, t = assembly.jittered_grid(side, 3, dev())
+    K, _ = assembly.p1_laplacian(v, t)
+    n = K.nrow
+    rng = np.random.default_rng(side)
+    X = torch.tensor(rng.normal(size=n * 3), device=dev())
+    outs = []
+    for stencil in (True, False):
+        monkeypatch.setattr(StreamPlan, "STENCIL", stencil)
+        plan = StreamPlan(K, lanes=StreamPlan.SLICED)
+        Z = torch.empty_like(X)
+        dots = torch.zeros(3, dtype=torch.float64, device=dev())
+        plan.run(X, None, Z, 1.0, 0.0, 3, w=X, out=dots)
+        assert (int(plan._sell[6]) == StreamPlan.COL_STENCIL) == stencil
+        outs.append((host(Z), host(dots)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
